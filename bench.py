@@ -1,0 +1,195 @@
+"""Benchmark of the HQRRP hot path on one MI355X.
+
+Workload: the plain all-FP64 HQRRP of a 16384 x 16384 matrix
+A = U diag(sigma) V' (U, V Haar-orthogonal, sigma_i = 10^(-12 i/n)),
+nb = 64, pp = 10, panel pivoting on, Omega = the reference's stream.
+A step is one complete factorisation (NoFLA_HQRRP_WY_blk_var4 semantics,
+device-pointer entry) of a fresh copy of A that is already resident in HBM;
+the device-to-device copy that restores A is inside the timed step.
+
+metric: QRCP GFLOP/s with the flop count 2mn^2 - 2/3 n^3 (m >= n).
+
+Also reported:
+  roofline      -- the phase with the largest summed kernel time during the
+                   timed steps, its algorithmic flops (or bytes) per launch
+                   divided by its average launch duration, both from HIP
+                   events recorded on the library's stream;
+  cpu_baseline  -- the oracle (a CPU restatement of the reference driver
+                   making the same LAPACK calls, OpenBLAS) timed on the host
+                   cores on the first --cpu-blocks blocks of the same matrix,
+                   rated with the flops of those columns.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
+PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
+
+# Phase -> (bound, kernel symbol prefix in rocprof)
+PHASE_BOUND = {
+    "sketch_gemm": "mfma",
+    "sketch_qrcp": "hbm",
+    "gather": "hbm",
+    "panel": "hbm",
+    "panel_post": "hbm",
+    "trail_vtc": "mfma",
+    "trail_tw": "mfma",
+    "trail_update": "hbm",
+    "downdate": "hbm",
+}
+
+
+def qr_flops(m, n):
+    if m >= n:
+        return 2.0 * m * n * n - 2.0 / 3.0 * n ** 3
+    return 2.0 * n * m * m - 2.0 / 3.0 * m ** 3
+
+
+def partial_qr_flops(m, n, k):
+    """Householder QR flops of the first k columns of an m x n matrix."""
+    return 4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 / 3.0 * k ** 3
+
+
+def cpu_baseline(A_host, nb, pp, blocks, threads):
+    import numpy as np
+
+    import oracle
+
+    oracle.set_threads(threads)
+    m, n = A_host.shape
+    t0 = time.perf_counter()
+    oracle.hqrrp(A_host, nb, pp, 1, max_blocks=blocks)
+    dt = time.perf_counter() - t0
+    k = min(blocks * nb, min(m, n))
+    fl = partial_qr_flops(m, n, k)
+    return {"value": round(fl / dt / 1e9, 2), "unit": "GFLOP/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"oracle restatement of NoFLA_HQRRP_WY_blk_var4 "
+                       f"(OpenBLAS, {threads} threads) on the first {k} "
+                       f"columns ({blocks} blocks) of the same {m}x{n} "
+                       f"matrix: {dt:.2f} s for {fl:.3e} flops")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=16384)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--nb", type=int, default=64)
+    ap.add_argument("--pp", type=int, default=10)
+    ap.add_argument("--cpu-blocks", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: HQRRP is single-device in this "
+              "build (multi-GPU sharding is a design note only); run with "
+              "--gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import hqrrp_amd
+    from hqrrp_amd import matgen
+
+    m, n, nb, pp = args.m, args.n, args.nb, args.pp
+    dev = "cuda:0"
+    print(f"# generating graded {m}x{n} input on the GPU", file=sys.stderr,
+          flush=True)
+    A0 = matgen.graded_device(m, n, seed=0, device=dev)
+    A = torch.empty_like(A0)
+    jp = torch.empty(n, dtype=torch.int32, device=dev)
+    ident = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
+    tau = torch.empty(min(m, n), dtype=torch.float64, device=dev)
+    h = hqrrp_amd.Handle(0)
+    h.use_stream(torch.cuda.current_stream())
+
+    def step():
+        A.copy_(A0)
+        jp.copy_(ident)
+        hqrrp_amd.hqrrp_device(h, A, jp, tau, m, n, nb=nb, pp=pp,
+                               panel_pivoting=1)
+
+    for i in range(args.warmup):
+        step()
+        h.synchronize()
+        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+
+    if not args.no_profile:
+        h.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    h.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    print(f"# timed {args.steps} steps: {ms:.1f} ms/step", file=sys.stderr,
+          flush=True)
+    prof = h.profile_read() if not args.no_profile else {}
+    h.profile(False)
+
+    flops = qr_flops(m, n)
+    value = flops / (ms / 1e3) / 1e9
+
+    roof = None
+    phase_ms = {}
+    if prof:
+        phase_ms = {k: round(v[0] / args.steps, 3) for k, v in prof.items()}
+        name = max(prof, key=lambda k: prof[k][0])
+        t_ms, launches, fl, by = prof[name]
+        bound = PHASE_BOUND[name]
+        avg_s = t_ms / 1e3 / max(1, launches)
+        if bound == "mfma":
+            ach = fl / max(1, launches) / avg_s / 1e12
+            roof = {"bound": "mfma", "kernel": name, "achieved": round(ach, 3),
+                    "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                    "frac": round(ach * 1e12 / PEAK_FP64_MFMA, 4),
+                    "traffic": None, "launches": launches,
+                    "avg_launch_us": round(avg_s * 1e6, 2)}
+        else:
+            ach = by / max(1, launches) / avg_s / 1e9
+            roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
+                    "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": round(ach * 1e9 / PEAK_HBM, 4), "traffic": None,
+                    "launches": launches,
+                    "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"# cpu baseline on {threads} threads", file=sys.stderr,
+              flush=True)
+        A_host = np.asfortranarray(A0.cpu().numpy().T)
+        cpu = cpu_baseline(A_host, nb, pp, args.cpu_blocks, threads)
+
+    h.close()
+    line = {
+        "metric": "QRCP GFLOP/s (2mn^2-2/3n^3), plain FP64 sketch",
+        "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"HQRRP plain FP64 {m}x{n} graded "
+                               f"(Haar U,V, sigma_i=10^(-12i/n))",
+                   "m": m, "n": n, "nb": nb, "pp": pp, "panel_pivoting": 1,
+                   "sketch": "fp64", "omega": "reference stream seed 12"},
+        "roofline": roof, "cpu_baseline": cpu, "phase_ms": phase_ms,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,13 @@
+"""MI355X-native HQRRP (Householder QR with randomized column pivoting).
+
+Python face of libhqrrp_amd.so.  It mirrors the reference's two entry points
+(``dgeqp4`` and ``NoFLA_HQRRP_WY_blk_var4``, host arrays) and exposes the
+device-pointer interface on torch tensors.  All numerical work runs in the
+library's HIP kernels; nothing here computes a factorization.
+"""
+from .api import (Handle, dgeqp4, hqrrp, hqrrp_device, reference_omega,
+                  to_colmajor, from_colmajor)
+from ._lib import HQRRPError, Options, lib
+
+__all__ = ["Handle", "dgeqp4", "hqrrp", "hqrrp_device", "reference_omega",
+           "to_colmajor", "from_colmajor", "HQRRPError", "Options", "lib"]

@@ -1,0 +1,161 @@
+"""Reference-shaped API over the C ABI.
+
+Host entry points take numpy arrays (column-major, float64) exactly like the
+reference's C functions take host buffers:
+
+* :func:`dgeqp4` -- LAPACK ``dgeqp3``-compatible driver
+  (reference ``NoFLA_HQRRP_WY_blk_var4.c:127-300``);
+* :func:`hqrrp` -- ``NoFLA_HQRRP_WY_blk_var4(m, n, A, lda, jpvt, tau, nb_alg,
+  pp, panel_pivoting)`` (reference ``NoFLA_HQRRP_WY_blk_var4.c:303-571``).
+
+Device entry points take torch tensors on the GPU.  A column-major m x n
+matrix is held as a contiguous torch tensor of shape (n, m) (its transpose in
+row-major order), see :func:`to_colmajor`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Options, check, lib
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def reference_omega(r: int, m: int, seed: int = 12) -> np.ndarray:
+    """The reference's Gaussian sketch matrix (r x m, column-major)."""
+    G = np.zeros((r, m), dtype=np.float64, order="F")
+    lib().hqrrp_reference_omega(r, m, _ptr(G), seed)
+    return G
+
+
+def dgeqp4(A, jpvt=None, lwork=None):
+    """QR with column pivoting of a copy of A (dgeqp3 calling sequence).
+
+    Returns (A_factored, jpvt (1-based), tau, info, work[0]).
+    """
+    A = np.asfortranarray(np.array(A, dtype=np.float64, copy=True))
+    m, n = A.shape
+    jp = (np.zeros(n, dtype=np.int32) if jpvt is None
+          else np.array(jpvt, dtype=np.int32, copy=True))
+    tau = np.zeros(max(1, min(m, n)))
+    lw = max(1, 128 * n) if lwork is None else int(lwork)
+    work = np.zeros(max(1, lw))
+    info = C.c_int(0)
+    lib().dgeqp4(C.byref(C.c_int(m)), C.byref(C.c_int(n)), _ptr(A),
+                 C.byref(C.c_int(max(1, m))), _ptr(jp), _ptr(tau), _ptr(work),
+                 C.byref(C.c_int(lw)), C.byref(info))
+    return A, jp, tau[:min(m, n)], info.value, work[0]
+
+
+def hqrrp(A, nb_alg=64, pp=10, panel_pivoting=1, jpvt=None):
+    """NoFLA_HQRRP_WY_blk_var4 on a copy of A; returns (A, jpvt, tau)."""
+    A = np.asfortranarray(np.array(A, dtype=np.float64, copy=True))
+    m, n = A.shape
+    jp = (np.arange(1, n + 1, dtype=np.int32) if jpvt is None
+          else np.array(jpvt, dtype=np.int32, copy=True))
+    tau = np.zeros(max(1, min(m, n)))
+    rc = lib().NoFLA_HQRRP_WY_blk_var4(m, n, _ptr(A), max(1, m), _ptr(jp),
+                                       _ptr(tau), nb_alg, pp, panel_pivoting)
+    check(rc, "NoFLA_HQRRP_WY_blk_var4")
+    return A, jp, tau[:min(m, n)]
+
+
+# ------------------------------------------------------------ device side
+def to_colmajor(A, device="cuda"):
+    """numpy/torch m x n -> contiguous torch (n, m) = column-major m x n."""
+    import torch
+
+    t = torch.as_tensor(np.asarray(A, dtype=np.float64) if not isinstance(
+        A, torch.Tensor) else A, dtype=torch.float64)
+    return t.t().contiguous().to(device)
+
+
+def from_colmajor(T):
+    """contiguous torch (n, m) -> numpy m x n (Fortran order)."""
+    return np.asfortranarray(T.detach().cpu().numpy().T)
+
+
+class Handle:
+    """Owns a library handle (device workspace, barrier words, a stream)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().hqrrp_create(C.byref(self._h), device), "hqrrp_create")
+        self.device = device
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def use_stream(self, stream) -> None:
+        """Run this handle's work on a torch.cuda.Stream (None: own stream)."""
+        check(lib().hqrrp_set_stream(
+            self._h, None if stream is None else C.c_void_p(stream.cuda_stream)),
+            "hqrrp_set_stream")
+
+    def synchronize(self) -> None:
+        check(lib().hqrrp_synchronize(self._h), "hqrrp_synchronize")
+
+    def profile(self, on: bool = True) -> None:
+        """Start (and reset) or stop per-phase event timing."""
+        check(lib().hqrrp_profile_enable(self._h, 1 if on else 0),
+              "hqrrp_profile_enable")
+
+    def profile_read(self) -> dict:
+        """{phase: (ms, launches, algorithmic flops, algorithmic bytes)}."""
+        L = lib()
+        n = L.hqrrp_phase_count()
+        ms = (C.c_double * n)()
+        la = (C.c_longlong * n)()
+        fl = (C.c_double * n)()
+        by = (C.c_double * n)()
+        check(L.hqrrp_profile_read(self._h, n, ms, la, fl, by),
+              "hqrrp_profile_read")
+        return {L.hqrrp_phase_name(i).decode(): (ms[i], la[i], fl[i], by[i])
+                for i in range(n)}
+
+    def close(self) -> None:
+        if self._h:
+            lib().hqrrp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def options(nb=64, pp=10, panel_pivoting=1, seed=12, omega=None,
+            max_rank=0, sketch_prec=0) -> Options:
+    o = Options()
+    lib().hqrrp_default_options(C.byref(o))
+    o.nb, o.pp, o.panel_pivoting, o.seed = nb, pp, panel_pivoting, seed
+    o.max_rank = max_rank
+    o.sketch_prec = sketch_prec
+    if omega is not None:
+        o.omega_kind = 1
+        o.omega = C.c_void_p(omega.data_ptr())
+    return o
+
+
+def hqrrp_device(handle: Handle, At, jpvt_t, tau_t, m: int, n: int,
+                 lda: int | None = None, **kw) -> None:
+    """Factor the column-major m x n matrix held by torch tensor At in place.
+
+    jpvt_t: int32 tensor (n), permuted in place; tau_t: float64 (min(m,n)).
+    Stream-ordered on the handle's stream; call handle.synchronize().
+    """
+    o = options(**kw)
+    rc = lib().hqrrp_factor_dev(handle.ptr, m, n, C.c_void_p(At.data_ptr()),
+                                m if lda is None else lda,
+                                C.c_void_p(jpvt_t.data_ptr()),
+                                C.c_void_p(tau_t.data_ptr()), C.byref(o))
+    check(rc, "hqrrp_factor_dev")

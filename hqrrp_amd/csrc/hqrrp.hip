@@ -1,0 +1,810 @@
+// hqrrp.hip -- host orchestration of HQRRP on one MI355X and the C ABI.
+//
+// The block loop restates NoFLA_HQRRP_WY_blk_var4 (reference
+// lapack_compatible_sources/NoFLA_HQRRP_WY_blk_var4.c:303-571): sketch
+// Y = Omega*A, then per block of nb columns
+//   sketch pivots (k_sketch) -> gather of A/Y/jpvt columns (k_misc)
+//   -> panel QRCP + T (k_panel) -> trailing dlarfb (k_gemm)
+//   -> sketch downdate (k_gemm).
+// Everything is stream-ordered; pivots never leave the device and the host
+// never waits inside the loop (all shapes depend only on j).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hqrrp_amd.h"
+#include "common.h"
+#include "kernels.h"
+
+#define HQ_TRY(expr)                                                  \
+  do {                                                                \
+    hipError_t e_ = (expr);                                           \
+    if (e_ != hipSuccess) {                                           \
+      fprintf(stderr, "hqrrp: %s failed: %s (%s:%d)\n", #expr,        \
+              hipGetErrorString(e_), __FILE__, __LINE__);             \
+      return HQRRP_ERR_HIP;                                           \
+    }                                                                 \
+  } while (0)
+
+#define HQ_RET(expr)            \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != HQRRP_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (b.bytes >= bytes) return HQRRP_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    fprintf(stderr, "hqrrp: device allocation of %zu bytes failed\n", bytes);
+    return HQRRP_ERR_ALLOC;
+  }
+  b.bytes = bytes;
+  return HQRRP_OK;
+}
+
+// Reentrant restatement of glibc rand(): additive lagged-Fibonacci generator
+// r[i] = r[i-3] + r[i-31] (mod 2^32) seeded by the minimal-standard LCG,
+// first 310 outputs discarded, outputs shifted right by one bit.
+struct RefRand {
+  uint32_t r[31];
+  int f = 3, b = 0;
+  explicit RefRand(unsigned seed) {
+    int32_t word = (int32_t)(seed == 0 ? 1 : seed);
+    r[0] = (uint32_t)word;
+    for (int i = 1; i < 31; ++i) {
+      int32_t hi = word / 127773, lo = word % 127773;
+      word = 16807 * lo - 2836 * hi;
+      if (word < 0) word += 2147483647;
+      r[i] = (uint32_t)word;
+    }
+    for (int i = 0; i < 310; ++i) next();
+  }
+  int next() {
+    r[f] += r[b];
+    int out = (int)((r[f] >> 1) & 0x7fffffff);
+    f = (f + 1) % 31;
+    b = (b + 1) % 31;
+    return out;
+  }
+};
+
+}  // namespace
+
+extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
+  // Marsaglia polar method exactly as NoFLA_Normal_random_number
+  // (NoFLA_HQRRP_WY_blk_var4.c:593-614): pairs of uniforms on (-1,1) with
+  // 0 < a < 1, returning c1*f and then the cached c2*f, column-major fill.
+  RefRand rng(seed);
+  const double rmax = 2147483647.0;
+  double cached = 0.0;
+  bool have = false;
+  long total = (long)r * m;
+  for (long idx = 0; idx < total; ++idx) {
+    if (have) {
+      have = false;
+      G[idx] = 0.0 + 1.0 * cached;
+      continue;
+    }
+    double c1, c2, a;
+    do {
+      c1 = -1.0 + 2.0 * ((double)rng.next() / rmax);
+      c2 = -1.0 + 2.0 * ((double)rng.next() / rmax);
+      a = c1 * c1 + c2 * c2;
+    } while ((a == 0) || (a >= 1));
+    double f = sqrt((-2 * log(a)) / a);
+    cached = c2 * f;
+    have = true;
+    G[idx] = 0.0 + 1.0 * (c1 * f);
+  }
+}
+
+// Phases timed by the optional profiler (hqrrp_profile_*).
+enum {
+  PH_SKETCH_GEMM = 0,  // Y = Omega*A
+  PH_SKETCH_QRCP,      // pivot selection on the sketch
+  PH_GATHER,           // A/Y/jpvt column gathers
+  PH_PANEL,            // panel QRCP kernel
+  PH_PANEL_POST,       // T factor + permutation of the rows above the panel
+  PH_TRAIL_VTC,        // trailing M = V'C (+ split-K reduction)
+  PH_TRAIL_TW,         // W = T'M
+  PH_TRAIL_UPD,        // trailing C -= V W
+  PH_DOWNDATE,         // sketch downdate
+  HQ_PHASES
+};
+static const char* const kPhaseNames[HQ_PHASES] = {
+    "sketch_gemm", "sketch_qrcp", "gather", "panel", "panel_post",
+    "trail_vtc", "trail_tw", "trail_update", "downdate"};
+
+struct hqrrp_handle_s {
+  int device = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  int cu_count = 0;
+  size_t lds_max = 0;
+  int panel_rows = 256;  // rows of the panel held by one workgroup
+  int sketch_max_wg = 0, panel_max_wg = 0;
+  DevBuf words;  // [0..1] sketch barrier, [2..3] panel barrier, [8] error
+  DevBuf Y, G, Gcache, cand, moves, piv, ppart, rowbuf, gram, T, perm;
+  DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
+  DevBuf hA, hJ, hT;  // device copies used by the host entry points
+  int om_r = -1, om_m = -1;
+  unsigned om_seed = 0;
+  std::mutex mu;
+  // Optional per-phase timing with HIP events recorded on `stream`.
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Span {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> spans;
+  double ph_flops[HQ_PHASES] = {}, ph_bytes[HQ_PHASES] = {};
+  long long ph_launch[HQ_PHASES] = {};
+
+  unsigned* bar_sketch() const { return words.as<unsigned>(); }
+  unsigned* bar_panel() const { return words.as<unsigned>() + 2; }
+  int* err() const { return words.as<int>() + 8; }
+};
+
+namespace {
+
+int check_err(hqrrp_handle_t h) {
+  int e = 0;
+  HQ_TRY(hipMemcpyAsync(&e, h->err(), sizeof(int), hipMemcpyDeviceToHost,
+                        h->stream));
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  if (e != 0) {
+    fprintf(stderr, "hqrrp: kernel reported error %d (grid barrier timeout)\n",
+            e);
+    HQ_TRY(hipMemsetAsync(h->words.p, 0, h->words.bytes, h->stream));
+    HQ_TRY(hipStreamSynchronize(h->stream));
+    return HQRRP_ERR_KERNEL;
+  }
+  return HQRRP_OK;
+}
+
+hipEvent_t next_event(hqrrp_handle_t h) {
+  if (h->ev_used == h->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_used++];
+}
+
+// Brackets the launches of one phase with events when profiling is on and
+// accumulates the phase's algorithmic flops/bytes and launch count.
+struct PhaseScope {
+  hqrrp_handle_t h;
+  int ph;
+  hipEvent_t a = nullptr;
+  PhaseScope(hqrrp_handle_t h_, int ph_, double flops, double bytes,
+             int launches)
+      : h(h_), ph(ph_) {
+    if (!h->prof) return;
+    h->ph_flops[ph] += flops;
+    h->ph_bytes[ph] += bytes;
+    h->ph_launch[ph] += launches;
+    a = next_event(h);
+    if (a) (void)hipEventRecord(a, h->stream);
+  }
+  ~PhaseScope() {
+    if (!h->prof || !a) return;
+    hipEvent_t b = next_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, h->stream);
+    h->spans.push_back({ph, a, b});
+  }
+};
+
+// Split-K factor for a product with `tiles` output tiles and reduction
+// length K: aim for >= 2 workgroups per CU, slices of >= 256.
+int choose_splits(hqrrp_handle_t h, long tiles, long K, long* kchunk) {
+  long want = std::max(1L, (2L * h->cu_count + tiles - 1) / tiles);
+  long maxs = std::max(1L, K / 256);
+  long s = std::min(want, maxs);
+  long kc = (K + s - 1) / s;
+  kc = (kc + 15) / 16 * 16;
+  if (kc <= 0) kc = 16;
+  *kchunk = kc;
+  return (int)std::max(1L, (K + kc - 1) / kc);
+}
+
+// C := alpha op(A) op(B) + beta C with automatic split-K.
+int gemm(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K, double alpha,
+         const double* A, long lda, int maskA, const double* B, long ldb,
+         int maskB, double beta, double* C, long ldc, bool allow_split = true,
+         int tag = 0) {
+  if (M <= 0 || N <= 0) return HQRRP_OK;
+  GemmArgs g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.maskA = maskA;
+  g.maskB = maskB;
+  long tiles = (long)hq_cdiv(M, 64) * hq_cdiv(N, 64);
+  long kc = K > 0 ? K : 16;
+  int S = 1;
+  if (allow_split && K > 0) S = choose_splits(h, tiles, K, &kc);
+  g.kchunk = kc;
+  if (S > 1) {
+    HQ_RET(ensure(h->split, sizeof(double) * (size_t)S * M * N));
+    g.part = h->split.as<double>();
+    HQ_TRY(launch_gemm(g, ta, tb, S, h->stream, tag));
+    HQ_TRY(launch_reduce_splitk(M, N, S, g.part, beta, C, ldc, h->stream));
+  } else {
+    g.part = nullptr;
+    HQ_TRY(launch_gemm(g, ta, tb, 1, h->stream, tag));
+  }
+  return HQRRP_OK;
+}
+
+int panel_geometry(hqrrp_handle_t h, int mrows, int* R, int* nw) {
+  int rows = h->panel_rows;
+  int n = hq_cdiv(mrows, rows);
+  if (n > h->panel_max_wg) return HQRRP_ERR_UNSUPPORTED;
+  *R = rows;
+  *nw = n;
+  return HQRRP_OK;
+}
+
+int run_sketch(hqrrp_handle_t h, int r, int ncols, const double* dY, long ldy,
+               int stages, int* dpiv, int* dmoves) {
+  int rmax = sketch_rmax(r);
+  if (rmax < 0) return HQRRP_ERR_UNSUPPORTED;
+  SketchArgs a{};
+  a.Y = dY;
+  a.ldy = ldy;
+  a.r = r;
+  a.ncols = ncols;
+  a.stages = stages;
+  a.nw = hq_cdiv(ncols, sketch_cols_per_wg());
+  if (a.nw > h->sketch_max_wg) return HQRRP_ERR_UNSUPPORTED;
+  HQ_RET(ensure(h->cand, sizeof(double) * sketch_cand_doubles(a.nw, rmax)));
+  a.piv = dpiv;
+  a.moves = dmoves;
+  a.cand = h->cand.as<double>();
+  a.bar = h->bar_sketch();
+  a.err = h->err();
+  HQ_TRY(launch_sketch_qrcp(a, rmax, h->stream));
+  return HQRRP_OK;
+}
+
+int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
+              int pivoting, double* dtau, int* dperm, double* dT, long ldt) {
+  if (b < 1 || b > 64 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
+  PanelArgs a{};
+  HQ_RET(panel_geometry(h, mrows, &a.R, &a.nw));
+  a.ldR = a.R + 1;
+  a.P = dP;
+  a.ldp = ldp;
+  a.mrows = mrows;
+  a.b = b;
+  a.pivoting = pivoting;
+  a.tau = dtau;
+  a.perm = dperm;
+  HQ_RET(ensure(h->gram, sizeof(double) * 64 * 64));
+  HQ_RET(ensure(h->ppart, sizeof(double) * (size_t)a.nw * 192));
+  HQ_RET(ensure(h->rowbuf, sizeof(double) * 128));
+  a.gram = h->gram.as<double>();
+  a.part = h->ppart.as<double>();
+  a.rowbuf = h->rowbuf.as<double>();
+  a.bar = h->bar_panel();
+  a.err = h->err();
+  {
+    // Algorithmic traffic: the panel is read and written once.
+    PhaseScope ps(h, PH_PANEL, 2.0 * mrows * b * b - 2.0 / 3.0 * b * b * b,
+                  16.0 * mrows * b, 1);
+    HQ_TRY(launch_panel_qrcp(a, h->stream));
+  }
+  PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
+  HQ_TRY(launch_larft(b, a.gram, dtau, dT, ldt, h->stream));
+  return HQRRP_OK;
+}
+
+// [A12; A22] := (I - V T V')' [A12; A22]  (dlarfb Left/Transpose):
+// M = V'C (split-K), W = T'M, C -= V W.
+int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
+                  long ldv, const double* T, long ldt, double* C, long ldc) {
+  if (nc <= 0 || mrows <= 0) return HQRRP_OK;
+  HQ_RET(ensure(h->Mx, sizeof(double) * (size_t)b * nc));
+  HQ_RET(ensure(h->W2, sizeof(double) * (size_t)b * nc));
+  double* Mx = h->Mx.as<double>();
+  double* W2 = h->W2.as<double>();
+  const double mn = (double)mrows * nc;
+  {
+    PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * mrows * b, 1);
+    HQ_RET(gemm(h, true, false, b, nc, mrows, 1.0, V, ldv, 1, C, ldc, 0, 0.0,
+                Mx, b, true, 1));
+  }
+  {
+    PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc, 16.0 * b * nc, 1);
+    HQ_RET(gemm(h, true, false, b, nc, b, 1.0, T, ldt, 0, Mx, b, 0, 0.0, W2, b,
+                false));
+  }
+  {
+    // C is read and written once; V and W are small.
+    PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * mn, 16.0 * mn, 1);
+    HQ_RET(gemm(h, false, false, mrows, nc, b, -1.0, V, ldv, 1, W2, b, 0, 1.0,
+                C, ldc, false, 2));
+  }
+  return HQRRP_OK;
+}
+
+// Sketch downdate with the old G_R = [G1 G2]:  P = G_R V, Q = P T,
+// G_R -= Q V' (whose first b columns are the reference's B matrix), then
+// Y2 -= G1_new R12.
+int downdate_y(hqrrp_handle_t h, int r, int b, int mU, int nY2, const double* U,
+               long ldu, const double* R12, long ldr, const double* T, long ldt,
+               double* Y2, long ldy, double* G, long ldg) {
+  HQ_RET(ensure(h->Pp, sizeof(double) * (size_t)r * b));
+  HQ_RET(ensure(h->Qm, sizeof(double) * (size_t)r * b));
+  double* P = h->Pp.as<double>();
+  double* Q = h->Qm.as<double>();
+  HQ_RET(gemm(h, false, false, r, b, mU, 1.0, G, ldg, 0, U, ldu, 1, 0.0, P, r));
+  HQ_RET(gemm(h, false, false, r, b, b, 1.0, P, r, 0, T, ldt, 0, 0.0, Q, r,
+              false));
+  HQ_RET(gemm(h, false, true, r, mU, b, -1.0, Q, r, 0, U, ldu, 1, 1.0, G, ldg,
+              false));
+  if (nY2 > 0)
+    HQ_RET(gemm(h, false, false, r, nY2, b, -1.0, G, ldg, 0, R12, ldr, 0, 1.0,
+                Y2, ldy, false));
+  return HQRRP_OK;
+}
+
+int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
+  HQ_RET(ensure(h->G, sizeof(double) * (size_t)r * m));
+  if (o.omega_kind == HQRRP_OMEGA_USER) {
+    if (!o.omega) return HQRRP_ERR_ARG;
+    HQ_TRY(hipMemcpyAsync(h->G.p, o.omega, sizeof(double) * (size_t)r * m,
+                          hipMemcpyDeviceToDevice, h->stream));
+    return HQRRP_OK;
+  }
+  if (h->om_r != r || h->om_m != m || h->om_seed != o.seed) {
+    std::vector<double> host((size_t)r * m);
+    hqrrp_reference_omega(r, m, host.data(), o.seed);
+    HQ_RET(ensure(h->Gcache, sizeof(double) * (size_t)r * m));
+    HQ_TRY(hipMemcpy(h->Gcache.p, host.data(), sizeof(double) * host.size(),
+                     hipMemcpyHostToDevice));
+    h->om_r = r;
+    h->om_m = m;
+    h->om_seed = o.seed;
+  }
+  HQ_TRY(hipMemcpyAsync(h->G.p, h->Gcache.p, sizeof(double) * (size_t)r * m,
+                        hipMemcpyDeviceToDevice, h->stream));
+  return HQRRP_OK;
+}
+
+int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
+                  int* djpvt, double* dtau, const hqrrp_options& o) {
+  const int nb = o.nb, pp = o.pp, r = nb + pp;
+  const int mn = std::min(m, n);
+  if (mn == 0) return HQRRP_OK;
+  const int kmax = (o.max_rank > 0) ? std::min(mn, o.max_rank) : mn;
+  if (o.sketch_prec != HQRRP_SKETCH_FP64) return HQRRP_ERR_UNSUPPORTED;
+  if (sketch_rmax(r) < 0 || nb > 64) return HQRRP_ERR_UNSUPPORTED;
+
+  HQ_TRY(hipSetDevice(h->device));
+  HQ_RET(load_omega(h, r, m, o));
+  HQ_RET(ensure(h->Y, sizeof(double) * (size_t)r * n));
+  HQ_RET(ensure(h->moves, sizeof(int) * (size_t)(1 + 4 * nb)));
+  HQ_RET(ensure(h->piv, sizeof(int) * (size_t)nb));
+  HQ_RET(ensure(h->perm, sizeof(int) * 64));
+  HQ_RET(ensure(h->T, sizeof(double) * (size_t)nb * nb));
+  HQ_RET(ensure(h->tmpA, sizeof(double) * (size_t)2 * nb * m));
+  HQ_RET(ensure(h->tmpY, sizeof(double) * (size_t)2 * nb * r));
+  HQ_RET(ensure(h->tmpJ, sizeof(int) * (size_t)2 * nb));
+  double* Y = h->Y.as<double>();
+  double* G = h->G.as<double>();
+  double* T = h->T.as<double>();
+
+  // Y = Omega * A (NoFLA_HQRRP_WY_blk_var4.c:407-409).
+  {
+    PhaseScope ps(h, PH_SKETCH_GEMM, 2.0 * r * (double)m * n,
+                  8.0 * ((double)m * n + (double)r * (m + n)), 1);
+    HQ_RET(gemm(h, false, false, r, n, m, 1.0, G, r, 0, dA, lda, 0, 0.0, Y, r));
+  }
+
+  for (int j = 0; j < kmax; j += nb) {
+    const int b = std::min(nb, std::min(n - j, m - j));
+    const bool last = (j + nb >= m) || (j + nb >= n);
+    const int mrows = m - j;
+    double* Ajj = dA + j + (long)j * lda;
+    if (!last) {
+      {
+        PhaseScope ps(h, PH_SKETCH_QRCP, 4.0 * r * (double)(n - j) * b,
+                      8.0 * r * (double)(n - j), 1);
+        HQ_RET(run_sketch(h, r, n - j, Y + (long)j * r, r, b,
+                          h->piv.as<int>(), h->moves.as<int>()));
+      }
+      PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
+      HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b, dA + (long)j * lda,
+                                lda, m, Y + (long)j * r, r, r, djpvt + j,
+                                h->tmpA.as<double>(), h->tmpY.as<double>(),
+                                h->tmpJ.as<int>(), h->stream));
+    }
+    HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
+                     h->perm.as<int>(), T, nb));
+    {
+      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * j * b, 1);
+      HQ_TRY(launch_panel_permute(dA + (long)j * lda, lda, j, b,
+                                  h->perm.as<int>(), djpvt + j, h->stream));
+    }
+    if (j + b < n)
+      HQ_RET(apply_qt_left(h, mrows, n - j - b, b, Ajj, lda, T, nb,
+                           dA + j + (long)(j + b) * lda, lda));
+    if (!last) {
+      PhaseScope ps(h, PH_DOWNDATE,
+                    4.0 * r * b * (double)mrows + 2.0 * r * b * (n - j - b),
+                    8.0 * (2.0 * r * mrows + 2.0 * r * (n - j - b)), 4);
+      HQ_RET(downdate_y(h, r, b, mrows, n - j - b, Ajj, lda,
+                        dA + j + (long)(j + b) * lda, lda, T, nb,
+                        Y + (long)(j + b) * r, r, G + (long)j * r, r));
+    }
+  }
+  HQ_TRY(hipGetLastError());
+  return HQRRP_OK;
+}
+
+std::mutex g_default_mu;
+hqrrp_handle_t g_default[64];
+
+int default_handle(hqrrp_handle_t* out) {
+  int dev = 0;
+  HQ_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default[dev]) HQ_RET(hqrrp_create(&g_default[dev], dev));
+  *out = g_default[dev];
+  return HQRRP_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" void hqrrp_default_options(hqrrp_options* o) {
+  memset(o, 0, sizeof(*o));
+  o->nb = 64;
+  o->pp = 10;
+  o->panel_pivoting = 1;
+  o->sketch_prec = HQRRP_SKETCH_FP64;
+  o->omega_kind = HQRRP_OMEGA_REFERENCE;
+  o->seed = 12;
+  o->omega = nullptr;
+  o->max_rank = 0;
+}
+
+extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
+  if (!out) return HQRRP_ERR_ARG;
+  hqrrp_handle_t h = new (std::nothrow) hqrrp_handle_s();
+  if (!h) return HQRRP_ERR_ALLOC;
+  h->device = device;
+  HQ_TRY(hipSetDevice(device));
+  HQ_TRY(hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking));
+  h->stream = h->own;
+  HQ_TRY(hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount,
+                               device));
+  int lds = 0;
+  HQ_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                               device));
+  h->lds_max = (size_t)lds;
+  if (const char* s = getenv("HQRRP_PANEL_ROWS")) h->panel_rows = atoi(s);
+  // The slab (64 columns x (R+1) doubles) plus ~8 KB of static LDS must fit.
+  while (h->panel_rows > 8 &&
+         (size_t)64 * (h->panel_rows + 1) * 8 + 8192 > h->lds_max)
+    h->panel_rows /= 2;
+  size_t slab = (size_t)64 * (h->panel_rows + 1) * 8;
+  HQ_TRY(panel_set_lds_limit(slab));
+  // Persistent kernels need all their workgroups co-resident; both kernels
+  // are launched at most one workgroup per CU (sketch: register-bound,
+  // panel: LDS-bound).
+  h->sketch_max_wg = h->cu_count;
+  h->panel_max_wg = h->cu_count;
+  int rc = ensure(h->words, 64);
+  if (rc) return rc;
+  HQ_TRY(hipMemset(h->words.p, 0, h->words.bytes));
+  *out = h;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
+  if (!h) return HQRRP_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  DevBuf* bufs[] = {&h->words, &h->Y,     &h->G,     &h->Gcache, &h->cand,
+                    &h->moves, &h->piv,   &h->ppart, &h->rowbuf, &h->gram,
+                    &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
+                    &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
+                    &h->hA,    &h->hJ,    &h->hT};
+  for (DevBuf* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->own) (void)hipStreamDestroy(h->own);
+  delete h;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_phase_count(void) { return HQ_PHASES; }
+
+extern "C" const char* hqrrp_phase_name(int phase) {
+  return (phase >= 0 && phase < HQ_PHASES) ? kPhaseNames[phase] : "";
+}
+
+extern "C" int hqrrp_profile_enable(hqrrp_handle_t h, int on) {
+  if (!h) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  h->prof = on != 0;
+  h->spans.clear();
+  h->ev_used = 0;
+  for (int p = 0; p < HQ_PHASES; ++p) {
+    h->ph_flops[p] = h->ph_bytes[p] = 0.0;
+    h->ph_launch[p] = 0;
+  }
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_profile_read(hqrrp_handle_t h, int nphase, double* ms,
+                                  long long* launches, double* flops,
+                                  double* bytes) {
+  if (!h || nphase < HQ_PHASES) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  for (int p = 0; p < HQ_PHASES; ++p) ms[p] = 0.0;
+  for (const auto& sp : h->spans) {
+    float t = 0.f;
+    HQ_TRY(hipEventElapsedTime(&t, sp.a, sp.b));
+    ms[sp.phase] += t;
+  }
+  for (int p = 0; p < HQ_PHASES; ++p) {
+    launches[p] = h->ph_launch[p];
+    flops[p] = h->ph_flops[p];
+    bytes[p] = h->ph_bytes[p];
+  }
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_set_stream(hqrrp_handle_t h, void* stream) {
+  if (!h) return HQRRP_ERR_ARG;
+  h->stream = stream ? (hipStream_t)stream : h->own;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_synchronize(hqrrp_handle_t h) {
+  if (!h) return HQRRP_ERR_ARG;
+  return check_err(h);
+}
+
+extern "C" int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double* dA,
+                                int lda, int* djpvt, double* dtau,
+                                const hqrrp_options* opt) {
+  if (!h || m < 0 || n < 0 || lda < std::max(1, m)) return HQRRP_ERR_ARG;
+  hqrrp_options o;
+  if (opt)
+    o = *opt;
+  else
+    hqrrp_default_options(&o);
+  if (o.nb < 1 || o.pp < 0) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return factor_locked(h, m, n, dA, lda, djpvt, dtau, o);
+}
+
+extern "C" int hqrrp_sketch_qrcp_dev(hqrrp_handle_t h, int r, int ncols,
+                                     const double* dY, int ldy, int stages,
+                                     int* dpiv, int* dmoves) {
+  if (!h || r < 1 || ncols < 1 || ldy < r || stages < 1 ||
+      stages > std::min(r, ncols))
+    return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return run_sketch(h, r, ncols, dY, ldy, stages, dpiv, dmoves);
+}
+
+extern "C" int hqrrp_panel_qrcp_dev(hqrrp_handle_t h, int mrows, int b,
+                                    double* dP, int ldp, int pivoting,
+                                    double* dtau, int* dperm, double* dT,
+                                    int ldt) {
+  if (!h || b < 1 || mrows < b || ldp < mrows || ldt < b) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return run_panel(h, mrows, b, dP, ldp, pivoting, dtau, dperm, dT, ldt);
+}
+
+extern "C" int hqrrp_apply_qt_left_dev(hqrrp_handle_t h, int mrows, int ncols,
+                                       int b, const double* dV, int ldv,
+                                       const double* dT, int ldt, double* dC,
+                                       int ldc) {
+  if (!h || b < 1 || mrows < b || ncols < 0 || ldv < mrows || ldc < mrows ||
+      ldt < b)
+    return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return apply_qt_left(h, mrows, ncols, b, dV, ldv, dT, ldt, dC, ldc);
+}
+
+extern "C" int hqrrp_downdate_y_dev(hqrrp_handle_t h, int r, int b, int mU,
+                                    int nY2, const double* dU, int ldu,
+                                    const double* dR12, int ldr,
+                                    const double* dT, int ldt, double* dY2,
+                                    int ldy, double* dG, int ldg) {
+  if (!h || r < 1 || b < 1 || mU < b || nY2 < 0 || ldu < mU || ldr < b ||
+      ldt < b || ldy < r || ldg < r)
+    return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return downdate_y(h, r, b, mU, nY2, dU, ldu, dR12, ldr, dT, ldt, dY2, ldy,
+                    dG, ldg);
+}
+
+extern "C" int hqrrp_dgemm_dev(hqrrp_handle_t h, int ta, int tb, int M, int N,
+                               int K, double alpha, const double* dA, int lda,
+                               const double* dB, int ldb, double beta,
+                               double* dC, int ldc) {
+  if (!h || M < 0 || N < 0 || K < 0 || ldc < std::max(1, M)) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  return gemm(h, ta != 0, tb != 0, M, N, K, alpha, dA, lda, 0, dB, ldb, 0,
+              beta, dC, ldc);
+}
+
+// --------------------------------------------------- drop-in host entries
+
+namespace {
+
+// Host-pointer HQRRP on the default handle of the current device.
+int host_hqrrp(int m, int n, double* A, int lda, int* jpvt, double* tau,
+               const hqrrp_options& o) {
+  hqrrp_handle_t h = nullptr;
+  HQ_RET(default_handle(&h));
+  std::lock_guard<std::mutex> lk(h->mu);
+  const int mn = std::min(m, n);
+  HQ_RET(ensure(h->hA, sizeof(double) * (size_t)std::max(1, m) * n));
+  HQ_RET(ensure(h->hJ, sizeof(int) * (size_t)std::max(1, n)));
+  HQ_RET(ensure(h->hT, sizeof(double) * (size_t)std::max(1, mn)));
+  double* dA = h->hA.as<double>();
+  HQ_TRY(hipMemcpy2DAsync(dA, sizeof(double) * m, A, sizeof(double) * lda,
+                          sizeof(double) * m, n, hipMemcpyHostToDevice,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(h->hJ.p, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
+                        h->stream));
+  HQ_RET(factor_locked(h, m, n, dA, m, h->hJ.as<int>(), h->hT.as<double>(), o));
+  HQ_RET(check_err(h));
+  HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
+                          sizeof(double) * m, n, hipMemcpyDeviceToHost,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(jpvt, h->hJ.p, sizeof(int) * n, hipMemcpyDeviceToHost,
+                        h->stream));
+  HQ_TRY(hipMemcpyAsync(tau, h->hT.p, sizeof(double) * mn,
+                        hipMemcpyDeviceToHost, h->stream));
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  return HQRRP_OK;
+}
+
+}  // namespace
+
+extern "C" int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double* buff_A,
+                                       int ldim_A, int* buff_jpvt,
+                                       double* buff_tau, int nb_alg, int pp,
+                                       int panel_pivoting) {
+  // Argument messages as in NoFLA_HQRRP_WY_blk_var4.c:358-367; unlike the
+  // reference, an illegal argument returns an error instead of continuing.
+  int bad = 0;
+  if (m_A < 0) {
+    fprintf(stderr, "ERROR in NoFLA_HQRRP_WY_blk_var4: m_A is < 0.\n");
+    bad = 1;
+  }
+  if (n_A < 0) {
+    fprintf(stderr, "ERROR in NoFLA_HQRRP_WY_blk_var4: n_A is < 0.\n");
+    bad = 1;
+  }
+  if (ldim_A < std::max(1, m_A)) {
+    fprintf(stderr,
+            "ERROR in NoFLA_HQRRP_WY_blk_var4: ldim_A is < max( 1, m_A ).\n");
+    bad = 1;
+  }
+  if (bad) return HQRRP_ERR_ARG;
+  if (std::min(m_A, n_A) == 0) return 0;
+  if (nb_alg < 1 || pp < 0) return HQRRP_ERR_ARG;
+  hqrrp_options o;
+  hqrrp_default_options(&o);
+  o.nb = nb_alg;
+  o.pp = pp;
+  o.panel_pivoting = panel_pivoting;
+  return host_hqrrp(m_A, n_A, buff_A, ldim_A, buff_jpvt, buff_tau, o);
+}
+
+extern "C" void dgeqp4(int* m, int* n, double* A, int* lda, int* jpvt,
+                       double* tau, double* work, int* lwork, int* info) {
+  // Argument checking and workspace query as NoFLA_HQRRP_WY_blk_var4.c:
+  // 148-182 (the reference asks LAPACK's ilaenv for DGEQRF's block size;
+  // this library has no LAPACK and reports with nb = 32, netlib's value).
+  const int mA = *m, nA = *n, ld = *lda;
+  const int mn = std::min(mA, nA);
+  const bool lquery = (*lwork == -1);
+  *info = 0;
+  if (mA < 0)
+    *info = -1;
+  else if (nA < 0)
+    *info = -2;
+  else if (ld < std::max(1, mA))
+    *info = -4;
+  int iws = 1, lwkopt = 1;
+  if (*info == 0) {
+    if (mn != 0) {
+      iws = 3 * nA + 1;
+      lwkopt = 2 * nA + (nA + 1) * 32;
+    }
+    work[0] = (double)lwkopt;
+    if (*lwork < iws && !lquery) *info = -8;
+  }
+  if (*info != 0) {
+    fprintf(stderr,
+            " ** On entry to DGEQP3 parameter number %d had an illegal value\n",
+            -*info);
+    return;
+  }
+  if (lquery || mn == 0) return;
+
+  // Fixed columns (jpvt != 0) to the front, as WY.c:201-218.
+  int nfixed = 0;
+  std::vector<double> colbuf;
+  for (int c = 0; c < nA; ++c) {
+    if (jpvt[c] != 0) {
+      if (c != nfixed) {
+        colbuf.assign(A + (long)c * ld, A + (long)c * ld + mA);
+        memcpy(A + (long)c * ld, A + (long)nfixed * ld, sizeof(double) * mA);
+        memcpy(A + (long)nfixed * ld, colbuf.data(), sizeof(double) * mA);
+        jpvt[c] = jpvt[nfixed];
+        jpvt[nfixed] = c + 1;
+      } else {
+        jpvt[c] = c + 1;
+      }
+      ++nfixed;
+    } else {
+      jpvt[c] = c + 1;
+    }
+  }
+  const int nf = std::min(mA, nfixed);
+  if (nf > 0) {
+    // Fixed-column factorisation (dgeqrf + dormqr in the reference) is not
+    // implemented on the GPU yet.
+    fprintf(stderr, "dgeqp4: fixed columns are not supported by this build\n");
+    *info = -5;
+    return;
+  }
+  hqrrp_options o;
+  hqrrp_default_options(&o);
+  int rc = host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
+  if (rc != HQRRP_OK) {
+    fprintf(stderr, "dgeqp4: device factorisation failed (%d)\n", rc);
+    *info = rc;
+    return;
+  }
+  work[0] = iws;
+}

@@ -1,0 +1,177 @@
+// k_gemm.hip -- FP64 matrix-core products for the blocked update kernels.
+//
+// One kernel template covers every product of the block loop:
+//   trailing update (dlarfb Left/Transpose, reference
+//     NoFLA_HQRRP_WY_blk_var4.c:711-740):  M = V'C (split-K), C -= V (T'M);
+//   sketch Y = Omega A (407-409);
+//   sketch downdate (617-708): P = G V, G -= (P T) V', Y2 -= G1 R12.
+// C := alpha op(A) op(B) + beta C, all column-major.  A stored operand can be
+// masked as unit lower trapezoidal (entries above the diagonal read as 0,
+// the diagonal as 1): that is how the reflector block V is read straight out
+// of the factored panel, which holds R above the diagonal.
+//
+// Tiling: 64x64 output tile per 256-thread workgroup, K staged through LDS
+// in steps of 16; each wavefront owns a 32x32 sub-tile computed with 2x2
+// v_mfma_f64_16x16x4f64 (A fragment: lane l holds A[l%16][l/16]; B fragment:
+// B[l/16][l%16]; accumulator element r of lane l is C[4(l/16)+r][l%16]).
+// With split-K (gridDim.z > 1) each slice writes alpha*partial to its own
+// M x N buffer and hq_reduce_splitk adds the slices in a fixed order, so
+// results are deterministic.
+#include "common.h"
+#include "kernels.h"
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, LDP = 80;  // LDS row pitch (pad 16)
+
+__device__ __forceinline__ double unit_lower(double v, long i, long c,
+                                             int mask) {
+  if (!mask) return v;
+  return (i < c) ? 0.0 : ((i == c) ? 1.0 : v);
+}
+
+// TAG only gives distinct kernel symbols to the call sites profiled apart
+// (1: trailing M = V'C, 2: trailing C -= V W, 0: everything else).
+template <bool TA, bool TB, int TAG>
+__global__ __launch_bounds__(HQ_NT) void gemm_f64_kernel(GemmArgs g) {
+  __shared__ double As[BK][LDP];
+  __shared__ double Bs[BK][LDP];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const long m0 = (long)blockIdx.x * BM, n0 = (long)blockIdx.y * BN;
+  const long kbeg = (long)blockIdx.z * g.kchunk;
+  const long kend = min((long)g.K, kbeg + g.kchunk);
+
+  hq_d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = hq_d4{0.0, 0.0, 0.0, 0.0};
+
+  for (long k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (!TA) {  // stored A: M x K, element (m,k) at m + k*lda
+        int mm = t & 63, kk = (t >> 6) + 4 * q;
+        long gm = m0 + mm, gk = k0 + kk;
+        double v = 0.0;
+        if (gm < g.M && gk < kend)
+          v = unit_lower(g.A[gm + gk * g.lda], gm, gk, g.maskA);
+        As[kk][mm] = v;
+      } else {  // stored A: K x M, element (m,k) at k + m*lda
+        int kk = t & 15, mm = (t >> 4) + 16 * q;
+        long gm = m0 + mm, gk = k0 + kk;
+        double v = 0.0;
+        if (gm < g.M && gk < kend)
+          v = unit_lower(g.A[gk + gm * g.lda], gk, gm, g.maskA);
+        As[kk][mm] = v;
+      }
+      if (!TB) {  // stored B: K x N, element (k,n) at k + n*ldb
+        int kk = t & 15, nn = (t >> 4) + 16 * q;
+        long gn = n0 + nn, gk = k0 + kk;
+        double v = 0.0;
+        if (gn < g.N && gk < kend)
+          v = unit_lower(g.B[gk + gn * g.ldb], gk, gn, g.maskB);
+        Bs[kk][nn] = v;
+      } else {  // stored B: N x K, element (k,n) at n + k*ldb
+        int nn = t & 63, kk = (t >> 6) + 4 * q;
+        long gn = n0 + nn, gk = k0 + kk;
+        double v = 0.0;
+        if (gn < g.N && gk < kend)
+          v = unit_lower(g.B[gn + gk * g.ldb], gn, gk, g.maskB);
+        Bs[kk][nn] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        af[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        bf[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+                                                            acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        long gm = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        long gn = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (gm < g.M && gn < g.N) {
+          double v = g.alpha * acc[i][j][r];
+          if (g.part) {
+            g.part[(long)blockIdx.z * g.M * g.N + gm + gn * g.M] = v;
+          } else {
+            double* cp = g.C + gm + gn * g.ldc;
+            *cp = (g.beta == 0.0) ? v : v + g.beta * *cp;
+          }
+        }
+      }
+}
+
+// C := beta*C + sum_s part[s] (slices added in order).
+__global__ __launch_bounds__(HQ_NT) void reduce_splitk_kernel(
+    int M, int N, int S, const double* part, double beta, double* C,
+    long ldc) {
+  long idx = (long)blockIdx.x * HQ_NT + threadIdx.x;
+  long total = (long)M * N;
+  if (idx >= total) return;
+  long i = idx % M, j = idx / M;
+  double s = 0.0;
+  for (int q = 0; q < S; ++q) s += part[(long)q * total + idx];
+  double* cp = C + i + j * ldc;
+  *cp = (beta == 0.0) ? s : s + beta * *cp;
+}
+
+}  // namespace
+
+template <int TAG>
+hipError_t launch_gemm_tag(const GemmArgs& g, bool ta, bool tb, dim3 grid,
+                           hipStream_t stream) {
+  dim3 block(HQ_NT);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<false, false, TAG>), grid, block, 0,
+                       stream, g);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<false, true, TAG>), grid, block, 0,
+                       stream, g);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((gemm_f64_kernel<true, false, TAG>), grid, block, 0,
+                       stream, g);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<true, true, TAG>), grid, block, 0,
+                       stream, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(const GemmArgs& g, bool ta, bool tb, int splits,
+                       hipStream_t stream, int tag) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  dim3 grid(hq_cdiv(g.M, BM), hq_cdiv(g.N, BN), splits);
+  if (tag == 1 && ta && !tb) return launch_gemm_tag<1>(g, ta, tb, grid, stream);
+  if (tag == 2 && !ta && !tb) return launch_gemm_tag<2>(g, ta, tb, grid, stream);
+  return launch_gemm_tag<0>(g, ta, tb, grid, stream);
+}
+
+hipError_t launch_reduce_splitk(int M, int N, int S, const double* part,
+                                double beta, double* C, long ldc,
+                                hipStream_t stream) {
+  long total = (long)M * N;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_splitk_kernel, dim3(hq_cdiv(total, HQ_NT)),
+                     dim3(HQ_NT), 0, stream, M, N, S, part, beta, C, ldc);
+  return hipGetLastError();
+}

@@ -1,0 +1,236 @@
+// k_sketch.hip -- pivot selection on the sketch (greedy QRCP, pivots only).
+//
+// Restates NoFLA_QRPmod_WY_unb_var4(pivoting=1, num_stages=b) applied to the
+// working copy V_R of the sketch Y_R (reference NoFLA_HQRRP_WY_blk_var4.c:
+// 500-506, 774-886, norms 889-909, Drmac downdate 912-986, swaps 990-1036).
+//
+// Design (MI355X): one persistent launch per block of b pivots.  Two lanes of
+// a wavefront own one sketch column and keep it in registers for all b
+// steps; a 512-thread workgroup holds 256 columns, so n = 16384 needs 64
+// workgroups.  A step is:
+//   1. per-workgroup argmax of the partial norms (first position on ties),
+//      the winning column published to global memory;
+//   2. one grid barrier;
+//   3. every workgroup reduces the candidates identically, builds the same
+//      Householder reflector from the winning column and applies it, the
+//      LAWN-176 norm downdate, and the bookkeeping of positions, to its own
+//      columns.
+// Columns never move: a column's *position* is tracked instead, which is what
+// the reference's swap-not-shift semantics and idamax's first-index tie rule
+// depend on.  At the end each moved column appends (dst, src) to a list, the
+// composed permutation that the gather kernel applies to A, Y and jpvt.
+#include <climits>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace {
+
+constexpr int kSlotHead = 8;  // candidate slot: [key, pos, ...pad, column]
+constexpr int kThreads = 512;  // 8 wavefronts, 32 columns each
+constexpr int kCols = 256;     // sketch columns per workgroup
+
+// Column c of the workgroup lives in wavefront c/32; lane c%32 holds rows
+// [0, RMAX/2) and lane c%32+32 rows [RMAX/2, RMAX), so a column's dot
+// product is one cross-lane add (a+b == b+a, both lanes get the same bits).
+template <int RMAX>
+__global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
+  constexpr int H = RMAX / 2;
+  const int t = threadIdx.x, w = blockIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int half = lane >> 5;
+  const int r = a.r;
+  const int c = w * kCols + wave * 32 + (lane & 31);
+  const bool own = c < a.ncols;
+  const int ibase = half * H;  // first row held by this lane
+  const long slot = kSlotHead + RMAX;
+
+  __shared__ double s_v[RMAX];
+  __shared__ double s_key[8];
+  __shared__ int s_pos[8];
+  __shared__ int s_win_pos, s_win_wg;
+  __shared__ double s_tau;
+
+  double x[H];
+#pragma unroll
+  for (int q = 0; q < H; ++q) {
+    const int i = ibase + q;
+    x[q] = (own && i < r) ? a.Y[(long)c * a.ldy + i] : 0.0;
+  }
+  double ss = 0.0;
+#pragma unroll
+  for (int q = 0; q < H; ++q) ss += x[q] * x[q];
+  ss += __shfl_xor(ss, 32, 64);
+  double d = sqrt(ss), e = d;
+  int pos = own ? c : -1;  // current position; -1 once chosen as a pivot
+  int done_at = -1;
+
+  if (w == 0 && t == 0) a.moves[0] = 0;
+  unsigned phase = 0;
+
+  for (int k = 0; k < a.stages; ++k) {
+    double* cbuf = a.cand + (long)(k & 1) * a.nw * slot;
+    // 1. local argmax: largest norm, smallest position among equals.
+    double key = (pos >= 0) ? d : -1.0;
+    int kp = (pos >= 0) ? pos : INT_MAX;
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+      double ok = __shfl_xor(key, off, 64);
+      int op = __shfl_xor(kp, off, 64);
+      if (ok > key || (ok == key && op < kp)) {
+        key = ok;
+        kp = op;
+      }
+    }
+    if (lane == 0) {
+      s_key[wave] = key;
+      s_pos[wave] = kp;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double bk = s_key[0];
+      int bp = s_pos[0];
+      for (int q = 1; q < 8; ++q)
+        if (s_key[q] > bk || (s_key[q] == bk && s_pos[q] < bp)) {
+          bk = s_key[q];
+          bp = s_pos[q];
+        }
+      cbuf[(long)w * slot + 0] = bk;
+      cbuf[(long)w * slot + 1] = (double)bp;
+      s_win_pos = bp;
+    }
+    __syncthreads();
+    if (pos >= 0 && pos == s_win_pos) {
+      double* dst = cbuf + (long)w * slot + kSlotHead;
+#pragma unroll
+      for (int q = 0; q < H; ++q) {
+        const int i = ibase + q;
+        if (i >= k && i < r) dst[i] = x[q];
+      }
+    }
+    // 2. grid barrier
+    if (!hq_grid_sync(a.bar, a.nw, phase, a.err)) return;
+    // 3. global argmax over the workgroup candidates (wavefront 0).
+    if (wave == 0) {
+      double bk = -2.0;
+      int bp = INT_MAX, bw = -1;
+      for (int q = lane; q < a.nw; q += 64) {
+        double kq = cbuf[(long)q * slot + 0];
+        int pq = (int)cbuf[(long)q * slot + 1];
+        if (kq > bk || (kq == bk && pq < bp)) {
+          bk = kq;
+          bp = pq;
+          bw = q;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        double ok = __shfl_xor(bk, off, 64);
+        int op = __shfl_xor(bp, off, 64);
+        int ow = __shfl_xor(bw, off, 64);
+        if (ok > bk || (ok == bk && op < bp)) {
+          bk = ok;
+          bp = op;
+          bw = ow;
+        }
+      }
+      if (lane == 0) {
+        s_win_pos = bp;
+        s_win_wg = bw;
+      }
+    }
+    __syncthreads();
+    const int p = s_win_pos;
+    // 4. winning column into LDS; 5. reflector (serial, identical everywhere).
+    {
+      const double* src = cbuf + (long)s_win_wg * slot + kSlotHead;
+      for (int i = t; i < r; i += kThreads) s_v[i] = (i >= k) ? src[i] : 0.0;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double xn2 = 0.0;
+      for (int i = k + 1; i < r; ++i) xn2 += s_v[i] * s_v[i];
+      hq_reflector h = hq_dlarfg(s_v[k], xn2);
+      s_tau = h.tau;
+      for (int i = k + 1; i < r; ++i) s_v[i] *= h.scale;
+      s_v[k] = 1.0;
+      if (w == 0) a.piv[k] = p;
+    }
+    __syncthreads();
+    // 6. bookkeeping of positions (swap of positions k and p).
+    if (pos >= 0) {
+      if (pos == p) {
+        pos = -1;
+        done_at = k;
+      } else if (pos == k) {
+        pos = p;
+      }
+    }
+    // 7. apply the reflector to my column, then downdate its norm.  Both
+    // lanes of a column take the same branches (pos is per column).
+    if (pos >= 0) {
+      const double tau = s_tau;
+      if (tau != 0.0) {
+        double wd = 0.0;
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+          const int i = ibase + q;
+          if (i >= k && i < r) wd += s_v[i] * x[q];
+        }
+        wd += __shfl_xor(wd, 32, 64);
+        const double f = -tau * wd;
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+          const int i = ibase + q;
+          if (i >= k && i < r) x[q] += f * s_v[i];
+        }
+      }
+      double b2 = 0.0, wt = 0.0;
+#pragma unroll
+      for (int q = 0; q < H; ++q) {
+        const int i = ibase + q;
+        if (i == k) wt = x[q];
+        if (i > k && i < r) b2 += x[q] * x[q];
+      }
+      b2 += __shfl_xor(b2, 32, 64);
+      wt += __shfl_xor(wt, 32, 64);  // exactly one lane holds row k
+      hq_downdate_norm(d, e, wt, b2, r - k - 1);
+    }
+  }
+  // Composed permutation: final position of every column that moved.
+  if (own && half == 0) {
+    const int fp = (done_at >= 0) ? done_at : pos;
+    if (fp != c) {
+      int idx = atomicAdd(a.moves, 1);
+      a.moves[1 + 2 * idx] = fp;
+      a.moves[2 + 2 * idx] = c;
+    }
+  }
+  hq_grid_exit(a.bar, a.nw);
+}
+
+}  // namespace
+
+size_t sketch_cand_doubles(int nw, int rmax) {
+  return (size_t)2 * nw * (kSlotHead + rmax);
+}
+
+int sketch_cols_per_wg() { return kCols; }
+
+int sketch_rmax(int r) {
+  if (r <= 32) return 32;
+  if (r <= 80) return 80;
+  return -1;
+}
+
+hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax,
+                              hipStream_t stream) {
+  dim3 grid(a.nw), block(kThreads);
+  if (rmax == 32)
+    hipLaunchKernelGGL(sketch_qrcp_kernel<32>, grid, block, 0, stream, a);
+  else if (rmax == 80)
+    hipLaunchKernelGGL(sketch_qrcp_kernel<80>, grid, block, 0, stream, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}

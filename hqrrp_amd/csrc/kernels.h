@@ -1,0 +1,71 @@
+// kernels.h -- argument blocks and launchers of the device kernels
+// (host-side declarations shared by the driver and the kernel files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+struct SketchArgs {
+  const double* Y;  // sketch columns to pivot: r x ncols, ld ldy
+  long ldy;
+  int r, ncols, stages, nw;
+  int* piv;      // stages: position swapped with position k at step k
+  int* moves;    // [0] = count, then (dst, src) position pairs
+  double* cand;  // 2 * nw * (8 + rmax) doubles of scratch
+  unsigned* bar;
+  int* err;
+};
+
+struct PanelArgs {
+  double* P;  // panel: mrows x b, ld ldp (factored in place)
+  long ldp;
+  int mrows, b, pivoting;
+  int R, ldR, nw;  // rows per workgroup, LDS pitch, workgroups
+  double* tau;     // b
+  int* perm;       // b: original column at each position (0-based)
+  double* gram;    // b x b, strictly upper part: v_l' v_k
+  double* part;    // scratch: nw * 192 doubles
+  double* rowbuf;  // scratch: 128 doubles
+  unsigned* bar;
+  int* err;
+};
+
+struct GemmArgs {
+  int M, N, K;
+  double alpha, beta;
+  const double* A;
+  long lda;
+  const double* B;
+  long ldb;
+  double* C;
+  long ldc;
+  int maskA, maskB;  // read the stored operand as unit lower trapezoidal
+  long kchunk;       // K range of one split (blockIdx.z)
+  double* part;      // split-K output slices (M x N each), or null
+};
+
+// k_sketch.hip
+size_t sketch_cand_doubles(int nw, int rmax);
+int sketch_rmax(int r);
+int sketch_cols_per_wg();
+hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
+
+// k_panel.hip
+hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t s);
+hipError_t panel_set_lds_limit(size_t bytes);
+hipError_t launch_larft(int b, const double* gram, const double* tau,
+                        double* T, long ldt, hipStream_t s);
+hipError_t launch_panel_permute(double* A, long lda, int nrows, int b,
+                                const int* perm, int* jpvt, hipStream_t s);
+
+// k_gemm.hip
+hipError_t launch_gemm(const GemmArgs& g, bool ta, bool tb, int splits,
+                       hipStream_t s, int tag = 0);
+hipError_t launch_reduce_splitk(int M, int N, int S, const double* part,
+                                double beta, double* C, long ldc,
+                                hipStream_t s);
+
+// k_misc.hip
+hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
+                              long lda, int m, double* Y, long ldy, int r,
+                              int* jpvt, double* tmpA, double* tmpY, int* tmpJ,
+                              hipStream_t s);

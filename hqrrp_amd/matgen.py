@@ -1,0 +1,75 @@
+"""Seeded test matrices for the workload families HQRRP is measured on.
+
+* ``gaussian``      -- i.i.d. N(0,1) entries;
+* ``graded``        -- U diag(sigma) V' with Haar-orthogonal U, V (QR of seeded
+                       Gaussians, sign-fixed) and sigma_i = 10^(-decades*i/k);
+* ``lowrank_noise`` -- rank-k product of Gaussian factors with sigma_i = 1/i plus
+                       a small i.i.d. Gaussian perturbation.
+
+numpy versions are used at test sizes; ``graded_device`` builds the same
+family on the GPU with torch for the large bench sizes (input generation is
+outside every timed region).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gaussian(m: int, n: int, seed: int = 0) -> np.ndarray:
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal((m, n)))
+
+
+def _haar(rng, m: int, k: int) -> np.ndarray:
+    Q, R = np.linalg.qr(rng.standard_normal((m, k)))
+    return Q * np.sign(np.diag(R))
+
+
+def graded(m: int, n: int, seed: int = 0, decades: float = 12.0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    k = min(m, n)
+    U = _haar(rng, m, k)
+    V = _haar(rng, n, k)
+    sig = 10.0 ** (-decades * np.arange(k) / k)
+    return np.asfortranarray((U * sig) @ V.T)
+
+
+def lowrank_noise(m: int, n: int, rank: int = 512, seed: int = 0,
+                  noise: float = 1e-8) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal((m, rank)) / np.sqrt(m)
+    R = rng.standard_normal((n, rank)) / np.sqrt(n)
+    sig = 1.0 / np.arange(1, rank + 1)
+    return np.asfortranarray((L * sig) @ R.T + noise * rng.standard_normal((m, n)))
+
+
+def graded_device(m: int, n: int, seed: int = 0, decades: float = 12.0,
+                  device: str = "cuda"):
+    """Same family as :func:`graded`, built with torch on the GPU.
+
+    Returns a contiguous (n, m) tensor, i.e. the column-major m x n matrix.
+    """
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    k = min(m, n)
+
+    def haar(rows):
+        Z = torch.randn(rows, k, generator=g, device=device, dtype=torch.float64)
+        Q, R = torch.linalg.qr(Z)
+        return Q * torch.sign(torch.diagonal(R))
+
+    U = haar(m)
+    V = haar(n)
+    sig = 10.0 ** (-decades * torch.arange(k, device=device,
+                                           dtype=torch.float64) / k)
+    # (U diag(sig) V')' = V diag(sig) U'  -> row-major (n, m) buffer.
+    return ((V * sig) @ U.T).contiguous()
+
+
+def gaussian_device(m: int, n: int, seed: int = 0, device: str = "cuda"):
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    return torch.randn(n, m, generator=g, device=device, dtype=torch.float64)

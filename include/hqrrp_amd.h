@@ -1,0 +1,171 @@
+/*
+ * hqrrp_amd.h -- C ABI of the MI355X-native HQRRP library (libhqrrp_amd.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for the reference's API (host pointers, synchronous,
+ *     same signatures and argument meaning).  A program that links the
+ *     reference's NoFLA_HQRRP_WY_blk_var4.o can link this library instead.
+ *
+ *  2. Device-pointer, stream-ordered entry points used by the drop-in entries,
+ *     by the Python layer and by the tests.  All matrices are column-major
+ *     FP64 in device memory; pivot vectors are 1-based int32 like LAPACK's.
+ *
+ * Every function returns (or stores in *info) 0 on success.  Device entries
+ * return a negative hqrrp_status on failure; they never fall back to the CPU.
+ */
+#ifndef HQRRP_AMD_H
+#define HQRRP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum {
+  HQRRP_OK = 0,
+  HQRRP_ERR_ARG = -1,      /* illegal argument                            */
+  HQRRP_ERR_ALLOC = -2,    /* device or host allocation failed            */
+  HQRRP_ERR_HIP = -3,      /* a HIP runtime call failed                   */
+  HQRRP_ERR_KERNEL = -4,   /* a kernel reported failure (barrier timeout) */
+  HQRRP_ERR_UNSUPPORTED = -5 /* parameters outside the implemented range  */
+} hqrrp_status;
+
+/* ------------------------------------------------- 1. drop-in entry points */
+
+/* LAPACK dgeqp3-compatible QR with column pivoting (HQRRP for the free
+ * columns).  Replaces the reference's
+ *   void dgeqp4(int*, int*, double*, int*, int*, double*, double*, int*, int*)
+ * (lapack_compatible_sources/NoFLA_HQRRP_WY_blk_var4.h:2-3,
+ *  NoFLA_HQRRP_WY_blk_var4.c:127-300).  Host pointers; A is overwritten with
+ * R and the Householder vectors (dgeqrf storage), jpvt/tau as in dgeqp3.
+ * info: 0, -1 (m<0), -2 (n<0), -4 (lda<max(1,m)), -8 (lwork too small);
+ * lwork == -1 is a workspace query answered in work[0]. */
+void dgeqp4(int *m, int *n, double *A, int *lda, int *jpvt, double *tau,
+            double *work, int *lwork, int *info);
+
+/* Blocked HQRRP driver.  Replaces the reference's
+ *   int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double *buff_A, int ldim_A,
+ *        int *buff_jpvt, double *buff_tau, int nb_alg, int pp,
+ *        int panel_pivoting)
+ * (NoFLA_HQRRP_WY_blk_var4.h:5-7, NoFLA_HQRRP_WY_blk_var4.c:303-571).
+ * Host pointers.  jpvt is permuted in place (the caller initialises it, as
+ * with the reference).  Omega is the reference's stream (seed 12).  Returns
+ * 0, or a negative hqrrp_status (the reference prints and continues). */
+int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double *buff_A, int ldim_A,
+                            int *buff_jpvt, double *buff_tau, int nb_alg,
+                            int pp, int panel_pivoting);
+
+/* -------------------------------------------- 2. device-pointer interface */
+
+typedef struct hqrrp_handle_s *hqrrp_handle_t;
+
+/* Sketch precision: the sketch Y = Omega*A and its downdate only steer the
+ * pivot choice.  FP64 reproduces the reference. */
+typedef enum {
+  HQRRP_SKETCH_FP64 = 0,
+  HQRRP_SKETCH_FP32 = 1,
+  HQRRP_SKETCH_BF16 = 2
+} hqrrp_sketch_prec;
+
+/* Where Omega comes from. */
+typedef enum {
+  HQRRP_OMEGA_REFERENCE = 0, /* the reference's srand(seed)/polar stream    */
+  HQRRP_OMEGA_USER = 1       /* caller-supplied device matrix (r x m, ld r) */
+} hqrrp_omega_kind;
+
+typedef struct {
+  int nb;              /* block size nb_alg (reference default 64)           */
+  int pp;              /* oversampling (reference default 10)                */
+  int panel_pivoting;  /* 1: pivot inside each panel (reference default)     */
+  int sketch_prec;     /* hqrrp_sketch_prec                                  */
+  int omega_kind;      /* hqrrp_omega_kind                                   */
+  unsigned seed;       /* seed of the reference stream (reference: 12)       */
+  const double *omega; /* device pointer when omega_kind == HQRRP_OMEGA_USER */
+  int max_rank;        /* stop after this many columns (<=0: min(m,n))       */
+} hqrrp_options;
+
+/* Fills *opt with the reference defaults (64, 10, 1, FP64, seed 12). */
+void hqrrp_default_options(hqrrp_options *opt);
+
+/* Handle: owns device workspace, the grid-barrier words and a stream. */
+int hqrrp_create(hqrrp_handle_t *h, int device);
+int hqrrp_destroy(hqrrp_handle_t h);
+/* Use `stream` (a hipStream_t) for all work of this handle; NULL restores
+ * the handle's own stream. */
+int hqrrp_set_stream(hqrrp_handle_t h, void *stream);
+int hqrrp_synchronize(hqrrp_handle_t h);
+
+/* Optional per-phase timing (no counterpart in the reference, which only has
+ * compile-time wall-clock timers in its libflame variant): HIP events are
+ * recorded on the handle's stream around the launches of each phase of the
+ * block loop.  enable resets the counters; read synchronises the stream and
+ * returns, per phase, the summed event time (ms), the number of kernel
+ * launches, and the algorithmic flops and bytes of those launches. */
+int hqrrp_phase_count(void);
+const char *hqrrp_phase_name(int phase);
+int hqrrp_profile_enable(hqrrp_handle_t h, int on);
+int hqrrp_profile_read(hqrrp_handle_t h, int nphase, double *ms,
+                       long long *launches, double *flops, double *bytes);
+
+/* Device-pointer HQRRP, stream-ordered, no host synchronisation inside.
+ * Same semantics as NoFLA_HQRRP_WY_blk_var4 on device arrays:
+ * dA (m x n, ld lda), djpvt (n, permuted in place), dtau (min(m,n)). */
+int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
+                     int *djpvt, double *dtau, const hqrrp_options *opt);
+
+/* Host generation of the reference's Omega (r x m, column-major, ld r) with
+ * the stream srand(seed) + polar method, without touching the caller's
+ * rand() state.  (NoFLA_HQRRP_WY_blk_var4.c:380, 575-614) */
+void hqrrp_reference_omega(int r, int m, double *G, unsigned seed);
+
+/* ------------------------------------- component entry points (device) */
+
+/* Greedy QRCP of the sketch for `stages` steps, choosing pivots only
+ * (NoFLA_QRPmod_WY_unb_var4 with pivoting=1, num_stages=stages, on V_R,
+ * NoFLA_HQRRP_WY_blk_var4.c:500-506 and 774-886).  dY: r x ncols (ld ldy) is
+ * read, not modified.  dpiv[k] receives the position (0-based, relative) that
+ * is swapped with position k at step k.  dmoves receives the composed
+ * permutation: dmoves[0] = count, then (dst, src) position pairs. */
+int hqrrp_sketch_qrcp_dev(hqrrp_handle_t h, int r, int ncols,
+                          const double *dY, int ldy, int stages, int *dpiv,
+                          int *dmoves);
+
+/* Householder QR with optional column pivoting of an mrows x b panel
+ * (b <= 64), in place, with LAPACK dgeqrf storage; outputs tau (b), the
+ * position -> original-column permutation dperm (b, 0-based) and the
+ * compact-WY factor T (b x b, ld ldt, zero below the diagonal)
+ * (NoFLA_QRPmod_WY_unb_var4 with num_stages=-1 and build_T=1,
+ * NoFLA_HQRRP_WY_blk_var4.c:520-524, 774-886, dlarft at 875-878). */
+int hqrrp_panel_qrcp_dev(hqrrp_handle_t h, int mrows, int b, double *dP,
+                         int ldp, int pivoting, double *dtau, int *dperm,
+                         double *dT, int ldt);
+
+/* Trailing update C := (I - V T V')' C, V unit lower trapezoidal
+ * (mrows x b) and T upper triangular (dlarfb Left/Transpose/Forward/
+ * Columnwise, NoFLA_HQRRP_WY_blk_var4.c:711-740). */
+int hqrrp_apply_qt_left_dev(hqrrp_handle_t h, int mrows, int ncols, int b,
+                            const double *dV, int ldv, const double *dT,
+                            int ldt, double *dC, int ldc);
+
+/* Sketch downdate (NoFLA_Downdate_Y, NoFLA_HQRRP_WY_blk_var4.c:617-708):
+ * Y2 -= (G1 - (G1 U11 + G2 U21) T U11') R12 and [G1 G2] := [G1 G2](I-UTU').
+ * dU: mU x b unit lower (mU = b + rows of U21), dR12: b x nY2 (ld ldr),
+ * dY2: r x nY2 (ld ldy), dG: r x mU (ld ldg). */
+int hqrrp_downdate_y_dev(hqrrp_handle_t h, int r, int b, int mU, int nY2,
+                         const double *dU, int ldu, const double *dR12,
+                         int ldr, const double *dT, int ldt, double *dY2,
+                         int ldy, double *dG, int ldg);
+
+/* General FP64 matrix product on the matrix cores:
+ * C := alpha*op(A)*op(B) + beta*C, op = transpose when ta/tb != 0. */
+int hqrrp_dgemm_dev(hqrrp_handle_t h, int ta, int tb, int M, int N, int K,
+                    double alpha, const double *dA, int lda, const double *dB,
+                    int ldb, double beta, double *dC, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+"""The C ABI: libhqrrp_amd.so loads, exports every function that
+include/hqrrp_amd.h declares, and its host-side logic (argument checks,
+workspace query, Omega generation) behaves like the reference's -- all
+without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import hqrrp_amd
+from hqrrp_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include",
+                      "hqrrp_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(",
+                       src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while")))
+
+
+def test_header_symbols_exported():
+    L = C.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert "dgeqp4" in names and "NoFLA_HQRRP_WY_blk_var4" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # The Python binding covers every declared function too.
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_default_options_are_reference_defaults():
+    o = _lib.Options()
+    hqrrp_amd.lib().hqrrp_default_options(C.byref(o))
+    assert (o.nb, o.pp, o.panel_pivoting, o.seed) == (64, 10, 1, 12)
+    assert (o.sketch_prec, o.omega_kind, o.max_rank) == (0, 0, 0)
+
+
+def _call_dgeqp4(m, n, lda, lwork):
+    A = np.zeros(max(1, lda * max(n, 1)))
+    jp = np.zeros(max(1, n), np.int32)
+    tau = np.zeros(max(1, n))
+    work = np.zeros(max(1, lwork, 4))
+    info = C.c_int(99)
+    hqrrp_amd.lib().dgeqp4(C.byref(C.c_int(m)), C.byref(C.c_int(n)),
+                           C.c_void_p(A.ctypes.data), C.byref(C.c_int(lda)),
+                           C.c_void_p(jp.ctypes.data), C.c_void_p(tau.ctypes.data),
+                           C.c_void_p(work.ctypes.data), C.byref(C.c_int(lwork)),
+                           C.byref(info))
+    return info.value, work[0]
+
+
+def test_dgeqp4_argument_errors():
+    # Same info codes as the reference (NoFLA_HQRRP_WY_blk_var4.c:149-182).
+    assert _call_dgeqp4(-1, 5, 1, 100)[0] == -1
+    assert _call_dgeqp4(5, -1, 5, 100)[0] == -2
+    assert _call_dgeqp4(5, 5, 4, 100)[0] == -4
+    assert _call_dgeqp4(5, 5, 5, 15)[0] == -8  # lwork < 3n+1
+
+
+def test_dgeqp4_workspace_query_and_quick_return():
+    info, w0 = _call_dgeqp4(10, 8, 10, -1)
+    assert info == 0 and w0 == 2 * 8 + 9 * 32
+    info, w0 = _call_dgeqp4(0, 8, 1, 100)
+    assert info == 0 and w0 == 1.0
+
+
+def test_driver_argument_errors():
+    L = hqrrp_amd.lib()
+    buf = np.zeros(4)
+    ip = np.zeros(4, np.int32)
+    vp = C.c_void_p(buf.ctypes.data)
+    assert L.NoFLA_HQRRP_WY_blk_var4(-1, 2, vp, 1, C.c_void_p(ip.ctypes.data),
+                                     vp, 64, 10, 1) == -1
+    assert L.NoFLA_HQRRP_WY_blk_var4(4, 2, vp, 3, C.c_void_p(ip.ctypes.data),
+                                     vp, 64, 10, 1) == -1
+    # Quick return for an empty matrix, as the reference.
+    assert L.NoFLA_HQRRP_WY_blk_var4(0, 2, vp, 1, C.c_void_p(ip.ctypes.data),
+                                     vp, 64, 10, 1) == 0

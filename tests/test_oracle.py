@@ -1,0 +1,221 @@
+"""Pins the oracle (oracle/hqrrp_oracle.c) before anything is compared with it.
+
+1. Known answers: glibc rand() after srand(12), the first Omega values, and
+   the reference example driver's printed output (simple_test.c).
+2. Golden fixtures written by tests/golden/make_golden.py from the compiled
+   reference.
+3. The compiled reference itself (oracle/_ref), bitwise, when it was built.
+4. What the mathematics fixes, independent of the reference: backward error
+   and orthogonality; classic QRCP when nb >= n (LAPACK dgeqp3's pivots);
+   the sketch invariant Y_R = G_R A_R after a downdate; the first-index tie
+   rule.  Each check is shown to fail on a planted mistake.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from hqrrp_amd import matgen
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+EPS = np.finfo(np.float64).eps
+HAVE_REF = oracle.ref() is not None
+
+
+# ------------------------------------------------------------ known answers
+def test_rand_stream_known_answer():
+    s = oracle.RandStream(12)
+    assert [s.next() for _ in range(4)] == [1687063760, 945274514, 247215794,
+                                           1768547008]
+
+
+def test_rand_stream_equals_libc():
+    import ctypes
+
+    libc = ctypes.CDLL(None)
+    libc.srand(12)
+    want = [libc.rand() for _ in range(1000)]
+    s = oracle.RandStream(12)
+    assert [s.next() for _ in range(1000)] == want
+
+
+def test_omega_known_answer():
+    G = oracle.omega(74, 3)
+    first = G.ravel(order="F")[:6]
+    kat = [1.4365379616796561, -0.30089934168138049, 0.39098155517110622,
+           -0.76802946418308116, -1.9622006141490582, 0.17164440831200448]
+    assert np.array_equal(first, np.array(kat))
+
+
+def test_omega_fixture_and_product_generator():
+    import hqrrp_amd
+
+    G = np.load(os.path.join(GOLD, "omega_74x64.npz"))["G"]
+    assert np.array_equal(oracle.omega(74, 64), G)
+    # The product library's own generator (no GPU needed) agrees too.
+    assert np.array_equal(hqrrp_amd.reference_omega(74, 64), G)
+    assert np.array_equal(hqrrp_amd.reference_omega(74, 2000),
+                          oracle.omega(74, 2000))
+
+
+def test_simple_test_known_answer():
+    """simple_test.c's printout, recorded from the reference run: 7x5 integer
+    matrix, fixed columns {2,3,5}, dgeqpf branch of dgeqp4."""
+    d = np.load(os.path.join(GOLD, "simple_test_7x5.npz"))
+    Af, jp, tau, info, _ = oracle.dgeqp4(d["A"], d["jpvt_in"])
+    assert info == 0
+    assert list(jp) == [2, 3, 5, 1, 4]
+    np.testing.assert_allclose(tau, [1.473286, 1.981109, 1.731670, 1.173021,
+                                     1.384567], atol=5e-7)
+    np.testing.assert_allclose(np.diag(Af), [-29.58040, -8.851215, -11.74025,
+                                             -4.940191, -4.915331], atol=5e-6)
+
+
+# ------------------------------------------------------------ fixtures
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "hqrrp_*.npz"))),
+                         ids=os.path.basename)
+def test_hqrrp_fixture(path):
+    d = np.load(path)
+    m, n, nb, pp, pv, _ = [int(x) for x in d["params"]]
+    Af, jp, tau = oracle.hqrrp(d["A"], nb, pp, pv)
+    assert np.array_equal(jp, d["jpvt"])
+    scale = np.abs(d["A"]).max()
+    assert np.max(np.abs(Af - d["A_out"])) <= 1e-12 * scale * np.sqrt(m)
+    assert np.max(np.abs(tau - d["tau"])) <= 1e-12
+
+
+def test_qrpmod_fixtures():
+    d = np.load(os.path.join(GOLD, "qrpmod_sketch74x600.npz"))
+    r = oracle.qrpmod(d["Y"], pivoting=1, num_stages=64)
+    assert np.array_equal(r["p"], d["p"])
+    d = np.load(os.path.join(GOLD, "qrpmod_panel300x64.npz"))
+    r = oracle.qrpmod(d["P"], pivoting=1, num_stages=-1, build_T=True)
+    assert np.array_equal(r["p"], d["p"])
+    np.testing.assert_allclose(r["A"], d["A_out"], atol=1e-12 * np.abs(d["P"]).max() * 20)
+    np.testing.assert_allclose(r["T"], d["T"], atol=1e-11)
+
+
+def test_dgeqp4_fixture_fixed_columns():
+    d = np.load(os.path.join(GOLD, "dgeqp4_1000_fixed3.npz"))
+    A = matgen.gaussian(1000, 1000, int(d["seed"][0]))
+    assert np.allclose([A.sum(), np.abs(A).sum()], d["A_checksum"], rtol=0, atol=1e-9)
+    jp0 = np.zeros(1000, np.int32)
+    jp0[d["fixed"]] = 1
+    Af, jp, tau, info, w0 = oracle.dgeqp4(A, jp0)
+    assert info == 0 and w0 == d["work0"][0]
+    assert np.array_equal(jp, d["jpvt"])
+    assert list(jp[:3]) == [1, 6, 18]  # fixed columns first, in order
+    np.testing.assert_allclose(np.abs(np.diag(Af)), np.abs(d["diagR"]),
+                               rtol=1e-10)
+
+
+# ------------------------------------------------------------ live reference
+REF_CASES = [(300, 200, 32, 8, 1), (200, 300, 32, 8, 1), (257, 257, 16, 4, 0),
+             (2000, 2000, 64, 10, 1), (130, 70, 64, 10, 1), (500, 129, 64, 10, 1)]
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built here")
+@pytest.mark.parametrize("m,n,nb,pp,pv", REF_CASES)
+def test_oracle_bitwise_equals_reference(m, n, nb, pp, pv):
+    A = matgen.gaussian(m, n, m + n)
+    a1, j1, t1 = oracle.hqrrp(A, nb, pp, pv)
+    a2, j2, t2 = oracle.hqrrp(A, nb, pp, pv, use_ref=True)
+    assert np.array_equal(j1, j2)
+    assert np.array_equal(a1, a2)
+    assert np.array_equal(t1, t2)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built here")
+def test_helpers_bitwise_equal_reference():
+    rng = np.random.default_rng(5)
+    Y = rng.standard_normal((74, 900))
+    assert np.array_equal(oracle.qrpmod(Y, 1, 64)["p"],
+                          oracle.qrpmod(Y, 1, 64, use_ref=True)["p"])
+    P = rng.standard_normal((700, 64))
+    a = oracle.qrpmod(P, 1, -1, build_T=True)
+    b = oracle.qrpmod(P, 1, -1, build_T=True, use_ref=True)
+    assert all(np.array_equal(a[k], b[k]) for k in ("A", "p", "tau", "T"))
+    U = a["A"]
+    T = a["T"]
+    R12 = rng.standard_normal((64, 300))
+    Y2 = rng.standard_normal((74, 300))
+    G = rng.standard_normal((74, 700))
+    y1, g1 = oracle.downdate_y(U, R12, Y2, T, G)
+    y2, g2 = oracle.downdate_y(U, R12, Y2, T, G, use_ref=True)
+    assert np.array_equal(y1, y2) and np.array_equal(g1, g2)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built here")
+def test_dgeqp4_bitwise_equals_reference():
+    rng = np.random.default_rng(8)
+    for m, n, fixed in [(1000, 1000, [0, 5, 17]), (1200, 1000, []),
+                        (300, 300, [1, 2]), (100, 50, [3]), (0, 5, [])]:
+        A = rng.standard_normal((m, n))
+        jp = np.zeros(n, np.int32)
+        jp[fixed] = 1
+        r1 = oracle.dgeqp4(A, jp)
+        r2 = oracle.dgeqp4(A, jp, use_ref=True)
+        for x, y in zip(r1[:3], r2[:3]):
+            assert np.array_equal(x, y)
+        assert r1[3] == r2[3]
+
+
+# ------------------------------------------------------------ mathematics
+@pytest.mark.parametrize("m,n,nb,pp,fam", [(400, 300, 32, 8, "gaussian"),
+                                           (300, 400, 32, 8, "graded"),
+                                           (600, 600, 64, 10, "lowrank")])
+def test_backward_error_and_orthogonality(m, n, nb, pp, fam):
+    A = {"gaussian": matgen.gaussian, "graded": matgen.graded,
+         "lowrank": lambda a, b, s: matgen.lowrank_noise(a, b, 50, s)}[fam](m, n, 3)
+    Af, jp, tau = oracle.hqrrp(A, nb, pp, 1)
+    res, orth = oracle.residuals(A, Af, jp, tau)
+    assert res < 50 * max(m, n) * EPS and orth < 50 * max(m, n) * EPS
+    assert sorted(jp) == list(range(1, n + 1))
+    # Planted mistake: two pivots exchanged -> the residual check fails.
+    bad = jp.copy()
+    bad[[0, 1]] = bad[[1, 0]]
+    res_bad, _ = oracle.residuals(A, Af, bad, tau)
+    assert res_bad > 1e-3
+
+
+def test_nb_ge_n_is_classic_qrcp():
+    """With nb >= n only the panel QRCP runs: LAPACK dgeqp3's pivots."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((90, 40)) * np.logspace(0, 3, 40)
+    _, jp, _ = oracle.hqrrp(A, 64, 10, 1)
+    _, jp3, _, info = oracle.dgeqp3(A)
+    assert info == 0
+    assert np.array_equal(jp, jp3)
+    # Greedy by hand: the first pivot is the column of largest norm.
+    assert jp[0] - 1 == int(np.argmax(np.linalg.norm(A, axis=0)))
+
+
+def _downdate_case(rng, m=300, n=260, b=32, r=40, planted=False):
+    A = rng.standard_normal((m, n))
+    G = rng.standard_normal((r, m))
+    Y = G @ A
+    fac = oracle.qrpmod(A[:, :b], pivoting=0, num_stages=-1, build_T=True)
+    U, T = fac["A"], fac["T"]
+    if planted:
+        T = T.T.copy()  # planted mistake: T transposed
+    Ap = oracle.apply_q_left_t(U, fac["T"], A[:, b:])
+    Y2, Gn = oracle.downdate_y(U, Ap[:b, :], Y[:, b:], T, G)
+    return np.linalg.norm(Y2 - Gn[:, b:] @ Ap[b:, :]) / np.linalg.norm(Y)
+
+
+def test_downdate_keeps_sketch_invariant():
+    """Y_R = G_R A_R after the downdate (the reference's own
+    CHECK_DOWNDATING_OF_Y test, NoFLA_HQRRP_WY_blk_var4.c:457-490)."""
+    rng = np.random.default_rng(12)
+    assert _downdate_case(rng) < 1e-13
+    assert _downdate_case(np.random.default_rng(12), planted=True) > 1e-3
+
+
+def test_first_index_tie_rule():
+    base = np.random.default_rng(2).standard_normal((20, 6))
+    A = np.asfortranarray(np.concatenate([base, base], axis=1))
+    r = oracle.qrpmod(A, pivoting=1, num_stages=1)
+    c = int(np.argmax(np.linalg.norm(base, axis=0)))
+    assert r["p"][0] == c + 1  # the first of the two equal columns
