@@ -333,30 +333,49 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
 }
 
 // [A12; A22] := (I - V T V')' [A12; A22]  (dlarfb Left/Transpose):
-// M = V'C (split-K), W = T'M, C -= V W.
+// M = V'C (split-K), W = T'M, C -= V W  (k_trail.hip).
 int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
                   long ldv, const double* T, long ldt, double* C, long ldc) {
   if (nc <= 0 || mrows <= 0) return HQRRP_OK;
-  HQ_RET(ensure(h->Mx, sizeof(double) * (size_t)b * nc));
+  TrailArgs a{};
+  a.mrows = mrows;
+  a.nc = nc;
+  a.b = b;
+  a.V = V;
+  a.ldv = ldv;
+  a.C = C;
+  a.ldc = ldc;
+  a.T = T;
+  a.ldt = ldt;
+  // Split K so that >= 2 workgroups per CU are busy, slices >= 256 rows.
+  const long tiles = hq_cdiv(nc, 64);
+  const int kbk = trail_kbk();
+  long S = std::max(1L, (2L * h->cu_count + tiles - 1) / tiles);
+  S = std::min(S, std::max(1L, (long)mrows / 256));
+  long kc = (mrows + S - 1) / S;
+  kc = (kc + kbk - 1) / kbk * kbk;
+  S = (mrows + kc - 1) / kc;
+  a.splits = (int)S;
+  a.kchunk = kc;
+  HQ_RET(ensure(h->split, sizeof(double) * (size_t)S * b * nc));
   HQ_RET(ensure(h->W2, sizeof(double) * (size_t)b * nc));
-  double* Mx = h->Mx.as<double>();
-  double* W2 = h->W2.as<double>();
+  a.part = h->split.as<double>();
+  a.W = h->W2.as<double>();
+  a.ldw = b;
   const double mn = (double)mrows * nc;
   {
     PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * mrows * b, 1);
-    HQ_RET(gemm(h, true, false, b, nc, mrows, 1.0, V, ldv, 1, C, ldc, 0, 0.0,
-                Mx, b, true, 1));
+    HQ_TRY(launch_trail_vtc(a, h->stream));
   }
   {
-    PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc, 16.0 * b * nc, 1);
-    HQ_RET(gemm(h, true, false, b, nc, b, 1.0, T, ldt, 0, Mx, b, 0, 0.0, W2, b,
-                false));
+    PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc, 8.0 * (S + 1.0) * b * nc,
+                  1);
+    HQ_TRY(launch_trail_tw(a, h->stream));
   }
   {
     // C is read and written once; V and W are small.
     PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * mn, 16.0 * mn, 1);
-    HQ_RET(gemm(h, false, false, mrows, nc, b, -1.0, V, ldv, 1, W2, b, 0, 1.0,
-                C, ldc, false, 2));
+    HQ_TRY(launch_trail_update(a, h->stream));
   }
   return HQRRP_OK;
 }

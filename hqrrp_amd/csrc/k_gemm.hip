@@ -13,7 +13,8 @@
 // Tiling: 64x64 output tile per 256-thread workgroup, K staged through LDS
 // in steps of 16; each wavefront owns a 32x32 sub-tile computed with 2x2
 // v_mfma_f64_16x16x4f64 (A fragment: lane l holds A[l%16][l/16]; B fragment:
-// B[l/16][l%16]; accumulator element r of lane l is C[4(l/16)+r][l%16]).
+// B[l/16][l%16]; accumulator element r of lane l is C[l/16 + 4r][l%16] --
+// the f64 accumulator is row-strided, unlike the f32 16x16 layouts).
 // With split-K (gridDim.z > 1) each slice writes alpha*partial to its own
 // M x N buffer and hq_reduce_splitk adds the slices in a fixed order, so
 // results are deterministic.
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(HQ_NT) void gemm_f64_kernel(GemmArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        long gm = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        long gm = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
         long gn = n0 + wn * 32 + j * 16 + (lane & 15);
         if (gm < g.M && gn < g.N) {
           double v = g.alpha * acc[i][j][r];

@@ -43,6 +43,26 @@ struct GemmArgs {
   double* part;      // split-K output slices (M x N each), or null
 };
 
+struct TrailArgs {
+  int mrows, nc, b, splits;
+  const double* V;  // mrows x b reflectors (unit lower trapezoidal), ld ldv
+  long ldv;
+  double* C;  // mrows x nc trailing matrix, ld ldc
+  long ldc;
+  double* W;  // b x nc, ld ldw: T' V' C
+  long ldw;
+  const double* T;  // b x b upper triangular, ld ldt
+  long ldt;
+  double* part;  // splits slices of b x nc
+  long kchunk;   // rows of C per split
+};
+
+// k_trail.hip
+hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
+hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);
+hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
+int trail_kbk();
+
 // k_sketch.hip
 size_t sketch_cand_doubles(int nw, int rmax);
 int sketch_rmax(int r);
