@@ -18,16 +18,33 @@ enum {
 // sleep, about 1 us: a few seconds in total).
 #define HQ_SPIN_LIMIT (1u << 22)
 
+// Cross-workgroup data of the persistent kernels is exchanged with
+// agent-scope (device-coherent) atomic stores and loads, which write through
+// to and read from the coherence point instead of the per-XCD L2.
+__device__ __forceinline__ void hq_publish(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double hq_fetch(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Grid-wide barrier for persistent kernels whose grid the host has sized to
 // be co-resident.  bar[0] counts arrivals and only grows within a launch
 // (barrier number `phase` completes at phase*nblocks arrivals); bar[1] counts
 // workgroups leaving the kernel so the last one out can zero both words for
 // the next launch on the stream.  Returns false (in every thread) if the wait
 // timed out or another workgroup reported a timeout.
+//
+// Precondition: every global store that another workgroup reads after this
+// barrier was issued by wavefront 0 (the wavefront of thread 0), so thread
+// 0's release (which waits for its wavefront's stores) orders all of them
+// before the arrival; thread 0's acquire, followed by the workgroup barrier,
+// orders every thread's later loads after all arrivals.  Only thread 0 pays
+// for the release/acquire cache maintenance.
 __device__ __forceinline__ bool hq_grid_sync(unsigned* bar, unsigned nblocks,
                                              unsigned& phase, int* err) {
   __shared__ int s_ok;
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     phase += 1;
@@ -35,7 +52,9 @@ __device__ __forceinline__ bool hq_grid_sync(unsigned* bar, unsigned nblocks,
     __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     int ok = 1;
     unsigned spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+    // Poll with relaxed loads (an acquire load would invalidate the caches
+    // on every poll); one acquire fence once the count is reached.
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
            target) {
       if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         ok = 0;
@@ -49,10 +68,10 @@ __device__ __forceinline__ bool hq_grid_sync(unsigned* bar, unsigned nblocks,
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
-  __threadfence();
   return s_ok != 0;
 }
 

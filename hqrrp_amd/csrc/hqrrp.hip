@@ -142,7 +142,6 @@ struct hqrrp_handle_s {
   hipStream_t own = nullptr, stream = nullptr;
   int cu_count = 0;
   size_t lds_max = 0;
-  int panel_rows = 256;  // rows of the panel held by one workgroup
   int sketch_max_wg = 0, panel_max_wg = 0;
   DevBuf words;  // [0..1] sketch barrier, [2..3] panel barrier, [8] error
   DevBuf Y, G, Gcache, cand, moves, piv, ppart, rowbuf, gram, T, perm;
@@ -269,11 +268,9 @@ int gemm(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K, double alpha,
   return HQRRP_OK;
 }
 
-int panel_geometry(hqrrp_handle_t h, int mrows, int* R, int* nw) {
-  int rows = h->panel_rows;
-  int n = hq_cdiv(mrows, rows);
+int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
+  int n = hq_cdiv(mrows, panel_rows_per_wg());
   if (n > h->panel_max_wg) return HQRRP_ERR_UNSUPPORTED;
-  *R = rows;
   *nw = n;
   return HQRRP_OK;
 }
@@ -304,8 +301,7 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
               int pivoting, double* dtau, int* dperm, double* dT, long ldt) {
   if (b < 1 || b > 64 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
   PanelArgs a{};
-  HQ_RET(panel_geometry(h, mrows, &a.R, &a.nw));
-  a.ldR = a.R + 1;
+  HQ_RET(panel_geometry(h, mrows, &a.nw));
   a.P = dP;
   a.ldp = ldp;
   a.mrows = mrows;
@@ -361,7 +357,7 @@ int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   HQ_RET(ensure(h->W2, sizeof(double) * (size_t)b * nc));
   a.part = h->split.as<double>();
   a.W = h->W2.as<double>();
-  a.ldw = b;
+  a.ldw = nc;
   const double mn = (double)mrows * nc;
   {
     PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * mrows * b, 1);
@@ -538,13 +534,6 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
   HQ_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                device));
   h->lds_max = (size_t)lds;
-  if (const char* s = getenv("HQRRP_PANEL_ROWS")) h->panel_rows = atoi(s);
-  // The slab (64 columns x (R+1) doubles) plus ~8 KB of static LDS must fit.
-  while (h->panel_rows > 8 &&
-         (size_t)64 * (h->panel_rows + 1) * 8 + 8192 > h->lds_max)
-    h->panel_rows /= 2;
-  size_t slab = (size_t)64 * (h->panel_rows + 1) * 8;
-  HQ_TRY(panel_set_lds_limit(slab));
   // Persistent kernels need all their workgroups co-resident; both kernels
   // are launched at most one workgroup per CU (sketch: register-bound,
   // panel: LDS-bound).

@@ -7,11 +7,13 @@
 // remaining columns, the LAWN-176 norm downdate (912-986), and at the end
 // dlarft('Forward','Columnwise') (875-878).
 //
-// Design (MI355X): one persistent launch per panel.  Workgroup w owns a slab
-// of R consecutive rows of all b columns, resident in LDS for the whole
-// factorisation (R <= 240 rows x 64 columns = 120 KB), so the panel is read
-// from and written to HBM exactly once.  Each step needs two grid-wide
-// reductions:
+// Design (MI355X): one persistent launch per panel.  Workgroup w owns 256
+// rows of all b columns, resident in registers for the whole factorisation
+// (thread = one column x 32 consecutive rows), so the panel is read from and
+// written to HBM exactly once and every per-step loop runs on registers; LDS
+// only stages the coalesced load/store and broadcasts the reflector.  At
+// 16384 rows that is 64 workgroups; below 256 rows a single workgroup needs
+// no grid barrier at all.  Each step needs two grid-wide reductions:
 //   A: the dot products of the reflector with every column (the dlarf
 //      vector w for the pending columns and, for the columns already
 //      factored, the inner products dlarft needs -- same pass, same barrier);
@@ -20,7 +22,7 @@
 //      together with rows k and k+1 themselves.
 // Reductions are done in a fixed order by every workgroup, so all of them
 // hold bit-identical pivot/reflector state without further communication.
-// Columns stay where they are in LDS; a position->column map records the
+// Columns stay with their threads; a position->column map records the
 // pivoting and the slab is written back in pivoted order.
 #include "common.h"
 #include "kernels.h"
@@ -29,24 +31,72 @@ namespace {
 
 constexpr int kMaxB = 64;
 
-__global__ __launch_bounds__(HQ_NT) void panel_qrcp_kernel(PanelArgs a) {
-  extern __shared__ double slab[];  // kMaxB columns x ldR doubles
+// Sum of src[q*stride] for q in [q0, q1): loads issued 16 at a time so a
+// cross-workgroup reduction costs ~one memory round trip per 16 partials;
+// the summation order is fixed, so every workgroup gets the same bits.
+__device__ __forceinline__ double chunked_sum(const double* src, long stride,
+                                              int q0, int q1) {
+  double s = 0.0;
+  for (int base = q0; base < q1; base += 16) {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      v[i] = (base + i < q1) ? hq_fetch(src + (long)(base + i) * stride) : 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += v[i];
+  }
+  return s;
+}
+
+constexpr int kPT = 512;          // threads of the panel kernel
+constexpr int kPG = kPT / 64;     // row groups (one per wavefront)
+constexpr int kRT = 32;           // rows per thread (per row group)
+constexpr int kPR = kRT * kPG;    // panel rows held by one workgroup
+
+// Thread (c, g) = (t % 64, t / 64) holds column c of the 64 consecutive rows
+// row0 + 64g + i, i < 64, in registers x[i]; rows beyond mrows are zero.
+// Relative to the current step k, a wavefront's rows are either all above
+// k (finished: skipped), all at or below k+2 (branch-free fast path), or they
+// straddle k, k+1 (masked path, at most two wavefronts per step).
+__global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
+  __shared__ double stage[kMaxB][129];  // staging of 128 rows x 64 columns
+  __shared__ double s_v[kPR];           // reflector on this WG's rows
+  __shared__ double s_red[kPG][kMaxB], s_red2[kPG][kMaxB];
   __shared__ double s_d[kMaxB], s_e[kMaxB], s_S2[kMaxB], s_rowk[kMaxB];
-  __shared__ double s_full[kMaxB];
-  __shared__ double s_red[4][kMaxB], s_red2[4][kMaxB];
-  __shared__ int s_phys[kMaxB], s_done[kMaxB];
+  __shared__ double s_full[kMaxB], s_rk[kMaxB], s_rk1[kMaxB];
+  __shared__ int s_phys[kMaxB], s_done[kMaxB], s_pos[kMaxB];
   __shared__ double s_tau, s_beta, s_scale;
   __shared__ int s_pc;
 
   const int t = threadIdx.x, w = blockIdx.x;
-  const int b = a.b, R = a.R, ldR = a.ldR, nw = a.nw;
-  const long row0 = (long)w * R;
-  const int nrows = (int)min((long)R, (long)a.mrows - row0);
-  const int cc = t & 63, g = t >> 6;  // column / row-group of this thread
+  const int b = a.b, nw = a.nw;
+  const long row0 = (long)w * kPR;
+  const int cc = t & 63, g = t >> 6;
+  const bool col_ok = cc < b;
+  const long glo = row0 + kRT * g;  // first global row of my group
+  // Position of global row kk inside my group, clamped to [-1, 64]:
+  // -1: kk is above my rows, 64: kk is below them.
+  auto rel = [&](long kk) -> int {
+    long v = kk - glo;
+    return (int)(v < -1 ? -1 : (v > kRT ? kRT : v));
+  };
 
-  for (int idx = t; idx < R * b; idx += HQ_NT) {
-    int i = idx % R, c = idx / R;
-    slab[c * ldR + i] = (i < nrows) ? a.P[(row0 + i) + (long)c * a.ldp] : 0.0;
+  // ---- load, 128 rows per round, coalesced along the rows of a column.
+  double x[kRT];
+#pragma unroll
+  for (int rd = 0; rd < kPR / 128; ++rd) {
+#pragma unroll
+    for (int q = 0; q < 128 * kMaxB / kPT; ++q) {
+      const int idx = t + kPT * q, rl = idx & 127, c = idx >> 7;
+      const long gr = row0 + rd * 128 + rl;
+      stage[c][rl] = (c < b && gr < a.mrows) ? a.P[gr + (long)c * a.ldp] : 0.0;
+    }
+    __syncthreads();
+    if (g / (128 / kRT) == rd) {
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) x[i] = stage[cc][(g % (128 / kRT)) * kRT + i];
+    }
+    __syncthreads();
   }
   if (t < kMaxB) {
     s_phys[t] = t;
@@ -55,170 +105,214 @@ __global__ __launch_bounds__(HQ_NT) void panel_qrcp_kernel(PanelArgs a) {
   __syncthreads();
 
   unsigned phase = 0;
-  double* partA = a.part;                 // nw x 64
+  double* partA = a.part;                  // nw x 64
   double* partB = a.part + (long)nw * 64;  // nw x 128
+  const int per = (nw + kPG - 1) / kPG;
+  const int q0 = min(nw, g * per), q1 = min(nw, q0 + per);
 
-  // Sums of squares of pending columns over rows >= kk+1 (S1) and >= kk+2
-  // (S2), plus rows kk and kk+1 published by their owners.
-  auto norm_pass = [&](int kk) {
-    if (cc < b && !s_done[cc]) {
-      double s1 = 0.0, s2 = 0.0;
-      const double* col = slab + cc * ldR;
-      for (int i = g; i < nrows; i += 4) {
-        long gi = row0 + i;
-        double v = col[i];
-        if (gi >= kk + 1) s1 += v * v;
-        if (gi >= kk + 2) s2 += v * v;
+  // Squared norms of my column over rows >= kk+1 (S1) and >= kk+2 (S2), and
+  // rows kk, kk+1 (to s_rk / s_rk1); wavefront 0 combines and publishes.
+  auto norm_pass = [&](long kk) {
+    double s1 = 0.0, s2 = 0.0;
+    const int r = rel(kk);
+    if (r < 0) {  // every row of mine is >= kk+1; S2 differs if kk+1 is mine
+      if (rel(kk + 1) < 0) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) s1 += x[i] * x[i];
+        s2 = s1;
+      } else {  // kk+1 is my first row
+#pragma unroll
+        for (int i = 1; i < kRT; ++i) s2 += x[i] * x[i];
+        s1 = s2 + x[0] * x[0];
+        s_rk1[cc] = x[0];
       }
-      s_red[g][cc] = s1;
-      s_red2[g][cc] = s2;
-    } else if (cc < kMaxB) {
-      s_red[g][cc] = 0.0;
-      s_red2[g][cc] = 0.0;
+    } else if (r < kRT) {  // straddle: masked path
+      double rk = 0.0, rk1 = 0.0;
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) {
+        const double v2 = x[i] * x[i];
+        s1 += (i > r) ? v2 : 0.0;
+        s2 += (i > r + 1) ? v2 : 0.0;
+        rk = (i == r) ? x[i] : rk;
+        rk1 = (i == r + 1) ? x[i] : rk1;
+      }
+      s_rk[cc] = rk;
+      if (r + 1 < kRT) s_rk1[cc] = rk1;
     }
-    if (t < b && !s_done[t]) {
-      long lk = kk - row0, lk1 = kk + 1 - row0;
-      if (lk >= 0 && lk < nrows) a.rowbuf[t] = slab[t * ldR + lk];
-      if (lk1 >= 0 && lk1 < nrows) a.rowbuf[kMaxB + t] = slab[t * ldR + lk1];
-    }
+    s_red[g][cc] = s1;
+    s_red2[g][cc] = s2;
     __syncthreads();
-    if (t < b) {
-      partB[(long)w * 128 + t] =
-          ((s_red[0][t] + s_red[1][t]) + s_red[2][t]) + s_red[3][t];
-      partB[(long)w * 128 + 64 + t] =
-          ((s_red2[0][t] + s_red2[1][t]) + s_red2[2][t]) + s_red2[3][t];
+    if (t < b) {  // wavefront 0 publishes (hq_grid_sync precondition)
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) {
+        a1 += s_red[q][t];
+        a2 += s_red2[q][t];
+      }
+      hq_publish(partB + (long)w * 128 + t, a1);
+      hq_publish(partB + (long)w * 128 + 64 + t, a2);
+      const long lk = kk - row0, lk1 = kk + 1 - row0;
+      if (lk >= 0 && lk < kPR) hq_publish(a.rowbuf + t, s_rk[t]);
+      if (lk1 >= 0 && lk1 < kPR) hq_publish(a.rowbuf + kMaxB + t, s_rk1[t]);
     }
   };
-  // Reduce partB in a fixed order; returns S1 in s_full, S2 in s_red2[0].
   auto reduce_B = [&]() {
-    if (cc < b) {
-      double s1 = 0.0, s2 = 0.0;
-      for (int q = g; q < nw; q += 4) {
-        s1 += partB[(long)q * 128 + cc];
-        s2 += partB[(long)q * 128 + 64 + cc];
-      }
-      s_red[g][cc] = s1;
-      s_red2[g][cc] = s2;
+    if (col_ok) {
+      s_red[g][cc] = chunked_sum(partB + cc, 128, q0, q1);
+      s_red2[g][cc] = chunked_sum(partB + 64 + cc, 128, q0, q1);
     }
     __syncthreads();
     if (t < b) {
-      s_full[t] = ((s_red[0][t] + s_red[1][t]) + s_red[2][t]) + s_red[3][t];
-      s_S2[t] = ((s_red2[0][t] + s_red2[1][t]) + s_red2[2][t]) + s_red2[3][t];
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) {
+        a1 += s_red[q][t];
+        a2 += s_red2[q][t];
+      }
+      s_full[t] = a1;
+      s_S2[t] = a2;
     }
     __syncthreads();
   };
 
-  // Initial column norms (k = -1): S1 = full squared norms, S2 = rows >= 1.
+  // Initial column norms (kk = -1): S1 = full squared norms, S2 = rows >= 1.
   norm_pass(-1);
   if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
   reduce_B();
   if (t < b) {
     s_d[t] = sqrt(s_full[t]);
     s_e[t] = s_d[t];
-    s_rowk[t] = a.rowbuf[kMaxB + t];
+    s_rowk[t] = hq_fetch(a.rowbuf + kMaxB + t);
   }
   __syncthreads();
 
   for (int k = 0; k < b; ++k) {
-    // Phase 1: pivot choice and reflector (serial, identical in every WG).
-    if (t == 0) {
+    // Phase 1: pivot (wavefront 0: largest partial norm, first position on
+    // ties, as idamax) and reflector; identical in every workgroup.
+    if (t < 64) {
       int p = k;
       if (a.pivoting) {
-        double best = s_d[s_phys[k]];
-        for (int q = k + 1; q < b; ++q) {
-          double v = s_d[s_phys[q]];
-          if (v > best) {
-            best = v;
-            p = q;
+        double v = (t >= k && t < b) ? s_d[s_phys[t]] : -1.0;
+        int q = (t >= k && t < b) ? t : 1 << 30;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          double ov = __shfl_xor(v, off, 64);
+          int oq = __shfl_xor(q, off, 64);
+          if (ov > v || (ov == v && oq < q)) {
+            v = ov;
+            q = oq;
           }
         }
+        p = q;
       }
-      int tmp = s_phys[k];
-      s_phys[k] = s_phys[p];
-      s_phys[p] = tmp;
-      const int pc = s_phys[k];
-      s_pc = pc;
-      s_done[pc] = 1;
-      hq_reflector h = hq_dlarfg(s_rowk[pc], (a.mrows - k > 1) ? s_S2[pc] : 0.0);
-      s_tau = h.tau;
-      s_beta = h.beta;
-      s_scale = h.scale;
-      if (w == 0) a.tau[k] = h.tau;
+      if (t == 0) {
+        int tmp = s_phys[k];
+        s_phys[k] = s_phys[p];
+        s_phys[p] = tmp;
+        const int pc = s_phys[k];
+        s_pc = pc;
+        s_done[pc] = 1;
+        hq_reflector h =
+            hq_dlarfg(s_rowk[pc], (a.mrows - k > 1) ? s_S2[pc] : 0.0);
+        s_tau = h.tau;
+        s_beta = h.beta;
+        s_scale = h.scale;
+        if (w == 0) a.tau[k] = h.tau;
+      }
     }
     __syncthreads();
     const int pc = s_pc;
     const double tau = s_tau;
-    // Phase 2: the pivot column becomes the reflector (unit entry implied).
-    {
-      double* col = slab + pc * ldR;
+    const int r = rel(k);  // k relative to my group (-1: above, kRT: below)
+    const bool active = r < kRT;  // some of my rows are >= k
+    double* vg = s_v + kRT * g;
+    // Phase 2: the pivot column becomes the reflector; its values on my rows
+    // go to LDS (1 at row k, 0 above).
+    if (cc == pc && active) {
       const double sc = s_scale, be = s_beta;
-      for (int i = t; i < nrows; i += HQ_NT) {
-        long gi = row0 + i;
-        if (gi > k)
-          col[i] *= sc;
-        else if (gi == k)
-          col[i] = be;
-      }
-    }
-    __syncthreads();
-    // Phase 3: partial dots v'*col for every other column.
-    if (cc < b) {
-      double s = 0.0;
-      if (cc != pc) {
-        const double* v = slab + pc * ldR;
-        const double* col = slab + cc * ldR;
-        for (int i = g; i < nrows; i += 4) {
-          long gi = row0 + i;
-          if (gi >= k) s += ((gi == k) ? 1.0 : v[i]) * col[i];
+      if (r < 0) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) {
+          x[i] *= sc;
+          vg[i] = x[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) {
+          const bool below = i > r, diag = i == r;
+          x[i] = below ? x[i] * sc : (diag ? be : x[i]);
+          vg[i] = below ? x[i] : (diag ? 1.0 : 0.0);
         }
       }
+    }
+    __syncthreads();
+    // Phase 3: partial dots v'*col for every other column (rows >= k).
+    {
+      double s = 0.0;
+      if (active && col_ok && cc != pc) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) s += vg[i] * x[i];
+      }
       s_red[g][cc] = s;
     }
     __syncthreads();
-    if (t < b)
-      partA[(long)w * 64 + t] =
-          ((s_red[0][t] + s_red[1][t]) + s_red[2][t]) + s_red[3][t];
+    if (t < b) {  // wavefront 0 publishes
+      double a1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) a1 += s_red[q][t];
+      hq_publish(partA + (long)w * 64 + t, a1);
+    }
     if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
     // Phase 4: full dots in a fixed order.
-    if (cc < b) {
-      double s = 0.0;
-      for (int q = g; q < nw; q += 4) s += partA[(long)q * 64 + cc];
-      s_red[g][cc] = s;
-    }
+    if (col_ok) s_red[g][cc] = chunked_sum(partA + cc, 64, q0, q1);
     __syncthreads();
-    if (t < b)
-      s_full[t] = ((s_red[0][t] + s_red[1][t]) + s_red[2][t]) + s_red[3][t];
+    if (t < b) {
+      double a1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) a1 += s_red[q][t];
+      s_full[t] = a1;
+    }
     __syncthreads();
     if (w == 0 && t < k) a.gram[t + (long)k * b] = s_full[s_phys[t]];
     // Phase 5: rank-1 update of the pending columns over rows >= k.
-    if (tau != 0.0 && cc < b && !s_done[cc]) {
+    if (active && tau != 0.0 && col_ok && !s_done[cc]) {
       const double f = -tau * s_full[cc];
-      const double* v = slab + pc * ldR;
-      double* col = slab + cc * ldR;
-      for (int i = g; i < nrows; i += 4) {
-        long gi = row0 + i;
-        if (gi >= k) col[i] += ((gi == k) ? 1.0 : v[i]) * f;
-      }
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) x[i] += vg[i] * f;
     }
-    __syncthreads();
     if (k + 1 < b) {
+      __syncthreads();
       norm_pass(k);
       if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
       reduce_B();
       // Phase 6: norm downdate and the next step's alpha / ||x||^2.
       if (t < b && !s_done[t]) {
-        double wt = a.rowbuf[t];
+        double wt = hq_fetch(a.rowbuf + t);
         if (a.pivoting)
           hq_downdate_norm(s_d[t], s_e[t], wt, s_full[t], a.mrows - k - 1);
-        if (k + 1 < a.mrows) s_rowk[t] = a.rowbuf[kMaxB + t];
+        if (k + 1 < a.mrows) s_rowk[t] = hq_fetch(a.rowbuf + kMaxB + t);
       }
       __syncthreads();
     }
   }
-  // Write the slab back in pivoted order.
-  for (int idx = t; idx < R * b; idx += HQ_NT) {
-    int i = idx % R, q = idx / R;
-    if (i < nrows) a.P[(row0 + i) + (long)q * a.ldp] = slab[s_phys[q] * ldR + i];
+  // ---- write back in pivoted order (position q holds column s_phys[q]).
+  if (t < b) s_pos[s_phys[t]] = t;
+  __syncthreads();
+  const int myq = col_ok ? s_pos[cc] : 0;
+#pragma unroll
+  for (int rd = 0; rd < kPR / 128; ++rd) {
+    if (col_ok && g / (128 / kRT) == rd) {
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) stage[myq][(g % (128 / kRT)) * kRT + i] = x[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 128 * kMaxB / kPT; ++q) {
+      const int idx = t + kPT * q, rl = idx & 127, c = idx >> 7;
+      const long gr = row0 + rd * 128 + rl;
+      if (c < b && gr < a.mrows) a.P[gr + (long)c * a.ldp] = stage[c][rl];
+    }
+    __syncthreads();
   }
   if (w == 0 && t < b) a.perm[t] = s_phys[t];
   hq_grid_exit(a.bar, nw);
@@ -231,14 +325,20 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
                                                       const double* tau,
                                                       double* T, long ldt) {
   __shared__ double sT[kMaxB][kMaxB + 1];
-  __shared__ double sg[kMaxB];
+  __shared__ double sG[kMaxB][kMaxB + 1];
+  __shared__ double sg[kMaxB], stau[kMaxB];
   const int t = threadIdx.x;
   for (int idx = t; idx < kMaxB * (kMaxB + 1); idx += HQ_NT)
     (&sT[0][0])[idx] = 0.0;
+  for (int idx = t; idx < b * b; idx += HQ_NT) {
+    int i = idx % b, j = idx / b;
+    sG[i][j] = (i < j) ? gram[idx] : 0.0;
+  }
+  if (t < b) stau[t] = tau[t];
   __syncthreads();
   for (int k = 0; k < b; ++k) {
-    const double tk = tau[k];
-    if (t < k) sg[t] = -tk * gram[t + (long)k * b];
+    const double tk = stau[k];
+    if (t < k) sg[t] = -tk * sG[t][k];
     __syncthreads();
     if (t < k) {
       double s = 0.0;
@@ -283,17 +383,11 @@ __global__ __launch_bounds__(HQ_NT) void panel_permute_kernel(
 }  // namespace
 
 hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t stream) {
-  size_t lds = sizeof(double) * (size_t)kMaxB * a.ldR;
-  hipLaunchKernelGGL(panel_qrcp_kernel, dim3(a.nw), dim3(HQ_NT), lds, stream,
-                     a);
+  hipLaunchKernelGGL(panel_qrcp_kernel, dim3(a.nw), dim3(kPT), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t panel_set_lds_limit(size_t bytes) {
-  return hipFuncSetAttribute((const void*)panel_qrcp_kernel,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes);
-}
+int panel_rows_per_wg() { return kPR; }
 
 hipError_t launch_larft(int b, const double* gram, const double* tau,
                         double* T, long ldt, hipStream_t stream) {

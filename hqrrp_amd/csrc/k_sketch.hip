@@ -46,6 +46,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   const long slot = kSlotHead + RMAX;
 
   __shared__ double s_v[RMAX];
+  __shared__ double s_col[RMAX];  // this workgroup's candidate column
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
   __shared__ int s_win_pos, s_win_wg;
@@ -95,18 +96,24 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
           bk = s_key[q];
           bp = s_pos[q];
         }
-      cbuf[(long)w * slot + 0] = bk;
-      cbuf[(long)w * slot + 1] = (double)bp;
+      hq_publish(cbuf + (long)w * slot + 0, bk);
+      hq_publish(cbuf + (long)w * slot + 1, (double)bp);
       s_win_pos = bp;
     }
     __syncthreads();
     if (pos >= 0 && pos == s_win_pos) {
-      double* dst = cbuf + (long)w * slot + kSlotHead;
 #pragma unroll
       for (int q = 0; q < H; ++q) {
         const int i = ibase + q;
-        if (i >= k && i < r) dst[i] = x[q];
+        if (i >= k && i < r) s_col[i] = x[q];
       }
+    }
+    __syncthreads();
+    // Wavefront 0 publishes the candidate column (hq_grid_sync precondition).
+    if (wave == 0 && s_win_pos != INT_MAX) {
+      double* dst = cbuf + (long)w * slot + kSlotHead;
+      if (lane >= k && lane < r) hq_publish(dst + lane, s_col[lane]);
+      if (lane + 64 >= k && lane + 64 < r) hq_publish(dst + lane + 64, s_col[lane + 64]);
     }
     // 2. grid barrier
     if (!hq_grid_sync(a.bar, a.nw, phase, a.err)) return;
@@ -115,8 +122,8 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       double bk = -2.0;
       int bp = INT_MAX, bw = -1;
       for (int q = lane; q < a.nw; q += 64) {
-        double kq = cbuf[(long)q * slot + 0];
-        int pq = (int)cbuf[(long)q * slot + 1];
+        double kq = hq_fetch(cbuf + (long)q * slot + 0);
+        int pq = (int)hq_fetch(cbuf + (long)q * slot + 1);
         if (kq > bk || (kq == bk && pq < bp)) {
           bk = kq;
           bp = pq;
@@ -141,20 +148,25 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     const int p = s_win_pos;
-    // 4. winning column into LDS; 5. reflector (serial, identical everywhere).
-    {
+    // 4. winning column (rows k..r-1) into wavefront 0's registers;
+    // 5. reflector from a fixed-order wavefront reduction, identical in every
+    //    workgroup, and v into LDS.
+    if (wave == 0) {
       const double* src = cbuf + (long)s_win_wg * slot + kSlotHead;
-      for (int i = t; i < r; i += kThreads) s_v[i] = (i >= k) ? src[i] : 0.0;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double xn2 = 0.0;
-      for (int i = k + 1; i < r; ++i) xn2 += s_v[i] * s_v[i];
-      hq_reflector h = hq_dlarfg(s_v[k], xn2);
-      s_tau = h.tau;
-      for (int i = k + 1; i < r; ++i) s_v[i] *= h.scale;
-      s_v[k] = 1.0;
-      if (w == 0) a.piv[k] = p;
+      const int i0 = lane, i1 = lane + 64;
+      double x0 = (i0 >= k && i0 < r) ? hq_fetch(src + i0) : 0.0;
+      double x1 = (i1 >= k && i1 < r) ? hq_fetch(src + i1) : 0.0;
+      double alpha = __shfl(x0, k & 63, 64);
+      if (k >= 64) alpha = __shfl(x1, k & 63, 64);
+      double sq = ((i0 > k) ? x0 * x0 : 0.0) + ((i1 > k) ? x1 * x1 : 0.0);
+      sq = hq_wave_sum(sq);
+      hq_reflector h = hq_dlarfg(alpha, sq);
+      if (i0 >= k && i0 < r) s_v[i0] = (i0 == k) ? 1.0 : x0 * h.scale;
+      if (i1 >= k && i1 < r) s_v[i1] = (i1 == k) ? 1.0 : x1 * h.scale;
+      if (lane == 0) {
+        s_tau = h.tau;
+        if (w == 0) a.piv[k] = p;
+      }
     }
     __syncthreads();
     // 6. bookkeeping of positions (swap of positions k and p).

@@ -29,81 +29,148 @@ __device__ __forceinline__ double vmask(double v, long i, long c) {
   return (i < c) ? 0.0 : ((i == c) ? 1.0 : v);
 }
 
-// C(mrows x nc) -= V(mrows x b) * W(b x nc).
-__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
-  __shared__ double Vs[kTile][kPitch];  // Vs[k][m] = -V(m0+m, k)
-  __shared__ double Ws[kTile][kPitch];  // Ws[k][n] = W(k, n0+n)
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const long m0 = (long)blockIdx.x * kTile, n0 = (long)blockIdx.y * kTile;
-  const int b = a.b, kpad = (b + 3) & ~3;
+// C(mrows x nc) -= V(mrows x b) * W(b x nc) on 128x128 tiles, computed
+// transposed as C' := C' + W' (-V') so that the row-strided f64 accumulator
+// layout maps to 16 consecutive rows of one column of C per lane group:
+// every C load and store moves whole 128-byte lines.  The C tile is loaded
+// straight into the accumulators, so C makes exactly one round trip through
+// HBM.  K = b <= 64 is staged in chunks of 16 through double-buffered LDS
+// (70 KB: two workgroups per CU), the next chunk prefetched into registers
+// while the matrix cores work on the current one.  Row tiles vary fastest
+// across workgroups so concurrently running tiles cover different rows
+// (different HBM channels) of the power-of-two-strided columns.
+constexpr int kUT = 128;          // update tile edge
+constexpr int kUK = 16;           // K rows per pipeline stage
+constexpr int kUP = kUT + 8;      // LDS pitch (doubles)
 
-  hq_d4 acc[2][2];
+__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
+  __shared__ double Vs[2][kUK][kUP];  // Vs[buf][k][m] = -V(m0+m, k0+k)
+  __shared__ double Ws[2][kUK][kUP];  // Ws[buf][k][n] = W(k0+k, n0+n) (from W')
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 64-row / 64-column quadrant
+  const int l15 = lane & 15, l4 = lane >> 4;
+  const int b = a.b;
+  const int nchunks = (b + kUK - 1) / kUK;
+  const long m0 = (long)blockIdx.x * kUT, n0 = (long)blockIdx.y * kUT;
+  const bool masked = m0 < b;  // only the top tile holds R / the diagonal
+
+  // acc[i][j] element r: C'(n, m), n = n0+64wn+16i+l/16+4r, m = m0+64wm+16j+l%16,
+  // i.e. cbase[16j + (16i+4r)*ldc] with the lane's base below.
+  const bool interior = m0 + kUT <= a.mrows && n0 + kUT <= a.nc;
+  const long lm = m0 + wm * 64 + l15, ln = n0 + wn * 64 + l4;
+  double* cbase = a.C + lm + ln * a.ldc;
+  hq_d4 acc[4][4];
+  if (interior) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        long gm = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-        long gn = n0 + wn * 32 + j * 16 + (lane & 15);
-        acc[i][j][r] =
-            (gm < a.mrows && gn < a.nc) ? a.C[gm + gn * a.ldc] : 0.0;
+        const double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j][r] = cc[16 * j];
       }
-  for (int idx = t; idx < kTile * kpad; idx += HQ_NT) {
-    int m = idx & 63, k = idx >> 6;
-    long gm = m0 + m;
-    double v = 0.0;
-    if (gm < a.mrows && k < b) v = vmask(a.V[gm + (long)k * a.ldv], gm, k);
-    Vs[k][m] = -v;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool nok = ln + 16 * i + 4 * r < a.nc;
+        const double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j][r] = (nok && lm + 16 * j < a.mrows) ? cc[16 * j] : 0.0;
+      }
   }
-  for (int idx = t; idx < kTile * kpad; idx += HQ_NT) {
-    int k = idx % kpad, n = idx / kpad;
-    long gn = n0 + n;
-    Ws[k][n] = (k < b && gn < a.nc) ? a.W[k + gn * a.ldw] : 0.0;
-  }
+
+  constexpr int kPer = kUT * kUK / HQ_NT;  // 8 elements of V and of W'
+  double rv[kPer], rw[kPer];
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = t + HQ_NT * q, mn = idx & (kUT - 1), kl = idx >> 7;
+      const int k = kc * kUK + kl;
+      const long gm = m0 + mn, gn = n0 + mn;
+      double v = 0.0;
+      if (k < b && gm < a.mrows) {
+        v = a.V[gm + (long)k * a.ldv];
+        if (masked) v = vmask(v, gm, k);
+      }
+      rv[q] = -v;
+      rw[q] = (k < b && gn < a.nc) ? a.W[gn + (long)k * a.ldw] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = t + HQ_NT * q, mn = idx & (kUT - 1), kl = idx >> 7;
+      Vs[buf][kl][mn] = rv[q];
+      Ws[buf][kl][mn] = rw[q];
+    }
+  };
+  load(0);
+  stash(0);
   __syncthreads();
-  for (int kk = 0; kk < kpad; kk += 4) {
-    double af[2], bf[2];
+  for (int kc = 0; kc < nchunks; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nchunks) load(kc + 1);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-      af[i] = Vs[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+    for (int kk = 0; kk < kUK; kk += 4) {
+      double af[4], bf[4];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      bf[j] = Ws[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) af[i] = Ws[buf][kk + l4][wn * 64 + i * 16 + l15];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 4; ++j) bf[j] = Vs[buf][kk + l4][wm * 64 + j * 16 + l15];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] =
-            __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+                                                            acc[i][j], 0, 0, 0);
+    }
+    if (kc + 1 < nchunks) stash(buf ^ 1);
+    __syncthreads();
   }
+  if (interior) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        long gm = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-        long gn = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (gm < a.mrows && gn < a.nc) a.C[gm + gn * a.ldc] = acc[i][j][r];
+        double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cc[16 * j] = acc[i][j][r];
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool nok = ln + 16 * i + 4 * r < a.nc;
+        double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (nok && lm + 16 * j < a.mrows) cc[16 * j] = acc[i][j][r];
+      }
+  }
 }
 
-// part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s.
-// K rows are staged kBK at a time; the next slice of V and C is loaded into
-// registers while the matrix cores work on the current one.
+// part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
+// computed transposed (C' V) so the accumulator stores whole columns of the
+// partial.  K rows are staged kBK at a time, k-contiguous in LDS (pitch kBK+4:
+// conflict-free stores, at most 2-way conflicts on fragment reads); the next
+// slice is loaded into registers while the matrix cores work.
 __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
-  __shared__ double Vs[kBK][kPitch];  // Vs[k][p] = V(k0+k, p)
-  __shared__ double Cs[kBK][kPitch];  // Cs[k][n] = C(k0+k, n0+n)
+  constexpr int P = kBK + 4;
+  __shared__ double Vs[kTile][P];  // Vs[p][k] = V(k0+k, p)
+  __shared__ double Cs[kTile][P];  // Cs[n][k] = C(k0+k, n0+n)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wn = wave & 1, wp = wave >> 1;  // 32-column / 32-p quadrant
   const long n0 = (long)blockIdx.x * kTile;
   const int s = blockIdx.y;
   const long kbeg = (long)s * a.kchunk;
   const long kend = min((long)a.mrows, kbeg + a.kchunk);
   const int b = a.b;
   constexpr int kPer = kBK * kTile / HQ_NT;  // 8 elements per thread
+  const int l15 = lane & 15, l4 = lane >> 4;
 
   hq_d4 acc[2][2];
 #pragma unroll
@@ -124,26 +191,30 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
       rc[q] = (gk < kend && gn < a.nc) ? a.C[gk + gn * a.ldc] : 0.0;
     }
   };
-  if (kbeg < kend) load(kbeg);
-  for (long k0 = kbeg; k0 < kend; k0 += kBK) {
+  // The K slice is walked starting at a per-column-tile rotation, so that
+  // concurrently running workgroups read different rows (spreading the
+  // power-of-two-strided columns over HBM channels).
+  const long nsteps = (kend - kbeg + kBK - 1) / kBK;
+  const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
+  auto kstart = [&](long it) { return kbeg + ((it + rot) % nsteps) * kBK; };
+  if (nsteps > 0) load(kstart(0));
+  for (long it = 0; it < nsteps; ++it) {
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       int idx = t + HQ_NT * q;
       int k = idx & (kBK - 1), p = idx / kBK;
-      Vs[k][p] = rv[q];
-      Cs[k][p] = rc[q];
+      Vs[p][k] = rv[q];
+      Cs[p][k] = rc[q];
     }
     __syncthreads();
-    if (k0 + kBK < kend) load(k0 + kBK);
+    if (it + 1 < nsteps) load(kstart(it + 1));
 #pragma unroll
     for (int kk = 0; kk < kBK; kk += 4) {
-      double af[2], bf[2];
+      double af[2], bf[2];  // A = C' (n x k), B = V (k x p)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        af[i] = Vs[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 2; ++i) af[i] = Cs[wn * 32 + i * 16 + l15][kk + l4];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        bf[j] = Cs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 2; ++j) bf[j] = Vs[wp * 32 + j * 16 + l15][kk + l4];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -153,6 +224,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
     }
     __syncthreads();
   }
+  // acc element r of lane l: partial(p, n) with p = 32wp+16j+l%16 and
+  // n = n0+32wn+16i+l/16+4r.
   double* out = a.part + (long)s * b * a.nc;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -160,13 +233,14 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int p = wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-        long gn = n0 + wn * 32 + j * 16 + (lane & 15);
+        int p = wp * 32 + j * 16 + l15;
+        long gn = n0 + wn * 32 + i * 16 + l4 + 4 * r;
         if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
       }
 }
 
-// W(:, n0:n0+64) = T' * sum_s part[s](:, n0:n0+64); slices added in order.
+// W(:, n0:n0+64) = T' * sum_s part[s](:, n0:n0+64), slices added in order,
+// stored transposed (W' is nc x b) for coalesced staging in trail_update.
 __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
   __shared__ double Ms[kTile][kTile + 1];  // Ms[p][n]
   __shared__ double Ts[kTile][kTile + 1];  // Ts[p][q] = T(p, q)
@@ -187,12 +261,12 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
   }
   __syncthreads();
   for (int idx = t; idx < b * kTile; idx += HQ_NT) {
-    int q = idx % b, n = idx / b;
+    int n = idx & (kTile - 1), q = idx / kTile;
     long gn = n0 + n;
     if (gn >= a.nc) continue;
     double s = 0.0;
     for (int p = 0; p <= q; ++p) s += Ts[p][q] * Ms[p][n];
-    a.W[q + gn * a.ldw] = s;
+    a.W[gn + (long)q * a.ldw] = s;  // W' (nc x b), ld ldw
   }
 }
 
@@ -211,7 +285,7 @@ hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s) {
-  dim3 grid(hq_cdiv(a.mrows, kTile), hq_cdiv(a.nc, kTile));
+  dim3 grid(hq_cdiv(a.mrows, kUT), hq_cdiv(a.nc, kUT));
   hipLaunchKernelGGL(trail_update_kernel, grid, dim3(HQ_NT), 0, s, a);
   return hipGetLastError();
 }

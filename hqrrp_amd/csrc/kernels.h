@@ -19,12 +19,12 @@ struct PanelArgs {
   double* P;  // panel: mrows x b, ld ldp (factored in place)
   long ldp;
   int mrows, b, pivoting;
-  int R, ldR, nw;  // rows per workgroup, LDS pitch, workgroups
+  int nw;          // workgroups (panel_rows_per_wg() rows each)
   double* tau;     // b
   int* perm;       // b: original column at each position (0-based)
   double* gram;    // b x b, strictly upper part: v_l' v_k
-  double* part;    // scratch: nw * 192 doubles
-  double* rowbuf;  // scratch: 128 doubles
+  double* part;    // scratch: nw * 192 doubles (coherent exchange)
+  double* rowbuf;  // scratch: 128 doubles (coherent exchange)
   unsigned* bar;
   int* err;
 };
@@ -49,7 +49,7 @@ struct TrailArgs {
   long ldv;
   double* C;  // mrows x nc trailing matrix, ld ldc
   long ldc;
-  double* W;  // b x nc, ld ldw: T' V' C
+  double* W;  // W' = (T' V' C)': nc x b, ld ldw
   long ldw;
   const double* T;  // b x b upper triangular, ld ldt
   long ldt;
@@ -71,7 +71,7 @@ hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
 
 // k_panel.hip
 hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t s);
-hipError_t panel_set_lds_limit(size_t bytes);
+int panel_rows_per_wg();
 hipError_t launch_larft(int b, const double* gram, const double* tau,
                         double* T, long ldt, hipStream_t s);
 hipError_t launch_panel_permute(double* A, long lda, int nrows, int b,
