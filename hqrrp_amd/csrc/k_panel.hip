@@ -34,6 +34,7 @@ constexpr int kMaxB = 64;
 // Sum of src[q*stride] for q in [q0, q1): loads issued 16 at a time so a
 // cross-workgroup reduction costs ~one memory round trip per 16 partials;
 // the summation order is fixed, so every workgroup gets the same bits.
+// chunked_sum2 does the same for two arrays with all loads in flight.
 __device__ __forceinline__ double chunked_sum(const double* src, long stride,
                                               int q0, int q1) {
   double s = 0.0;
@@ -46,6 +47,28 @@ __device__ __forceinline__ double chunked_sum(const double* src, long stride,
     for (int i = 0; i < 16; ++i) s += v[i];
   }
   return s;
+}
+
+__device__ __forceinline__ void chunked_sum2(const double* s1p, const double* s2p,
+                                             long stride, int q0, int q1,
+                                             double& o1, double& o2) {
+  double a1 = 0.0, a2 = 0.0;
+  for (int base = q0; base < q1; base += 16) {
+    double v1[16], v2[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const bool ok = base + i < q1;
+      v1[i] = ok ? hq_fetch(s1p + (long)(base + i) * stride) : 0.0;
+      v2[i] = ok ? hq_fetch(s2p + (long)(base + i) * stride) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      a1 += v1[i];
+      a2 += v2[i];
+    }
+  }
+  o1 = a1;
+  o2 = a2;
 }
 
 constexpr int kPT = 512;          // threads of the panel kernel
@@ -157,10 +180,9 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
   };
   auto reduce_B = [&]() {
-    if (col_ok) {
-      s_red[g][cc] = chunked_sum(partB + cc, 128, q0, q1);
-      s_red2[g][cc] = chunked_sum(partB + 64 + cc, 128, q0, q1);
-    }
+    if (col_ok)
+      chunked_sum2(partB + cc, partB + 64 + cc, 128, q0, q1, s_red[g][cc],
+                   s_red2[g][cc]);
     __syncthreads();
     if (t < b) {
       double a1 = 0.0, a2 = 0.0;
@@ -318,35 +340,52 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   hq_grid_exit(a.bar, nw);
 }
 
-// dlarft('Forward','Columnwise') from the reflector inner products:
-// T(0:k,k) = -tau_k * T(0:k,0:k) * gram(0:k,k), T(k,k) = tau_k; a zero tau
-// gives a zero column.  One workgroup; thread l owns row l of T.
+// dlarft('Forward','Columnwise') from the reflector inner products
+// G(l,q) = v_l'v_q (l < q): T is the compact-WY factor of H_0 H_1 ... with
+// T(q,q) = tau_q and, for adjacent column blocks B1, B2,
+//   T(B1,B2) = -T(B1,B1) G(B1,B2) T(B2,B2),
+// built by doubling the block size (1, 2, 4, ..., 32): six levels of small
+// triangular products instead of LAPACK's b-step recurrence (same matrix in
+// exact arithmetic; a zero tau gives a zero column, as LAPACK's).
 __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
                                                       const double* tau,
                                                       double* T, long ldt) {
   __shared__ double sT[kMaxB][kMaxB + 1];
   __shared__ double sG[kMaxB][kMaxB + 1];
-  __shared__ double sg[kMaxB], stau[kMaxB];
+  __shared__ double sX[kMaxB][kMaxB + 1];
   const int t = threadIdx.x;
-  for (int idx = t; idx < kMaxB * (kMaxB + 1); idx += HQ_NT)
+  for (int idx = t; idx < kMaxB * (kMaxB + 1); idx += HQ_NT) {
     (&sT[0][0])[idx] = 0.0;
+    (&sX[0][0])[idx] = 0.0;
+  }
+  __syncthreads();
   for (int idx = t; idx < b * b; idx += HQ_NT) {
     int i = idx % b, j = idx / b;
     sG[i][j] = (i < j) ? gram[idx] : 0.0;
+    if (i == j) sT[i][i] = tau[i];
   }
-  if (t < b) stau[t] = tau[t];
   __syncthreads();
-  for (int k = 0; k < b; ++k) {
-    const double tk = stau[k];
-    if (t < k) sg[t] = -tk * sG[t][k];
-    __syncthreads();
-    if (t < k) {
-      double s = 0.0;
-      if (tk != 0.0)
-        for (int q = t; q < k; ++q) s += sT[t][q] * sg[q];
-      sT[t][k] = s;
+  for (int s = 1; s < b; s *= 2) {
+    // X(B1,B2) = G(B1,B2) T(B2,B2) for every pair (B1,B2) at this level.
+    for (int e = t; e < b * s; e += HQ_NT) {
+      const int col = e / s, r = e % s;           // col: global column in B2
+      const int c0 = (col / (2 * s)) * 2 * s;     // start of B1
+      const int c1 = c0 + s;                      // start of B2
+      if (col < c1 || col >= min(b, c0 + 2 * s)) continue;
+      double acc = 0.0;
+      for (int u = c1; u <= col; ++u) acc += sG[c0 + r][u] * sT[u][col];
+      sX[c0 + r][col] = acc;
     }
-    if (t == 0) sT[k][k] = tk;
+    __syncthreads();
+    // T(B1,B2) = -T(B1,B1) X(B1,B2).
+    for (int e = t; e < b * s; e += HQ_NT) {
+      const int col = e / s, r = e % s;
+      const int c0 = (col / (2 * s)) * 2 * s, c1 = c0 + s;
+      if (col < c1 || col >= min(b, c0 + 2 * s)) continue;
+      double acc = 0.0;
+      for (int u = c0 + r; u < c1; ++u) acc += sT[c0 + r][u] * sX[u][col];
+      sT[c0 + r][col] = -acc;
+    }
     __syncthreads();
   }
   for (int idx = t; idx < b * b; idx += HQ_NT) {

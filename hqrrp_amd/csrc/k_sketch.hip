@@ -29,10 +29,16 @@ namespace {
 constexpr int kSlotHead = 8;  // candidate slot: [key, pos, ...pad, column]
 constexpr int kThreads = 512;  // 8 wavefronts, 32 columns each
 constexpr int kCols = 256;     // sketch columns per workgroup
+constexpr int kMaxNw = 64;     // workgroups whose candidates are prefetched
 
-// Column c of the workgroup lives in wavefront c/32; lane c%32 holds rows
-// [0, RMAX/2) and lane c%32+32 rows [RMAX/2, RMAX), so a column's dot
-// product is one cross-lane add (a+b == b+a, both lanes get the same bits).
+// Column c of the workgroup lives in wavefront c/32; lane c%32 holds the
+// first RMAX/2 of the column's *active* rows and lane c%32+32 the next
+// RMAX/2.  Active rows are rows k..r-1 at step k: after every step the
+// registers shift up by one row (the boundary row crosses from the upper to
+// the lower lane by one shuffle), so row k is always x[0] of the lower lane
+// and every per-row operation uses static register indices with no tests
+// against k.  Rows at or beyond r are zero.  A column's dot product is one
+// cross-lane add (a+b == b+a, both lanes get the same bits).
 template <int RMAX>
 __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   constexpr int H = RMAX / 2;
@@ -42,11 +48,12 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   const int r = a.r;
   const int c = w * kCols + wave * 32 + (lane & 31);
   const bool own = c < a.ncols;
-  const int ibase = half * H;  // first row held by this lane
+  const int ibase = half * H;  // first active row (relative) of this lane
   const long slot = kSlotHead + RMAX;
 
-  __shared__ double s_v[RMAX];
+  __shared__ double s_v[RMAX];    // reflector on the active rows (relative)
   __shared__ double s_col[RMAX];  // this workgroup's candidate column
+  __shared__ double s_all[kMaxNw][RMAX];  // every workgroup's candidate
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
   __shared__ int s_win_pos, s_win_wg;
@@ -71,6 +78,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 
   for (int k = 0; k < a.stages; ++k) {
     double* cbuf = a.cand + (long)(k & 1) * a.nw * slot;
+    const int span = r - k;  // active rows
     // 1. local argmax: largest norm, smallest position among equals.
     double key = (pos >= 0) ? d : -1.0;
     int kp = (pos >= 0) ? pos : INT_MAX;
@@ -103,21 +111,23 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     __syncthreads();
     if (pos >= 0 && pos == s_win_pos) {
 #pragma unroll
-      for (int q = 0; q < H; ++q) {
-        const int i = ibase + q;
-        if (i >= k && i < r) s_col[i] = x[q];
-      }
+      for (int q = 0; q < H; ++q) s_col[ibase + q] = x[q];
     }
     __syncthreads();
-    // Wavefront 0 publishes the candidate column (hq_grid_sync precondition).
+    // Wavefront 0 publishes the candidate's active rows (hq_grid_sync
+    // precondition).
     if (wave == 0 && s_win_pos != INT_MAX) {
       double* dst = cbuf + (long)w * slot + kSlotHead;
-      if (lane >= k && lane < r) hq_publish(dst + lane, s_col[lane]);
-      if (lane + 64 >= k && lane + 64 < r) hq_publish(dst + lane + 64, s_col[lane + 64]);
+      if (lane < span) hq_publish(dst + lane, s_col[lane]);
+      if (lane + 64 < span) hq_publish(dst + lane + 64, s_col[lane + 64]);
     }
     // 2. grid barrier
     if (!hq_grid_sync(a.bar, a.nw, phase, a.err)) return;
-    // 3. global argmax over the workgroup candidates (wavefront 0).
+    // 3. global argmax over the workgroup candidates (wavefront 0) while the
+    //    other wavefronts fetch every candidate column into LDS in the same
+    //    memory round trip (when nw <= kMaxNw); the winner's column is then
+    //    already local.
+    const bool prefetch = a.nw <= kMaxNw;
     if (wave == 0) {
       double bk = -2.0;
       int bp = INT_MAX, bw = -1;
@@ -145,24 +155,48 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
         s_win_pos = bp;
         s_win_wg = bw;
       }
+    } else if (prefetch) {
+      constexpr int kPF = (kMaxNw * RMAX + kThreads - 65) / (kThreads - 64);
+      const int total = a.nw * span;
+      double vbuf[kPF];
+#pragma unroll
+      for (int u = 0; u < kPF; ++u) {
+        const int e2 = (t - 64) + u * (kThreads - 64);
+        vbuf[u] = 0.0;
+        if (e2 < total) {
+          const int q = e2 / span, i = e2 % span;
+          vbuf[u] = hq_fetch(cbuf + (long)q * slot + kSlotHead + i);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kPF; ++u) {
+        const int e2 = (t - 64) + u * (kThreads - 64);
+        if (e2 < total) s_all[e2 / span][e2 % span] = vbuf[u];
+      }
     }
     __syncthreads();
     const int p = s_win_pos;
-    // 4. winning column (rows k..r-1) into wavefront 0's registers;
+    // 4. winning column (active rows) into wavefront 0's registers;
     // 5. reflector from a fixed-order wavefront reduction, identical in every
-    //    workgroup, and v into LDS.
+    //    workgroup; v (relative rows, zero beyond span) into LDS.
     if (wave == 0) {
-      const double* src = cbuf + (long)s_win_wg * slot + kSlotHead;
+      const int wg = s_win_wg;
+      const double* src = cbuf + (long)wg * slot + kSlotHead;
       const int i0 = lane, i1 = lane + 64;
-      double x0 = (i0 >= k && i0 < r) ? hq_fetch(src + i0) : 0.0;
-      double x1 = (i1 >= k && i1 < r) ? hq_fetch(src + i1) : 0.0;
-      double alpha = __shfl(x0, k & 63, 64);
-      if (k >= 64) alpha = __shfl(x1, k & 63, 64);
-      double sq = ((i0 > k) ? x0 * x0 : 0.0) + ((i1 > k) ? x1 * x1 : 0.0);
+      double x0 = 0.0, x1 = 0.0;
+      if (prefetch) {
+        if (i0 < span) x0 = s_all[wg][i0];
+        if (i1 < span) x1 = s_all[wg][i1];
+      } else {
+        if (i0 < span) x0 = hq_fetch(src + i0);
+        if (i1 < span) x1 = hq_fetch(src + i1);
+      }
+      const double alpha = __shfl(x0, 0, 64);
+      double sq = ((i0 > 0) ? x0 * x0 : 0.0) + x1 * x1;
       sq = hq_wave_sum(sq);
       hq_reflector h = hq_dlarfg(alpha, sq);
-      if (i0 >= k && i0 < r) s_v[i0] = (i0 == k) ? 1.0 : x0 * h.scale;
-      if (i1 >= k && i1 < r) s_v[i1] = (i1 == k) ? 1.0 : x1 * h.scale;
+      if (i0 < RMAX) s_v[i0] = (i0 == 0) ? 1.0 : x0 * h.scale;
+      if (i1 < RMAX) s_v[i1] = x1 * h.scale;
       if (lane == 0) {
         s_tau = h.tau;
         if (w == 0) a.piv[k] = p;
@@ -178,35 +212,32 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
         pos = p;
       }
     }
-    // 7. apply the reflector to my column, then downdate its norm.  Both
-    // lanes of a column take the same branches (pos is per column).
+    // 7. apply the reflector to my column, downdate its norm, and shift the
+    //    active rows up by one (row k leaves).  Both lanes of a column take
+    //    the same branches (pos is per column).
     if (pos >= 0) {
       const double tau = s_tau;
       if (tau != 0.0) {
         double wd = 0.0;
 #pragma unroll
-        for (int q = 0; q < H; ++q) {
-          const int i = ibase + q;
-          if (i >= k && i < r) wd += s_v[i] * x[q];
-        }
+        for (int q = 0; q < H; ++q) wd += s_v[ibase + q] * x[q];
         wd += __shfl_xor(wd, 32, 64);
         const double f = -tau * wd;
 #pragma unroll
-        for (int q = 0; q < H; ++q) {
-          const int i = ibase + q;
-          if (i >= k && i < r) x[q] += f * s_v[i];
-        }
+        for (int q = 0; q < H; ++q) x[q] += f * s_v[ibase + q];
       }
-      double b2 = 0.0, wt = 0.0;
+      double b2 = 0.0;
 #pragma unroll
-      for (int q = 0; q < H; ++q) {
-        const int i = ibase + q;
-        if (i == k) wt = x[q];
-        if (i > k && i < r) b2 += x[q] * x[q];
-      }
+      for (int q = 1; q < H; ++q) b2 += x[q] * x[q];
+      const double x0 = x[0];
+      const double up = __shfl_xor(x0, 32, 64);  // partner's first row
+      b2 += half ? x0 * x0 : 0.0;
       b2 += __shfl_xor(b2, 32, 64);
-      wt += __shfl_xor(wt, 32, 64);  // exactly one lane holds row k
+      const double wt = half ? up : x0;  // row k
       hq_downdate_norm(d, e, wt, b2, r - k - 1);
+#pragma unroll
+      for (int q = 0; q < H - 1; ++q) x[q] = x[q + 1];
+      x[H - 1] = half ? 0.0 : up;
     }
   }
   // Composed permutation: final position of every column that moved.
