@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--cpu-blocks", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--lookahead", type=int, default=1,
+                    help="overlap the next panel with the trailing update")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -118,7 +120,7 @@ def main():
         A.copy_(A0)
         jp.copy_(ident)
         hqrrp_amd.hqrrp_device(h, A, jp, tau, m, n, nb=nb, pp=pp,
-                               panel_pivoting=1)
+                               panel_pivoting=1, lookahead=args.lookahead)
 
     for i in range(args.warmup):
         step()
@@ -196,7 +198,8 @@ def main():
         "config": {"workload": f"HQRRP plain FP64 {m}x{n} graded "
                                f"(Haar U,V, sigma_i=10^(-12i/n))",
                    "m": m, "n": n, "nb": nb, "pp": pp, "panel_pivoting": 1,
-                   "sketch": "fp64", "omega": "reference stream seed 12"},
+                   "sketch": "fp64", "omega": "reference stream seed 12",
+                   "lookahead": args.lookahead},
         "roofline": roof, "cpu_baseline": cpu, "phase_ms": phase_ms,
         "gemm_roofline": gemm_roof,
     }

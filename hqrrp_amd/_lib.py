@@ -28,6 +28,7 @@ class Options(C.Structure):
         ("seed", C.c_uint),
         ("omega", _vp),
         ("max_rank", C.c_int),
+        ("lookahead", C.c_int),
     ]
 
 

@@ -134,12 +134,14 @@ class Handle:
 
 
 def options(nb=64, pp=10, panel_pivoting=1, seed=12, omega=None,
-            max_rank=0, sketch_prec=0) -> Options:
+            max_rank=0, sketch_prec=0, lookahead=None) -> Options:
     o = Options()
     lib().hqrrp_default_options(C.byref(o))
     o.nb, o.pp, o.panel_pivoting, o.seed = nb, pp, panel_pivoting, seed
     o.max_rank = max_rank
     o.sketch_prec = sketch_prec
+    if lookahead is not None:
+        o.lookahead = int(lookahead)
     if omega is not None:
         o.omega_kind = 1
         o.omega = C.c_void_p(omega.data_ptr())

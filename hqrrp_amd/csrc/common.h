@@ -13,9 +13,9 @@ enum {
   HQ_ERR_BARRIER_TIMEOUT = 1,  // a persistent kernel's grid barrier timed out
 };
 
-// Upper bound on spins of a grid barrier before the kernel gives up and
-// reports HQ_ERR_BARRIER_TIMEOUT (a spin is a coherent load plus a short
-// sleep, about 1 us: a few seconds in total).
+// Upper bound on polls of a cross-workgroup exchange before the kernel gives
+// up and reports HQ_ERR_BARRIER_TIMEOUT (a poll is a coherent load plus a
+// short sleep, about 1 us: a few seconds in total).
 #define HQ_SPIN_LIMIT (1u << 22)
 
 // Cross-workgroup data of the persistent kernels is exchanged with
@@ -29,65 +29,63 @@ __device__ __forceinline__ double hq_fetch(const double* p) {
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide barrier for persistent kernels whose grid the host has sized to
-// be co-resident.  bar[0] counts arrivals and only grows within a launch
-// (barrier number `phase` completes at phase*nblocks arrivals); bar[1] counts
-// workgroups leaving the kernel so the last one out can zero both words for
-// the next launch on the stream.  Returns false (in every thread) if the wait
-// timed out or another workgroup reported a timeout.
-//
-// Precondition: every global store that another workgroup reads after this
-// barrier was issued by wavefront 0 (the wavefront of thread 0), so thread
-// 0's release (which waits for its wavefront's stores) orders all of them
-// before the arrival; thread 0's acquire, followed by the workgroup barrier,
-// orders every thread's later loads after all arrivals.  Only thread 0 pays
-// for the release/acquire cache maintenance.
-__device__ __forceinline__ bool hq_grid_sync(unsigned* bar, unsigned nblocks,
-                                             unsigned& phase, int* err) {
-  __shared__ int s_ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    phase += 1;
-    const unsigned target = phase * nblocks;
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    int ok = 1;
-    unsigned spins = 0;
-    // Poll with relaxed loads (an acquire load would invalidate the caches
-    // on every poll); one acquire fence once the count is reached.
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-           target) {
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        ok = 0;
-        break;
-      }
-      if (++spins > HQ_SPIN_LIMIT) {
-        __hip_atomic_store(err, HQ_ERR_BARRIER_TIMEOUT, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    s_ok = ok;
-  }
-  __syncthreads();
-  return s_ok != 0;
+// Barrier-free exchange.  Every exchange of a launch has slots of its own in
+// an exchange buffer that holds only the sentinel below when the launch
+// starts.  A producer stores its values with hq_publish; a consumer re-reads
+// each value until it no longer holds the sentinel.  Every value is one
+// single-copy-atomic 64-bit word that validates itself, so no fence, counter
+// or barrier orders anything, and an exchange costs one memory round trip
+// once the data is there.  The sentinel is a signalling NaN whose payload no
+// arithmetic produces (arithmetic NaNs are quiet).  Two buffers alternate
+// between consecutive launches: each launch restores the sentinel over the
+// region the previous launch used in the other buffer.
+#define HQ_XSENT 0x7FF4DEAD7FF4DEADll
+
+__device__ __forceinline__ double hq_sentinel() {
+  return __longlong_as_double(HQ_XSENT);
+}
+__device__ __forceinline__ bool hq_pending(double v) {
+  return __double_as_longlong(v) == HQ_XSENT;
 }
 
-// Called once by every workgroup after its last hq_grid_sync: the last
-// workgroup to leave resets the barrier words.
-__device__ __forceinline__ void hq_grid_exit(unsigned* bar, unsigned nblocks) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned old = __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_ACQ_REL,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    if (old == nblocks - 1) {
-      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(bar + 1, 0u, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
+// v[i] = *addr(i) for every i < N with addr(i) != nullptr (0.0 otherwise),
+// all loads in flight together, re-reading those still pending.  Returns
+// false after HQ_SPIN_LIMIT polls (recording HQ_ERR_BARRIER_TIMEOUT) or when
+// another workgroup has recorded an error.
+template <int N, class Addr>
+__device__ __forceinline__ bool hq_await(double (&v)[N], Addr addr, int* err) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double* p = addr(i);
+    v[i] = p ? hq_fetch(p) : 0.0;
   }
+  for (unsigned spins = 0;; ++spins) {
+    bool pend = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) pend |= hq_pending(v[i]);
+    if (!pend) return true;
+    if (spins > HQ_SPIN_LIMIT) {
+      __hip_atomic_store(err, HQ_ERR_BARRIER_TIMEOUT, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 255) == 255 &&
+        __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (hq_pending(v[i])) v[i] = hq_fetch(addr(i));
+  }
+}
+
+// Restores the sentinel over x[0, len), the share of workgroup w of nw.
+__device__ __forceinline__ void hq_xreset(double* x, long len, int w, int nw,
+                                          int nthreads) {
+  const double s = hq_sentinel();
+  for (long i = (long)w * nthreads + threadIdx.x; i < len;
+       i += (long)nw * nthreads)
+    x[i] = s;
 }
 
 // Wavefront (64-lane) sum; every lane receives the total.  The butterfly

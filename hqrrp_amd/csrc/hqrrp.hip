@@ -90,6 +90,17 @@ struct RefRand {
   }
 };
 
+// Exchange buffers of a persistent kernel (common.h): two buffers used by
+// alternate launches.  used[i] is the prefix of buffer i written by the last
+// launch that ran on it; the next launch (on the other buffer) restores the
+// sentinel over it, so a launch always starts on an all-sentinel buffer.
+struct Xchg {
+  DevBuf buf[2];
+  long used[2] = {0, 0};
+  int par = 0;
+  bool dirty = false;  // a launch aborted: both buffers are refilled
+};
+
 }  // namespace
 
 extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
@@ -140,12 +151,21 @@ static const char* const kPhaseNames[HQ_PHASES] = {
 struct hqrrp_handle_s {
   int device = 0;
   hipStream_t own = nullptr, stream = nullptr;
+  hipStream_t cur = nullptr;  // stream the internal helpers launch on
+  // Look-ahead streams: the next block's panel runs on s_chain (restricted
+  // to chain_cus CUs, so its persistent workgroups are co-resident at once)
+  // while the trailing update runs on s_upd (the remaining CUs).
+  hipStream_t s_chain = nullptr, s_upd = nullptr;
+  int chain_cus = 0;
+  hipEvent_t ev_fork = nullptr, ev_upd = nullptr, ev_chain = nullptr;
   int cu_count = 0;
   size_t lds_max = 0;
   int sketch_max_wg = 0, panel_max_wg = 0;
-  DevBuf words;  // [0..1] sketch barrier, [2..3] panel barrier, [8] error
-  DevBuf Y, G, Gcache, cand, moves, piv, ppart, rowbuf, gram, T, perm;
+  DevBuf words;  // [8] error word of the persistent kernels
+  DevBuf Y, G, Gcache, moves, piv, gram, T, perm;
+  Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
   DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
+  DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
   DevBuf hA, hJ, hT;  // device copies used by the host entry points
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
@@ -162,8 +182,6 @@ struct hqrrp_handle_s {
   double ph_flops[HQ_PHASES] = {}, ph_bytes[HQ_PHASES] = {};
   long long ph_launch[HQ_PHASES] = {};
 
-  unsigned* bar_sketch() const { return words.as<unsigned>(); }
-  unsigned* bar_panel() const { return words.as<unsigned>() + 2; }
   int* err() const { return words.as<int>() + 8; }
 };
 
@@ -179,6 +197,7 @@ int check_err(hqrrp_handle_t h) {
             e);
     HQ_TRY(hipMemsetAsync(h->words.p, 0, h->words.bytes, h->stream));
     HQ_TRY(hipStreamSynchronize(h->stream));
+    h->xs.dirty = h->xp.dirty = true;
     return HQRRP_ERR_KERNEL;
   }
   return HQRRP_OK;
@@ -207,13 +226,13 @@ struct PhaseScope {
     h->ph_bytes[ph] += bytes;
     h->ph_launch[ph] += launches;
     a = next_event(h);
-    if (a) (void)hipEventRecord(a, h->stream);
+    if (a) (void)hipEventRecord(a, h->cur);
   }
   ~PhaseScope() {
     if (!h->prof || !a) return;
     hipEvent_t b = next_event(h);
     if (!b) return;
-    (void)hipEventRecord(b, h->stream);
+    (void)hipEventRecord(b, h->cur);
     h->spans.push_back({ph, a, b});
   }
 };
@@ -259,13 +278,39 @@ int gemm(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K, double alpha,
   if (S > 1) {
     HQ_RET(ensure(h->split, sizeof(double) * (size_t)S * M * N));
     g.part = h->split.as<double>();
-    HQ_TRY(launch_gemm(g, ta, tb, S, h->stream, tag));
-    HQ_TRY(launch_reduce_splitk(M, N, S, g.part, beta, C, ldc, h->stream));
+    HQ_TRY(launch_gemm(g, ta, tb, S, h->cur, tag));
+    HQ_TRY(launch_reduce_splitk(M, N, S, g.part, beta, C, ldc, h->cur));
   } else {
     g.part = nullptr;
-    HQ_TRY(launch_gemm(g, ta, tb, 1, h->stream, tag));
+    HQ_TRY(launch_gemm(g, ta, tb, 1, h->cur, tag));
   }
   return HQRRP_OK;
+}
+
+// Buffers for a launch that writes `need` doubles of exchange slots.
+int xchg_begin(hqrrp_handle_t h, Xchg& x, size_t need, double** cur,
+               double** prev, long* prev_len) {
+  for (int i = 0; i < 2; ++i) {
+    const bool grow = x.buf[i].bytes < need * sizeof(double);
+    if (grow) HQ_RET(ensure(x.buf[i], need * sizeof(double)));
+    if (grow || x.dirty) {
+      HQ_TRY(launch_fill_sentinel(x.buf[i].as<double>(),
+                                  x.buf[i].bytes / sizeof(double), h->cur));
+      x.used[i] = 0;
+    }
+  }
+  x.dirty = false;
+  *cur = x.buf[x.par].as<double>();
+  *prev = x.buf[1 - x.par].as<double>();
+  *prev_len = x.used[1 - x.par];
+  return HQRRP_OK;
+}
+
+// After a successful launch that wrote `used` doubles of its buffer.
+void xchg_end(Xchg& x, size_t used) {
+  x.used[1 - x.par] = 0;
+  x.used[x.par] = (long)used;
+  x.par ^= 1;
 }
 
 int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
@@ -287,13 +332,13 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const double* dY, long ldy,
   a.stages = stages;
   a.nw = hq_cdiv(ncols, sketch_cols_per_wg());
   if (a.nw > h->sketch_max_wg) return HQRRP_ERR_UNSUPPORTED;
-  HQ_RET(ensure(h->cand, sizeof(double) * sketch_cand_doubles(a.nw, rmax)));
+  const size_t need = sketch_xchg_doubles(a.nw, rmax, stages);
+  HQ_RET(xchg_begin(h, h->xs, need, &a.xcur, &a.xprev, &a.xprev_len));
   a.piv = dpiv;
   a.moves = dmoves;
-  a.cand = h->cand.as<double>();
-  a.bar = h->bar_sketch();
   a.err = h->err();
-  HQ_TRY(launch_sketch_qrcp(a, rmax, h->stream));
+  HQ_TRY(launch_sketch_qrcp(a, rmax, h->cur));
+  xchg_end(h->xs, need);
   return HQRRP_OK;
 }
 
@@ -310,33 +355,35 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
   a.tau = dtau;
   a.perm = dperm;
   HQ_RET(ensure(h->gram, sizeof(double) * 64 * 64));
-  HQ_RET(ensure(h->ppart, sizeof(double) * (size_t)a.nw * 192));
-  HQ_RET(ensure(h->rowbuf, sizeof(double) * 128));
+  const size_t need = panel_xchg_doubles(a.nw, b);
+  HQ_RET(xchg_begin(h, h->xp, need, &a.xcur, &a.xprev, &a.xprev_len));
   a.gram = h->gram.as<double>();
-  a.part = h->ppart.as<double>();
-  a.rowbuf = h->rowbuf.as<double>();
-  a.bar = h->bar_panel();
   a.err = h->err();
   {
     // Algorithmic traffic: the panel is read and written once.
     PhaseScope ps(h, PH_PANEL, 2.0 * mrows * b * b - 2.0 / 3.0 * b * b * b,
                   16.0 * mrows * b, 1);
-    HQ_TRY(launch_panel_qrcp(a, h->stream));
+    HQ_TRY(launch_panel_qrcp(a, h->cur));
   }
+  xchg_end(h->xp, need);
   PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
-  HQ_TRY(launch_larft(b, a.gram, dtau, dT, ldt, h->stream));
+  HQ_TRY(launch_larft(b, a.gram, dtau, dT, ldt, h->cur));
   return HQRRP_OK;
 }
 
 // [A12; A22] := (I - V T V')' [A12; A22]  (dlarfb Left/Transpose):
-// M = V'C (split-K), W = T'M, C -= V W  (k_trail.hip).
-int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
-                  long ldv, const double* T, long ldt, double* C, long ldc) {
-  if (nc <= 0 || mrows <= 0) return HQRRP_OK;
+// M = V'C (split-K), W = T'M, C -= V W  (k_trail.hip).  trail_prepare sizes
+// the split and the buffers, trail_w computes W' (nc x b, in h->W2),
+// trail_apply the rank-b update; the look-ahead driver runs the last on a
+// stream of its own.
+int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
+                  long ldv, const double* T, long ldt, double* C, long ldc,
+                  TrailArgs* out) {
   TrailArgs a{};
   a.mrows = mrows;
   a.nc = nc;
   a.b = b;
+  a.vmask = 1;
   a.V = V;
   a.ldv = ldv;
   a.C = C;
@@ -358,22 +405,38 @@ int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   a.part = h->split.as<double>();
   a.W = h->W2.as<double>();
   a.ldw = nc;
-  const double mn = (double)mrows * nc;
-  {
-    PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * mrows * b, 1);
-    HQ_TRY(launch_trail_vtc(a, h->stream));
-  }
-  {
-    PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc, 8.0 * (S + 1.0) * b * nc,
-                  1);
-    HQ_TRY(launch_trail_tw(a, h->stream));
-  }
-  {
-    // C is read and written once; V and W are small.
-    PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * mn, 16.0 * mn, 1);
-    HQ_TRY(launch_trail_update(a, h->stream));
-  }
+  *out = a;
   return HQRRP_OK;
+}
+
+int trail_w(hqrrp_handle_t h, const TrailArgs& a) {
+  const double mn = (double)a.mrows * a.nc, b = a.b, S = a.splits;
+  {
+    PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * a.mrows * b,
+                  1);
+    HQ_TRY(launch_trail_vtc(a, h->cur));
+  }
+  PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * a.nc, 8.0 * (S + 1.0) * b * a.nc,
+                1);
+  HQ_TRY(launch_trail_tw(a, h->cur));
+  return HQRRP_OK;
+}
+
+int trail_apply(hqrrp_handle_t h, const TrailArgs& a) {
+  // C is read and written once; V and W are small.
+  const double mn = (double)a.mrows * a.nc;
+  PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * a.b * mn, 16.0 * mn, 1);
+  HQ_TRY(launch_trail_update(a, h->cur));
+  return HQRRP_OK;
+}
+
+int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
+                  long ldv, const double* T, long ldt, double* C, long ldc) {
+  if (nc <= 0 || mrows <= 0) return HQRRP_OK;
+  TrailArgs a;
+  HQ_RET(trail_prepare(h, mrows, nc, b, V, ldv, T, ldt, C, ldc, &a));
+  HQ_RET(trail_w(h, a));
+  return trail_apply(h, a);
 }
 
 // Sketch downdate with the old G_R = [G1 G2]:  P = G_R V, Q = P T,
@@ -402,7 +465,7 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
   if (o.omega_kind == HQRRP_OMEGA_USER) {
     if (!o.omega) return HQRRP_ERR_ARG;
     HQ_TRY(hipMemcpyAsync(h->G.p, o.omega, sizeof(double) * (size_t)r * m,
-                          hipMemcpyDeviceToDevice, h->stream));
+                          hipMemcpyDeviceToDevice, h->cur));
     return HQRRP_OK;
   }
   if (h->om_r != r || h->om_m != m || h->om_seed != o.seed) {
@@ -416,14 +479,68 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
     h->om_seed = o.seed;
   }
   HQ_TRY(hipMemcpyAsync(h->G.p, h->Gcache.p, sizeof(double) * (size_t)r * m,
-                        hipMemcpyDeviceToDevice, h->stream));
+                        hipMemcpyDeviceToDevice, h->cur));
   return HQRRP_OK;
 }
 
+// Sketch pivots of the block starting at column j (b stages over the n - j
+// remaining sketch columns) into h->piv / h->moves.
+int block_sketch(hqrrp_handle_t h, int r, int j, int n, int b, double* Y) {
+  PhaseScope ps(h, PH_SKETCH_QRCP, 4.0 * r * (double)(n - j) * b,
+                8.0 * r * (double)(n - j), 1);
+  return run_sketch(h, r, n - j, Y + (long)j * r, r, b, h->piv.as<int>(),
+                    h->moves.as<int>());
+}
+
+// Applies the composed moves of the last sketch to columns j.. of A (all m
+// rows), of Y and of jpvt.
+int block_gather(hqrrp_handle_t h, int r, int j, int m, int b, double* dA,
+                 long lda, double* Y, int* djpvt) {
+  PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
+  HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b, dA + (long)j * lda, lda,
+                            m, Y + (long)j * r, r, r, djpvt + j,
+                            h->tmpA.as<double>(), h->tmpY.as<double>(),
+                            h->tmpJ.as<int>(), h->cur));
+  return HQRRP_OK;
+}
+
+// Rows 0..j-1 of the panel columns and jpvt follow the panel's pivots.
+int block_permute(hqrrp_handle_t h, int j, int b, double* dA, long lda,
+                  int* djpvt) {
+  PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * j * b, 1);
+  HQ_TRY(launch_panel_permute(dA + (long)j * lda, lda, j, b, h->perm.as<int>(),
+                              djpvt + j, h->cur));
+  return HQRRP_OK;
+}
+
+int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
+                   const double* V, long ldv, const double* R12, long ldr,
+                   double* Y, double* G, int nb) {
+  PhaseScope ps(h, PH_DOWNDATE,
+                4.0 * r * b * (double)mrows + 2.0 * r * b * nc,
+                8.0 * (2.0 * r * mrows + 2.0 * r * nc), 4);
+  return downdate_y(h, r, b, mrows, nc, V, ldv, R12, ldr, h->T.as<double>(), nb,
+                    Y + (long)(j + b) * r, r, G + (long)j * r, r);
+}
+
+// The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.  With look-ahead the
+// pivot chain of block j+1 is computed from block j's W' = (T'V'C)' before
+// block j's trailing update has run:
+//   R12 rows: a copy of C's top b rows, updated by the same update kernel
+//             (bitwise the rows the full update will write);
+//   downdate and sketch pivots of block j+1 from them;
+//   the b2 selected columns (rows below R12) copied out of the not yet
+//             updated C and updated by the same kernel into a panel buffer;
+// then block j's full update (stream s_upd) runs concurrently with the
+// panel QRCP of block j+1 on the panel buffer (stream s_chain), after which
+// the column gather, the panel write-back and the row permutation follow on
+// the caller's stream.  Every value is produced by the same kernel with the
+// same operands as without look-ahead, so the results are bitwise identical.
 int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
                   int* djpvt, double* dtau, const hqrrp_options& o) {
   const int nb = o.nb, pp = o.pp, r = nb + pp;
   const int mn = std::min(m, n);
+  h->cur = h->stream;
   if (mn == 0) return HQRRP_OK;
   const int kmax = (o.max_rank > 0) ? std::min(mn, o.max_rank) : mn;
   if (o.sketch_prec != HQRRP_SKETCH_FP64) return HQRRP_ERR_UNSUPPORTED;
@@ -439,9 +556,16 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   HQ_RET(ensure(h->tmpA, sizeof(double) * (size_t)2 * nb * m));
   HQ_RET(ensure(h->tmpY, sizeof(double) * (size_t)2 * nb * r));
   HQ_RET(ensure(h->tmpJ, sizeof(int) * (size_t)2 * nb));
+  const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd;
+  if (use_la) {
+    HQ_RET(ensure(h->R12, sizeof(double) * (size_t)nb * n));
+    HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
+    HQ_RET(ensure(h->Wsel, sizeof(double) * (size_t)nb * nb));
+  }
   double* Y = h->Y.as<double>();
   double* G = h->G.as<double>();
   double* T = h->T.as<double>();
+  hipStream_t main_s = h->stream;
 
   // Y = Omega * A (NoFLA_HQRRP_WY_blk_var4.c:407-409).
   {
@@ -450,42 +574,104 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     HQ_RET(gemm(h, false, false, r, n, m, 1.0, G, r, 0, dA, lda, 0, 0.0, Y, r));
   }
 
+  bool have_panel = false;  // block j's panel already factored by look-ahead
   for (int j = 0; j < kmax; j += nb) {
     const int b = std::min(nb, std::min(n - j, m - j));
     const bool last = (j + nb >= m) || (j + nb >= n);
-    const int mrows = m - j;
+    const int mrows = m - j, nc = n - j - b;
     double* Ajj = dA + j + (long)j * lda;
-    if (!last) {
-      {
-        PhaseScope ps(h, PH_SKETCH_QRCP, 4.0 * r * (double)(n - j) * b,
-                      8.0 * r * (double)(n - j), 1);
-        HQ_RET(run_sketch(h, r, n - j, Y + (long)j * r, r, b,
-                          h->piv.as<int>(), h->moves.as<int>()));
+    double* C = Ajj + (long)b * lda;
+    if (!have_panel) {
+      if (!last) {
+        HQ_RET(block_sketch(h, r, j, n, b, Y));
+        HQ_RET(block_gather(h, r, j, m, b, dA, lda, Y, djpvt));
       }
-      PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
-      HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b, dA + (long)j * lda,
-                                lda, m, Y + (long)j * r, r, r, djpvt + j,
-                                h->tmpA.as<double>(), h->tmpY.as<double>(),
-                                h->tmpJ.as<int>(), h->stream));
+      HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
+                       h->perm.as<int>(), T, nb));
+      HQ_RET(block_permute(h, j, b, dA, lda, djpvt));
     }
-    HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
-                     h->perm.as<int>(), T, nb));
+    have_panel = false;
+    if (nc <= 0) continue;
+
+    TrailArgs ta;
+    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, lda, &ta));
+    HQ_RET(trail_w(h, ta));
+
+    const int jn = j + b;
+    int nw_next = 0;
+    const bool la = use_la && jn < kmax &&
+                    panel_geometry(h, m - jn, &nw_next) == HQRRP_OK &&
+                    nw_next <= h->chain_cus;
+    if (!la) {
+      HQ_RET(trail_apply(h, ta));
+      if (!last)
+        HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, C, lda, Y, G,
+                              nb));
+      continue;
+    }
+
+    // jn < kmax implies !last and b == nb.
+    double* R12 = h->R12.as<double>();
+    double* Pn = h->Pn.as<double>();
+    const int mnext = m - jn;
+    const int b2 = std::min(nb, std::min(n - jn, m - jn));
+    const bool last_next = (jn + nb >= m) || (jn + nb >= n);
     {
-      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * j * b, 1);
-      HQ_TRY(launch_panel_permute(dA + (long)j * lda, lda, j, b,
-                                  h->perm.as<int>(), djpvt + j, h->stream));
+      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
+                    16.0 * (double)b * nc, 2);
+      HQ_TRY(launch_copy_block(R12, b, C, lda, b, nc, h->cur));
+      TrailArgs tr = ta;
+      tr.mrows = b;
+      tr.C = R12;
+      tr.ldc = b;
+      HQ_TRY(launch_trail_update(tr, h->cur));
     }
-    if (j + b < n)
-      HQ_RET(apply_qt_left(h, mrows, n - j - b, b, Ajj, lda, T, nb,
-                           dA + j + (long)(j + b) * lda, lda));
-    if (!last) {
-      PhaseScope ps(h, PH_DOWNDATE,
-                    4.0 * r * b * (double)mrows + 2.0 * r * b * (n - j - b),
-                    8.0 * (2.0 * r * mrows + 2.0 * r * (n - j - b)), 4);
-      HQ_RET(downdate_y(h, r, b, mrows, n - j - b, Ajj, lda,
-                        dA + j + (long)(j + b) * lda, lda, T, nb,
-                        Y + (long)(j + b) * r, r, G + (long)j * r, r));
+    HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, R12, b, Y, G, nb));
+    if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
+    {
+      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
+                    16.0 * (double)mnext * b2, 2);
+      HQ_TRY(launch_select_cols(last_next ? nullptr : h->moves.as<int>(), b2, C,
+                                lda, b, mnext, ta.W, ta.ldw, b, Pn, mnext,
+                                h->Wsel.as<double>(), h->cur));
+      TrailArgs tp = ta;
+      tp.mrows = mnext;
+      tp.nc = b2;
+      tp.vmask = 0;
+      tp.V = Ajj + b;
+      tp.C = Pn;
+      tp.ldc = mnext;
+      tp.W = h->Wsel.as<double>();
+      tp.ldw = b2;
+      HQ_TRY(launch_trail_update(tp, h->cur));
     }
+    // Fork: block j's update || block jn's panel QRCP and T factor.
+    HQ_TRY(hipEventRecord(h->ev_fork, main_s));
+    HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_fork, 0));
+    HQ_TRY(hipStreamWaitEvent(h->s_chain, h->ev_fork, 0));
+    h->cur = h->s_upd;
+    int rc = trail_apply(h, ta);
+    if (rc == HQRRP_OK && hipEventRecord(h->ev_upd, h->s_upd) != hipSuccess)
+      rc = HQRRP_ERR_HIP;
+    h->cur = h->s_chain;
+    if (rc == HQRRP_OK)
+      rc = run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
+                     h->perm.as<int>(), T, nb);
+    if (rc == HQRRP_OK && hipEventRecord(h->ev_chain, h->s_chain) != hipSuccess)
+      rc = HQRRP_ERR_HIP;
+    h->cur = main_s;
+    HQ_RET(rc);
+    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
+    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
+    // Join: block jn's columns into place, then its factored panel over them.
+    if (!last_next) HQ_RET(block_gather(h, r, jn, m, b2, dA, lda, Y, djpvt));
+    {
+      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mnext * b2, 1);
+      HQ_TRY(launch_copy_block(dA + jn + (long)jn * lda, lda, Pn, mnext, mnext,
+                               b2, h->cur));
+    }
+    HQ_RET(block_permute(h, jn, b2, dA, lda, djpvt));
+    have_panel = true;
   }
   HQ_TRY(hipGetLastError());
   return HQRRP_OK;
@@ -518,6 +704,7 @@ extern "C" void hqrrp_default_options(hqrrp_options* o) {
   o->seed = 12;
   o->omega = nullptr;
   o->max_rank = 0;
+  o->lookahead = 1;
 }
 
 extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
@@ -537,8 +724,45 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
   // Persistent kernels need all their workgroups co-resident; both kernels
   // are launched at most one workgroup per CU (sketch: register-bound,
   // panel: LDS-bound).
-  h->sketch_max_wg = h->cu_count;
+  h->sketch_max_wg = std::min(h->cu_count, 256);
   h->panel_max_wg = h->cu_count;
+  // Look-ahead streams: a quarter of the CUs for the panel chain, the rest
+  // for the trailing update.  Mask bits are taken in runs of 8 (one run in
+  // four), so that whether the bits map to the 8 XCDs interleaved or in
+  // contiguous ranges, every XCD contributes the same number of CUs to each
+  // stream (workgroups are dealt to the XCDs round-robin).  A panel needing
+  // more workgroups than the chain holds runs without look-ahead.
+  {
+    const int words = (h->cu_count + 31) / 32;
+    std::vector<uint32_t> mc(words, 0u), mu(words, 0u);
+    int nchain = 0;
+    for (int c = 0; c < h->cu_count; ++c) {
+      if ((c / 8) % 4 == 0) {
+        mc[c / 32] |= 1u << (c % 32);
+        ++nchain;
+      } else {
+        mu[c / 32] |= 1u << (c % 32);
+      }
+    }
+    if (nchain > 0 &&
+        hipExtStreamCreateWithCUMask(&h->s_chain, words, mc.data()) ==
+            hipSuccess &&
+        hipExtStreamCreateWithCUMask(&h->s_upd, words, mu.data()) ==
+            hipSuccess) {
+      h->chain_cus = nchain;
+    } else {
+      (void)hipGetLastError();
+      if (h->s_chain) (void)hipStreamDestroy(h->s_chain);
+      if (h->s_upd) (void)hipStreamDestroy(h->s_upd);
+      h->s_chain = h->s_upd = nullptr;
+      h->chain_cus = 0;
+    }
+    if (h->s_chain) {
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, hipEventDisableTiming));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming));
+    }
+  }
   int rc = ensure(h->words, 64);
   if (rc) return rc;
   HQ_TRY(hipMemset(h->words.p, 0, h->words.bytes));
@@ -550,14 +774,20 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
   if (!h) return HQRRP_OK;
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
-  DevBuf* bufs[] = {&h->words, &h->Y,     &h->G,     &h->Gcache, &h->cand,
-                    &h->moves, &h->piv,   &h->ppart, &h->rowbuf, &h->gram,
+  DevBuf* bufs[] = {&h->words,     &h->Y,         &h->G,     &h->Gcache,
+                    &h->xs.buf[0], &h->xs.buf[1], &h->xp.buf[0], &h->xp.buf[1],
+                    &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
-                    &h->hA,    &h->hJ,    &h->hT};
+                    &h->hA,    &h->hJ,    &h->hT,     &h->R12,
+                    &h->Pn,    &h->Wsel};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain})
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t st : {h->s_chain, h->s_upd})
+    if (st) (void)hipStreamDestroy(st);
   if (h->own) (void)hipStreamDestroy(h->own);
   delete h;
   return HQRRP_OK;
@@ -635,6 +865,7 @@ extern "C" int hqrrp_sketch_qrcp_dev(hqrrp_handle_t h, int r, int ncols,
       stages > std::min(r, ncols))
     return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
+  h->cur = h->stream;
   return run_sketch(h, r, ncols, dY, ldy, stages, dpiv, dmoves);
 }
 
@@ -644,6 +875,7 @@ extern "C" int hqrrp_panel_qrcp_dev(hqrrp_handle_t h, int mrows, int b,
                                     int ldt) {
   if (!h || b < 1 || mrows < b || ldp < mrows || ldt < b) return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
+  h->cur = h->stream;
   return run_panel(h, mrows, b, dP, ldp, pivoting, dtau, dperm, dT, ldt);
 }
 
@@ -655,6 +887,7 @@ extern "C" int hqrrp_apply_qt_left_dev(hqrrp_handle_t h, int mrows, int ncols,
       ldt < b)
     return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
+  h->cur = h->stream;
   return apply_qt_left(h, mrows, ncols, b, dV, ldv, dT, ldt, dC, ldc);
 }
 
@@ -667,6 +900,7 @@ extern "C" int hqrrp_downdate_y_dev(hqrrp_handle_t h, int r, int b, int mU,
       ldt < b || ldy < r || ldg < r)
     return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
+  h->cur = h->stream;
   return downdate_y(h, r, b, mU, nY2, dU, ldu, dR12, ldr, dT, ldt, dY2, ldy,
                     dG, ldg);
 }
@@ -677,6 +911,7 @@ extern "C" int hqrrp_dgemm_dev(hqrrp_handle_t h, int ta, int tb, int M, int N,
                                double* dC, int ldc) {
   if (!h || M < 0 || N < 0 || K < 0 || ldc < std::max(1, M)) return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
+  h->cur = h->stream;
   return gemm(h, ta != 0, tb != 0, M, N, K, alpha, dA, lda, 0, dB, ldb, 0,
               beta, dC, ldc);
 }

@@ -55,7 +55,89 @@ __global__ __launch_bounds__(HQ_NT) void gather_in_kernel(
   }
 }
 
+// dst(rows x cols, ld ldd) = src(rows x cols, ld lds).
+__global__ __launch_bounds__(HQ_NT) void copy_block_kernel(double* dst, long ldd,
+                                                           const double* src,
+                                                           long lds, int rows,
+                                                           int cols) {
+  const long i0 = (long)blockIdx.x * kRowsPerBlock;
+  for (long c = blockIdx.y; c < cols; c += gridDim.y) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      long i = i0 + q * HQ_NT + threadIdx.x;
+      if (i < rows) dst[i + c * ldd] = src[i + c * lds];
+    }
+  }
+}
+
+// Look-ahead selection: the b2 columns that the next block's panel will hold
+// (position q <- source position src(q), from the composed pivot moves, or
+// the identity when moves is null) are copied (rows [r0, r0+rows) of C) into
+// P, and the matching rows of W' (nc x b) into Wsel' (b2 x b).
+__global__ __launch_bounds__(HQ_NT) void select_cols_kernel(
+    const int* moves, int b2, const double* C, long ldc, int r0, int rows,
+    const double* Wt, long ldw, int b, double* P, long ldp, double* Wsel) {
+  __shared__ int src[64];
+  const int t = threadIdx.x;
+  if (t < 64) src[t] = t;
+  __syncthreads();
+  if (moves && t == 0) {
+    const int cnt = moves[0];
+    for (int e = 0; e < cnt; ++e) {
+      const int dst = moves[1 + 2 * e];
+      if (dst < b2) src[dst] = moves[2 + 2 * e];
+    }
+  }
+  __syncthreads();
+  const int q = blockIdx.y;
+  const long s = src[q];
+  const long i0 = (long)blockIdx.x * kRowsPerBlock;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    long i = i0 + u * HQ_NT + t;
+    if (i < rows) P[i + q * ldp] = C[(r0 + i) + s * ldc];
+  }
+  if (blockIdx.x == 0 && t < b) Wsel[q + (long)t * b2] = Wt[s + (long)t * ldw];
+}
+
+__global__ __launch_bounds__(HQ_NT) void fill_sentinel_kernel(double* p,
+                                                              size_t n) {
+  const double v = hq_sentinel();
+  for (size_t i = (size_t)blockIdx.x * HQ_NT + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * HQ_NT)
+    p[i] = v;
+}
+
 }  // namespace
+
+hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + HQ_NT - 1) / HQ_NT;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_sentinel_kernel, dim3((unsigned)blocks), dim3(HQ_NT),
+                     0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_block(double* dst, long ldd, const double* src,
+                             long lds, int rows, int cols, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  dim3 grid(hq_cdiv(rows, kRowsPerBlock), cols < 32768 ? cols : 32768);
+  hipLaunchKernelGGL(copy_block_kernel, grid, dim3(HQ_NT), 0, s, dst, ldd, src,
+                     lds, rows, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_select_cols(const int* moves, int b2, const double* C,
+                              long ldc, int r0, int rows, const double* Wt,
+                              long ldw, int b, double* P, long ldp,
+                              double* Wsel, hipStream_t s) {
+  if (b2 <= 0 || b2 > 64 || b > 64) return hipErrorInvalidValue;
+  dim3 grid(rows > 0 ? hq_cdiv(rows, kRowsPerBlock) : 1, b2);
+  hipLaunchKernelGGL(select_cols_kernel, grid, dim3(HQ_NT), 0, s, moves, b2, C,
+                     ldc, r0, rows, Wt, ldw, b, P, ldp, Wsel);
+  return hipGetLastError();
+}
 
 hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
                               long lda, int m, double* Y, long ldy, int r,

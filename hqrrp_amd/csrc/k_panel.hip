@@ -13,7 +13,9 @@
 // written to HBM exactly once and every per-step loop runs on registers; LDS
 // only stages the coalesced load/store and broadcasts the reflector.  At
 // 16384 rows that is 64 workgroups; below 256 rows a single workgroup needs
-// no grid barrier at all.  Each step needs two grid-wide reductions:
+// no cross-workgroup exchange at all.  Each step needs two grid-wide
+// reductions, each a barrier-free exchange (common.h) through slots of its
+// own:
 //   A: the dot products of the reflector with every column (the dlarf
 //      vector w for the pending columns and, for the columns already
 //      factored, the inner products dlarft needs -- same pass, same barrier);
@@ -21,7 +23,9 @@
 //      below rows k and k+1 (Drmac's recompute and the next dlarfg's ||x||)
 //      together with rows k and k+1 themselves.
 // Reductions are done in a fixed order by every workgroup, so all of them
-// hold bit-identical pivot/reflector state without further communication.
+// hold bit-identical pivot/reflector state without further communication;
+// rows k and k+1 come from the slots of the workgroups that own them, in the
+// same memory round trip as the partial sums.
 // Columns stay with their threads; a position->column map records the
 // pivoting and the slab is written back in pivoted order.
 #include "common.h"
@@ -31,50 +35,15 @@ namespace {
 
 constexpr int kMaxB = 64;
 
-// Sum of src[q*stride] for q in [q0, q1): loads issued 16 at a time so a
-// cross-workgroup reduction costs ~one memory round trip per 16 partials;
-// the summation order is fixed, so every workgroup gets the same bits.
-// chunked_sum2 does the same for two arrays with all loads in flight.
-__device__ __forceinline__ double chunked_sum(const double* src, long stride,
-                                              int q0, int q1) {
-  double s = 0.0;
-  for (int base = q0; base < q1; base += 16) {
-    double v[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-      v[i] = (base + i < q1) ? hq_fetch(src + (long)(base + i) * stride) : 0.0;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) s += v[i];
-  }
-  return s;
-}
-
-__device__ __forceinline__ void chunked_sum2(const double* s1p, const double* s2p,
-                                             long stride, int q0, int q1,
-                                             double& o1, double& o2) {
-  double a1 = 0.0, a2 = 0.0;
-  for (int base = q0; base < q1; base += 16) {
-    double v1[16], v2[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const bool ok = base + i < q1;
-      v1[i] = ok ? hq_fetch(s1p + (long)(base + i) * stride) : 0.0;
-      v2[i] = ok ? hq_fetch(s2p + (long)(base + i) * stride) : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      a1 += v1[i];
-      a2 += v2[i];
-    }
-  }
-  o1 = a1;
-  o2 = a2;
-}
-
 constexpr int kPT = 512;          // threads of the panel kernel
 constexpr int kPG = kPT / 64;     // row groups (one per wavefront)
 constexpr int kRT = 32;           // rows per thread (per row group)
 constexpr int kPR = kRT * kPG;    // panel rows held by one workgroup
+constexpr int kXS = 4 * kMaxB;    // exchange slot of one workgroup: A uses
+                                  // [0,64) (dots), B [0,128) (S1, S2 of
+                                  // column c at 2c, 2c+1), [128,192) row kk,
+                                  // [192,256) row kk+1
+constexpr int kChunk = 8;         // workgroups' partials awaited together
 
 // Thread (c, g) = (t % 64, t / 64) holds column c of the 64 consecutive rows
 // row0 + 64g + i, i < 64, in registers x[i]; rows beyond mrows are zero.
@@ -84,12 +53,18 @@ constexpr int kPR = kRT * kPG;    // panel rows held by one workgroup
 __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   __shared__ double stage[kMaxB][129];  // staging of 128 rows x 64 columns
   __shared__ double s_v[kPR];           // reflector on this WG's rows
+  // s_red*: this workgroup's per-group partials (combined by wavefront 0
+  // and published); s_acc*: per-group sums of the published partials.  Two
+  // sets, since wavefront 0 may still be combining the former when the other
+  // wavefronts already hold the exchange's results.
   __shared__ double s_red[kPG][kMaxB], s_red2[kPG][kMaxB];
+  __shared__ double s_acc[kPG][kMaxB], s_acc2[kPG][kMaxB];
   __shared__ double s_d[kMaxB], s_e[kMaxB], s_S2[kMaxB], s_rowk[kMaxB];
   __shared__ double s_full[kMaxB], s_rk[kMaxB], s_rk1[kMaxB];
+  __shared__ double s_rkin[kMaxB], s_rk1in[kMaxB];  // rows kk, kk+1 (reduced)
   __shared__ int s_phys[kMaxB], s_done[kMaxB], s_pos[kMaxB];
   __shared__ double s_tau, s_beta, s_scale;
-  __shared__ int s_pc;
+  __shared__ int s_pc, s_abort;
 
   const int t = threadIdx.x, w = blockIdx.x;
   const int b = a.b, nw = a.nw;
@@ -125,17 +100,21 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     s_phys[t] = t;
     s_done[t] = (t < b) ? 0 : 1;
   }
+  if (t == 0) s_abort = 0;
   __syncthreads();
 
-  unsigned phase = 0;
-  double* partA = a.part;                  // nw x 64
-  double* partB = a.part + (long)nw * 64;  // nw x 128
+  // Exchange e's slot of workgroup q; e = 0: initial norms, then per step
+  // k: A at 1 + 2k, B at 2 + 2k.  Row group g reduces the partials of
+  // workgroups [q0, q1).
+  auto xslot = [&](int e, int q) -> double* {
+    return a.xcur + ((long)e * nw + q) * kXS;
+  };
   const int per = (nw + kPG - 1) / kPG;
   const int q0 = min(nw, g * per), q1 = min(nw, q0 + per);
 
   // Squared norms of my column over rows >= kk+1 (S1) and >= kk+2 (S2), and
   // rows kk, kk+1 (to s_rk / s_rk1); wavefront 0 combines and publishes.
-  auto norm_pass = [&](long kk) {
+  auto norm_pass = [&](long kk, int e) {
     double s1 = 0.0, s2 = 0.0;
     const int r = rel(kk);
     if (r < 0) {  // every row of mine is >= kk+1; S2 differs if kk+1 is mine
@@ -165,25 +144,6 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     s_red[g][cc] = s1;
     s_red2[g][cc] = s2;
     __syncthreads();
-    if (t < b) {  // wavefront 0 publishes (hq_grid_sync precondition)
-      double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-      for (int q = 0; q < kPG; ++q) {
-        a1 += s_red[q][t];
-        a2 += s_red2[q][t];
-      }
-      hq_publish(partB + (long)w * 128 + t, a1);
-      hq_publish(partB + (long)w * 128 + 64 + t, a2);
-      const long lk = kk - row0, lk1 = kk + 1 - row0;
-      if (lk >= 0 && lk < kPR) hq_publish(a.rowbuf + t, s_rk[t]);
-      if (lk1 >= 0 && lk1 < kPR) hq_publish(a.rowbuf + kMaxB + t, s_rk1[t]);
-    }
-  };
-  auto reduce_B = [&]() {
-    if (col_ok)
-      chunked_sum2(partB + cc, partB + 64 + cc, 128, q0, q1, s_red[g][cc],
-                   s_red2[g][cc]);
-    __syncthreads();
     if (t < b) {
       double a1 = 0.0, a2 = 0.0;
 #pragma unroll
@@ -191,20 +151,72 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
         a1 += s_red[q][t];
         a2 += s_red2[q][t];
       }
+      double* my = xslot(e, w);
+      hq_publish(my + 2 * t, a1);
+      hq_publish(my + 2 * t + 1, a2);
+      const long lk = kk - row0, lk1 = kk + 1 - row0;
+      if (lk >= 0 && lk < kPR) hq_publish(my + 128 + t, s_rk[t]);
+      if (lk1 >= 0 && lk1 < kPR) hq_publish(my + 192 + t, s_rk1[t]);
+    }
+  };
+  // Full S1 / S2 of every column in a fixed order (s_full, s_S2) and rows
+  // kk, kk+1 (s_rkin, s_rk1in; row group 0 fetches them with its first
+  // chunk).  Returns false, in every thread, if the exchange failed.
+  auto reduce_B = [&](long kk, int e) -> bool {
+    bool ok = true;
+    if (col_ok) {
+      double a1 = 0.0, a2 = 0.0;
+      for (int base = q0; ok && base < q1; base += kChunk) {
+        const bool rows = g == 0 && base == q0;
+        double v[2 * kChunk + 2];
+        ok = hq_await(v, [&](int i) -> const double* {
+          if (i < 2 * kChunk) {
+            const int q = base + i / 2;
+            return q < q1 ? xslot(e, q) + 2 * cc + (i & 1) : nullptr;
+          }
+          if (!rows) return nullptr;
+          if (i == 2 * kChunk)
+            return kk >= 0 ? xslot(e, (int)(kk / kPR)) + 128 + cc : nullptr;
+          return kk + 1 < a.mrows ? xslot(e, (int)((kk + 1) / kPR)) + 192 + cc
+                                  : nullptr;
+        }, a.err);
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          a1 += v[2 * i];
+          a2 += v[2 * i + 1];
+        }
+        if (rows) {
+          s_rkin[cc] = v[2 * kChunk];
+          s_rk1in[cc] = v[2 * kChunk + 1];
+        }
+      }
+      s_acc[g][cc] = a1;
+      s_acc2[g][cc] = a2;
+    }
+    if (!ok) s_abort = 1;
+    __syncthreads();
+    if (s_abort) return false;
+    if (t < b) {
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) {
+        a1 += s_acc[q][t];
+        a2 += s_acc2[q][t];
+      }
       s_full[t] = a1;
       s_S2[t] = a2;
     }
     __syncthreads();
+    return true;
   };
 
   // Initial column norms (kk = -1): S1 = full squared norms, S2 = rows >= 1.
-  norm_pass(-1);
-  if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
-  reduce_B();
+  norm_pass(-1, 0);
+  if (!reduce_B(-1, 0)) return;
   if (t < b) {
     s_d[t] = sqrt(s_full[t]);
     s_e[t] = s_d[t];
-    s_rowk[t] = hq_fetch(a.rowbuf + kMaxB + t);
+    s_rowk[t] = s_rk1in[t];
   }
   __syncthreads();
 
@@ -278,20 +290,37 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
       s_red[g][cc] = s;
     }
     __syncthreads();
-    if (t < b) {  // wavefront 0 publishes
-      double a1 = 0.0;
-#pragma unroll
-      for (int q = 0; q < kPG; ++q) a1 += s_red[q][t];
-      hq_publish(partA + (long)w * 64 + t, a1);
-    }
-    if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
-    // Phase 4: full dots in a fixed order.
-    if (col_ok) s_red[g][cc] = chunked_sum(partA + cc, 64, q0, q1);
-    __syncthreads();
+    const int eA = 1 + 2 * k;
     if (t < b) {
       double a1 = 0.0;
 #pragma unroll
       for (int q = 0; q < kPG; ++q) a1 += s_red[q][t];
+      hq_publish(xslot(eA, w) + t, a1);
+    }
+    // Phase 4: full dots in a fixed order.
+    {
+      bool ok = true;
+      if (col_ok) {
+        double a1 = 0.0;
+        for (int base = q0; ok && base < q1; base += kChunk) {
+          double v[kChunk];
+          ok = hq_await(v, [&](int i) -> const double* {
+            const int q = base + i;
+            return q < q1 ? xslot(eA, q) + cc : nullptr;
+          }, a.err);
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) a1 += v[i];
+        }
+        s_acc[g][cc] = a1;
+      }
+      if (!ok) s_abort = 1;
+    }
+    __syncthreads();
+    if (s_abort) return;
+    if (t < b) {
+      double a1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPG; ++q) a1 += s_acc[q][t];
       s_full[t] = a1;
     }
     __syncthreads();
@@ -304,15 +333,14 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
     if (k + 1 < b) {
       __syncthreads();
-      norm_pass(k);
-      if (!hq_grid_sync(a.bar, nw, phase, a.err)) return;
-      reduce_B();
+      norm_pass(k, 2 + 2 * k);
+      if (!reduce_B(k, 2 + 2 * k)) return;
       // Phase 6: norm downdate and the next step's alpha / ||x||^2.
       if (t < b && !s_done[t]) {
-        double wt = hq_fetch(a.rowbuf + t);
+        const double wt = s_rkin[t];
         if (a.pivoting)
           hq_downdate_norm(s_d[t], s_e[t], wt, s_full[t], a.mrows - k - 1);
-        if (k + 1 < a.mrows) s_rowk[t] = hq_fetch(a.rowbuf + kMaxB + t);
+        if (k + 1 < a.mrows) s_rowk[t] = s_rk1in[t];
       }
       __syncthreads();
     }
@@ -337,7 +365,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     __syncthreads();
   }
   if (w == 0 && t < b) a.perm[t] = s_phys[t];
-  hq_grid_exit(a.bar, nw);
+  hq_xreset(a.xprev, a.xprev_len, w, nw, kPT);
 }
 
 // dlarft('Forward','Columnwise') from the reflector inner products
@@ -427,6 +455,10 @@ hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t stream) {
 }
 
 int panel_rows_per_wg() { return kPR; }
+
+size_t panel_xchg_doubles(int nw, int b) {
+  return (size_t)(2 * b + 1) * nw * kXS;
+}
 
 hipError_t launch_larft(int b, const double* gram, const double* tau,
                         double* T, long ldt, hipStream_t stream) {

@@ -9,16 +9,19 @@
 // steps; a 512-thread workgroup holds 256 columns, so n = 16384 needs 64
 // workgroups.  A step is:
 //   1. per-workgroup argmax of the partial norms (first position on ties),
-//      the winning column published to global memory;
-//   2. one grid barrier;
-//   3. every workgroup reduces the candidates identically, builds the same
-//      Householder reflector from the winning column and applies it, the
-//      LAWN-176 norm downdate, and the bookkeeping of positions, to its own
-//      columns.
+//      the winning column published to this step's exchange slot;
+//   2. every workgroup reads every slot header as soon as it is written (no
+//      barrier: the exchange in common.h), reduces the candidates
+//      identically, builds the same Householder reflector from the winner's
+//      header and rows and applies it, the LAWN-176 norm downdate, and the
+//      bookkeeping of positions, to its own columns.  Reading only the
+//      winner's rows (one more round trip) is cheaper than polling every
+//      candidate's rows: the polling traffic of 64 workgroups x 64 columns
+//      slows the exchange more than the round trip costs.
 // Columns never move: a column's *position* is tracked instead, which is what
 // the reference's swap-not-shift semantics and idamax's first-index tie rule
-// depend on.  At the end each moved column appends (dst, src) to a list, the
-// composed permutation that the gather kernel applies to A, Y and jpvt.
+// depend on.  At the end workgroup 0 composes the b swaps into the list of
+// (dst, src) moves that the gather kernel applies to A, Y and jpvt.
 #include <climits>
 
 #include "common.h"
@@ -26,10 +29,11 @@
 
 namespace {
 
-constexpr int kSlotHead = 8;  // candidate slot: [key, pos, ...pad, column]
+constexpr int kSlotHead = 8;  // slot: [key, pos, alpha, ||x||^2, pad, rows]
 constexpr int kThreads = 512;  // 8 wavefronts, 32 columns each
 constexpr int kCols = 256;     // sketch columns per workgroup
-constexpr int kMaxNw = 64;     // workgroups whose candidates are prefetched
+constexpr int kMaxWg = 256;    // workgroups of one launch at most
+constexpr int kMaxStages = 80;
 
 // Column c of the workgroup lives in wavefront c/32; lane c%32 holds the
 // first RMAX/2 of the column's *active* rows and lane c%32+32 the next
@@ -39,6 +43,13 @@ constexpr int kMaxNw = 64;     // workgroups whose candidates are prefetched
 // and every per-row operation uses static register indices with no tests
 // against k.  Rows at or beyond r are zero.  A column's dot product is one
 // cross-lane add (a+b == b+a, both lanes get the same bits).
+//
+// Exchange slot of a workgroup at step k: [0] key (partial norm of its best
+// column, -1 if none), [1] its position, [2] its row k (alpha of the next
+// reflector), [3] the squared norm of its rows below k, [kSlotHead + i] its
+// row k + i.  Every workgroup polls the headers, picks the same winner,
+// builds the reflector from the winner's alpha and norm, and reads the
+// winner's rows.
 template <int RMAX>
 __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   constexpr int H = RMAX / 2;
@@ -51,13 +62,14 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   const int ibase = half * H;  // first active row (relative) of this lane
   const long slot = kSlotHead + RMAX;
 
-  __shared__ double s_v[RMAX];    // reflector on the active rows (relative)
-  __shared__ double s_col[RMAX];  // this workgroup's candidate column
-  __shared__ double s_all[kMaxNw][RMAX];  // every workgroup's candidate
+  __shared__ double s_v[RMAX];  // reflector on the active rows (relative)
+  __shared__ double s_slot[kSlotHead + RMAX];  // this workgroup's slot
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
   __shared__ int s_win_pos, s_win_wg;
-  __shared__ double s_tau;
+  __shared__ double s_tau, s_scale;
+  __shared__ int s_piv[kMaxStages];
+  __shared__ int s_abort;
 
   double x[H];
 #pragma unroll
@@ -70,149 +82,159 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   for (int q = 0; q < H; ++q) ss += x[q] * x[q];
   ss += __shfl_xor(ss, 32, 64);
   double d = sqrt(ss), e = d;
+  // Squared norm of the rows below the first active one (the ||x||^2 of
+  // this column's reflector, should it win).
+  double sqn;
+  {
+    double s1 = 0.0;
+#pragma unroll
+    for (int q = 1; q < H; ++q) s1 += x[q] * x[q];
+    s1 += half ? x[0] * x[0] : 0.0;
+    sqn = s1 + __shfl_xor(s1, 32, 64);
+  }
   int pos = own ? c : -1;  // current position; -1 once chosen as a pivot
-  int done_at = -1;
 
-  if (w == 0 && t == 0) a.moves[0] = 0;
-  unsigned phase = 0;
+  if (t == 0) s_abort = 0;
+  __syncthreads();
 
   for (int k = 0; k < a.stages; ++k) {
-    double* cbuf = a.cand + (long)(k & 1) * a.nw * slot;
+    const double* cbuf = a.xcur + (long)k * a.nw * slot;
+    double* mine = a.xcur + ((long)k * a.nw + w) * slot;
     const int span = r - k;  // active rows
-    // 1. local argmax: largest norm, smallest position among equals.
-    double key = (pos >= 0) ? d : -1.0;
-    int kp = (pos >= 0) ? pos : INT_MAX;
+    // 1. local argmax: largest norm, smallest position among equals; every
+    //    thread combines the 8 wavefront winners identically and the
+    //    winning column's lanes publish the slot.
+    {
+      double key = (pos >= 0) ? d : -1.0;
+      int kp = (pos >= 0) ? pos : INT_MAX;
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1) {
-      double ok = __shfl_xor(key, off, 64);
-      int op = __shfl_xor(kp, off, 64);
-      if (ok > key || (ok == key && op < kp)) {
-        key = ok;
-        kp = op;
+      for (int off = 16; off > 0; off >>= 1) {
+        double ok = __shfl_xor(key, off, 64);
+        int op = __shfl_xor(kp, off, 64);
+        if (ok > key || (ok == key && op < kp)) {
+          key = ok;
+          kp = op;
+        }
+      }
+      if (lane == 0) {
+        s_key[wave] = key;
+        s_pos[wave] = kp;
       }
     }
-    if (lane == 0) {
-      s_key[wave] = key;
-      s_pos[wave] = kp;
-    }
     __syncthreads();
-    if (t == 0) {
+    {
       double bk = s_key[0];
-      int bp = s_pos[0];
+      int bp = s_pos[0], bq = 0;
+#pragma unroll
       for (int q = 1; q < 8; ++q)
         if (s_key[q] > bk || (s_key[q] == bk && s_pos[q] < bp)) {
           bk = s_key[q];
           bp = s_pos[q];
+          bq = q;
         }
-      hq_publish(cbuf + (long)w * slot + 0, bk);
-      hq_publish(cbuf + (long)w * slot + 1, (double)bp);
-      s_win_pos = bp;
-    }
-    __syncthreads();
-    if (pos >= 0 && pos == s_win_pos) {
+      if (bp == INT_MAX) {  // no candidate here
+        if (t == 0) {
+          hq_publish(mine + 0, -1.0);
+          hq_publish(mine + 1, (double)INT_MAX);
+          hq_publish(mine + 2, 0.0);
+          hq_publish(mine + 3, 0.0);
+        }
+      } else if (wave == bq) {
+        // The winner's two lanes stage the slot in LDS and their wavefront
+        // publishes it with two coalesced stores (the LDS traffic stays
+        // inside this wavefront, ordered by wavefront-scope fences).
+        if (pos == bp) {
 #pragma unroll
-      for (int q = 0; q < H; ++q) s_col[ibase + q] = x[q];
+          for (int q = 0; q < H; ++q) s_slot[kSlotHead + ibase + q] = x[q];
+          if (half == 0) {
+            s_slot[0] = bk;
+            s_slot[1] = (double)bp;
+            s_slot[2] = x[0];
+            s_slot[3] = sqn;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < 4 || (lane >= kSlotHead && lane < slot))
+          hq_publish(mine + lane, s_slot[lane]);
+        if (lane + 64 < slot) hq_publish(mine + lane + 64, s_slot[lane + 64]);
+      }
     }
-    __syncthreads();
-    // Wavefront 0 publishes the candidate's active rows (hq_grid_sync
-    // precondition).
-    if (wave == 0 && s_win_pos != INT_MAX) {
-      double* dst = cbuf + (long)w * slot + kSlotHead;
-      if (lane < span) hq_publish(dst + lane, s_col[lane]);
-      if (lane + 64 < span) hq_publish(dst + lane + 64, s_col[lane + 64]);
-    }
-    // 2. grid barrier
-    if (!hq_grid_sync(a.bar, a.nw, phase, a.err)) return;
-    // 3. global argmax over the workgroup candidates (wavefront 0) while the
-    //    other wavefronts fetch every candidate column into LDS in the same
-    //    memory round trip (when nw <= kMaxNw); the winner's column is then
-    //    already local.
-    const bool prefetch = a.nw <= kMaxNw;
+    // 2. wavefront 0: every header, the global winner and the reflector's
+    //    scalars (dlarfg needs only alpha and ||x||^2).
+    bool ok = true;
     if (wave == 0) {
-      double bk = -2.0;
+      double bk = -2.0, ba = 0.0, bs = 0.0;
       int bp = INT_MAX, bw = -1;
-      for (int q = lane; q < a.nw; q += 64) {
-        double kq = hq_fetch(cbuf + (long)q * slot + 0);
-        int pq = (int)hq_fetch(cbuf + (long)q * slot + 1);
-        if (kq > bk || (kq == bk && pq < bp)) {
-          bk = kq;
+      for (int q = lane; ok && q - lane < a.nw; q += 64) {
+        double hv[4];
+        ok = hq_await(hv, [&](int i) -> const double* {
+          return q < a.nw ? cbuf + (long)q * slot + i : nullptr;
+        }, a.err);
+        const int pq = (int)hv[1];
+        if (q < a.nw && (hv[0] > bk || (hv[0] == bk && pq < bp))) {
+          bk = hv[0];
           bp = pq;
           bw = q;
+          ba = hv[2];
+          bs = hv[3];
         }
       }
+      int bl = lane;  // lane holding the winner's alpha and norm
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
-        double ok = __shfl_xor(bk, off, 64);
+        double okk = __shfl_xor(bk, off, 64);
         int op = __shfl_xor(bp, off, 64);
         int ow = __shfl_xor(bw, off, 64);
-        if (ok > bk || (ok == bk && op < bp)) {
-          bk = ok;
+        int ol = __shfl_xor(bl, off, 64);
+        if (okk > bk || (okk == bk && op < bp)) {
+          bk = okk;
           bp = op;
           bw = ow;
+          bl = ol;
         }
       }
+      ba = __shfl(ba, bl, 64);
+      bs = __shfl(bs, bl, 64);
       if (lane == 0) {
+        hq_reflector h = hq_dlarfg(ba, bs);
+        s_tau = h.tau;
+        s_scale = h.scale;
         s_win_pos = bp;
         s_win_wg = bw;
+        s_piv[k] = bp;
+        if (w == 0) a.piv[k] = bp;
       }
-    } else if (prefetch) {
-      constexpr int kPF = (kMaxNw * RMAX + kThreads - 65) / (kThreads - 64);
-      const int total = a.nw * span;
-      double vbuf[kPF];
-#pragma unroll
-      for (int u = 0; u < kPF; ++u) {
-        const int e2 = (t - 64) + u * (kThreads - 64);
-        vbuf[u] = 0.0;
-        if (e2 < total) {
-          const int q = e2 / span, i = e2 % span;
-          vbuf[u] = hq_fetch(cbuf + (long)q * slot + kSlotHead + i);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kPF; ++u) {
-        const int e2 = (t - 64) + u * (kThreads - 64);
-        if (e2 < total) s_all[e2 / span][e2 % span] = vbuf[u];
-      }
+      if (!ok) s_abort = 1;
     }
     __syncthreads();
+    if (s_abort) return;
     const int p = s_win_pos;
-    // 4. winning column (active rows) into wavefront 0's registers;
-    // 5. reflector from a fixed-order wavefront reduction, identical in every
-    //    workgroup; v (relative rows, zero beyond span) into LDS.
-    if (wave == 0) {
-      const int wg = s_win_wg;
-      const double* src = cbuf + (long)wg * slot + kSlotHead;
-      const int i0 = lane, i1 = lane + 64;
-      double x0 = 0.0, x1 = 0.0;
-      if (prefetch) {
-        if (i0 < span) x0 = s_all[wg][i0];
-        if (i1 < span) x1 = s_all[wg][i1];
-      } else {
-        if (i0 < span) x0 = hq_fetch(src + i0);
-        if (i1 < span) x1 = hq_fetch(src + i1);
+    // 3. the winner's rows below k, scaled, as the reflector (relative rows,
+    //    1 at row k, zero beyond span).
+    if (t < RMAX) {
+      double v = 0.0;
+      if (t > 0 && t < span) {
+        double cv[1];
+        const double* src = cbuf + (long)s_win_wg * slot + kSlotHead + t;
+        ok = hq_await(cv, [&](int) -> const double* { return src; }, a.err);
+        v = cv[0] * s_scale;
       }
-      const double alpha = __shfl(x0, 0, 64);
-      double sq = ((i0 > 0) ? x0 * x0 : 0.0) + x1 * x1;
-      sq = hq_wave_sum(sq);
-      hq_reflector h = hq_dlarfg(alpha, sq);
-      if (i0 < RMAX) s_v[i0] = (i0 == 0) ? 1.0 : x0 * h.scale;
-      if (i1 < RMAX) s_v[i1] = x1 * h.scale;
-      if (lane == 0) {
-        s_tau = h.tau;
-        if (w == 0) a.piv[k] = p;
-      }
+      s_v[t] = (t == 0) ? 1.0 : v;
+      if (!ok) s_abort = 1;
     }
     __syncthreads();
-    // 6. bookkeeping of positions (swap of positions k and p).
+    if (s_abort) return;
+    // 4. bookkeeping of positions (swap of positions k and p).
     if (pos >= 0) {
       if (pos == p) {
         pos = -1;
-        done_at = k;
       } else if (pos == k) {
         pos = p;
       }
     }
-    // 7. apply the reflector to my column, downdate its norm, and shift the
+    // 5. apply the reflector to my column, downdate its norm, and shift the
     //    active rows up by one (row k leaves).  Both lanes of a column take
     //    the same branches (pos is per column).
     if (pos >= 0) {
@@ -226,13 +248,19 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 #pragma unroll
         for (int q = 0; q < H; ++q) x[q] += f * s_v[ibase + q];
       }
-      double b2 = 0.0;
+      // b2: rows below k (Drmac's recompute); s3: rows below k+1 (the next
+      // step's ||x||^2).
+      double b2 = 0.0, s3 = 0.0;
 #pragma unroll
       for (int q = 1; q < H; ++q) b2 += x[q] * x[q];
+#pragma unroll
+      for (int q = 2; q < H; ++q) s3 += x[q] * x[q];
       const double x0 = x[0];
       const double up = __shfl_xor(x0, 32, 64);  // partner's first row
       b2 += half ? x0 * x0 : 0.0;
+      s3 += half ? x[1] * x[1] + x0 * x0 : 0.0;
       b2 += __shfl_xor(b2, 32, 64);
+      sqn = s3 + __shfl_xor(s3, 32, 64);
       const double wt = half ? up : x0;  // row k
       hq_downdate_norm(d, e, wt, b2, r - k - 1);
 #pragma unroll
@@ -240,22 +268,70 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       x[H - 1] = half ? 0.0 : up;
     }
   }
-  // Composed permutation: final position of every column that moved.
-  if (own && half == 0) {
-    const int fp = (done_at >= 0) ? done_at : pos;
-    if (fp != c) {
-      int idx = atomicAdd(a.moves, 1);
-      a.moves[1 + 2 * idx] = fp;
-      a.moves[2 + 2 * idx] = c;
+  // Composed permutation (workgroup 0, wavefront 0): replay the swaps of
+  // positions k <-> piv[k] on a map position -> column held in registers
+  // (lane l owns map entries l, l+64, l+128; at most 2*stages entries), then
+  // emit (position, column) for every entry that moved.
+  if (w == 0 && wave == 0) {
+    int key[3] = {-1, -1, -1}, val[3] = {0, 0, 0};
+    int cnt = 0;
+    auto entry = [&](int ps) -> int {
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const unsigned long long m = __ballot(key[u] == ps);
+        if (m) return 64 * u + __builtin_ctzll(m);
+      }
+      const int sidx = cnt++;
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (sidx >> 6 == u && lane == (sidx & 63)) {
+          key[u] = ps;
+          val[u] = ps;
+        }
+      return sidx;
+    };
+    auto get = [&](int sidx) -> int {
+      int v = 0;
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (sidx >> 6 == u) v = __shfl(val[u], sidx & 63, 64);
+      return v;
+    };
+    auto put = [&](int sidx, int v) {
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (sidx >> 6 == u && lane == (sidx & 63)) val[u] = v;
+    };
+    for (int k = 0; k < a.stages; ++k) {
+      const int p = s_piv[k];
+      if (p == k) continue;
+      const int ia = entry(k), ib = entry(p);
+      const int va = get(ia), vb = get(ib);
+      put(ia, vb);
+      put(ib, va);
     }
+    int base = 0;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const bool mv = key[u] >= 0 && val[u] != key[u];
+      const unsigned long long m = __ballot(mv);
+      if (mv) {
+        const int idx =
+            base + __popcll(m & ((1ull << lane) - 1ull));
+        a.moves[1 + 2 * idx] = key[u];
+        a.moves[2 + 2 * idx] = val[u];
+      }
+      base += __popcll(m);
+    }
+    if (lane == 0) a.moves[0] = base;
   }
-  hq_grid_exit(a.bar, a.nw);
+  hq_xreset(a.xprev, a.xprev_len, w, a.nw, kThreads);
 }
 
 }  // namespace
 
-size_t sketch_cand_doubles(int nw, int rmax) {
-  return (size_t)2 * nw * (kSlotHead + rmax);
+size_t sketch_xchg_doubles(int nw, int rmax, int stages) {
+  return (size_t)stages * nw * (kSlotHead + rmax);
 }
 
 int sketch_cols_per_wg() { return kCols; }
@@ -268,6 +344,8 @@ int sketch_rmax(int r) {
 
 hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax,
                               hipStream_t stream) {
+  if (a.nw < 1 || a.nw > kMaxWg || a.stages < 1 || a.stages > kMaxStages)
+    return hipErrorInvalidValue;
   dim3 grid(a.nw), block(kThreads);
   if (rmax == 32)
     hipLaunchKernelGGL(sketch_qrcp_kernel<32>, grid, block, 0, stream, a);

@@ -22,7 +22,6 @@
 namespace {
 
 constexpr int kTile = 64;  // output tile edge (rows and columns)
-constexpr int kPitch = 72; // LDS row pitch in doubles
 constexpr int kBK = 32;    // K rows staged per step in trail_vtc
 
 __device__ __forceinline__ double vmask(double v, long i, long c) {
@@ -52,7 +51,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
   const int b = a.b;
   const int nchunks = (b + kUK - 1) / kUK;
   const long m0 = (long)blockIdx.x * kUT, n0 = (long)blockIdx.y * kUT;
-  const bool masked = m0 < b;  // only the top tile holds R / the diagonal
+  const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
 
   // acc[i][j] element r: C'(n, m), n = n0+64wn+16i+l/16+4r, m = m0+64wm+16j+l%16,
   // i.e. cbase[16j + (16i+4r)*ldc] with the lane's base below.
@@ -185,8 +184,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
       int idx = t + HQ_NT * q;
       int k = idx & (kBK - 1), p = idx / kBK;
       long gk = k0 + k;
-      rv[q] = (gk < kend && p < b) ? vmask(a.V[gk + (long)p * a.ldv], gk, p)
-                                   : 0.0;
+      double vv = (gk < kend && p < b) ? a.V[gk + (long)p * a.ldv] : 0.0;
+      rv[q] = a.vmask ? vmask(vv, gk, p) : vv;
       long gn = n0 + p;
       rc[q] = (gk < kend && gn < a.nc) ? a.C[gk + gn * a.ldc] : 0.0;
     }

@@ -8,10 +8,14 @@ struct SketchArgs {
   const double* Y;  // sketch columns to pivot: r x ncols, ld ldy
   long ldy;
   int r, ncols, stages, nw;
-  int* piv;      // stages: position swapped with position k at step k
-  int* moves;    // [0] = count, then (dst, src) position pairs
-  double* cand;  // 2 * nw * (8 + rmax) doubles of scratch
-  unsigned* bar;
+  int* piv;    // stages: position swapped with position k at step k
+  int* moves;  // [0] = count, then (dst, src) position pairs
+  // Exchange buffers (common.h): xcur holds only the sentinel on entry
+  // (sketch_xchg_doubles(nw, rmax, stages) of it are used); the sentinel is
+  // restored over xprev[0, xprev_len).
+  double* xcur;
+  double* xprev;
+  long xprev_len;
   int* err;
 };
 
@@ -23,9 +27,10 @@ struct PanelArgs {
   double* tau;     // b
   int* perm;       // b: original column at each position (0-based)
   double* gram;    // b x b, strictly upper part: v_l' v_k
-  double* part;    // scratch: nw * 192 doubles (coherent exchange)
-  double* rowbuf;  // scratch: 128 doubles (coherent exchange)
-  unsigned* bar;
+  // Exchange buffers as in SketchArgs; panel_xchg_doubles(nw, b) used.
+  double* xcur;
+  double* xprev;
+  long xprev_len;
   int* err;
 };
 
@@ -45,6 +50,7 @@ struct GemmArgs {
 
 struct TrailArgs {
   int mrows, nc, b, splits;
+  int vmask;        // 1: V's top b x b block is unit lower (masked)
   const double* V;  // mrows x b reflectors (unit lower trapezoidal), ld ldv
   long ldv;
   double* C;  // mrows x nc trailing matrix, ld ldc
@@ -64,7 +70,7 @@ hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
 int trail_kbk();
 
 // k_sketch.hip
-size_t sketch_cand_doubles(int nw, int rmax);
+size_t sketch_xchg_doubles(int nw, int rmax, int stages);
 int sketch_rmax(int r);
 int sketch_cols_per_wg();
 hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
@@ -72,6 +78,7 @@ hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
 // k_panel.hip
 hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t s);
 int panel_rows_per_wg();
+size_t panel_xchg_doubles(int nw, int b);
 hipError_t launch_larft(int b, const double* gram, const double* tau,
                         double* T, long ldt, hipStream_t s);
 hipError_t launch_panel_permute(double* A, long lda, int nrows, int b,
@@ -85,6 +92,13 @@ hipError_t launch_reduce_splitk(int M, int N, int S, const double* part,
                                 hipStream_t s);
 
 // k_misc.hip
+hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s);
+hipError_t launch_copy_block(double* dst, long ldd, const double* src,
+                             long lds, int rows, int cols, hipStream_t s);
+hipError_t launch_select_cols(const int* moves, int b2, const double* C,
+                              long ldc, int r0, int rows, const double* Wt,
+                              long ldw, int b, double* P, long ldp,
+                              double* Wsel, hipStream_t s);
 hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
                               long lda, int m, double* Y, long ldy, int r,
                               int* jpvt, double* tmpA, double* tmpY, int* tmpJ,

@@ -85,6 +85,8 @@ typedef struct {
   unsigned seed;       /* seed of the reference stream (reference: 12)       */
   const double *omega; /* device pointer when omega_kind == HQRRP_OMEGA_USER */
   int max_rank;        /* stop after this many columns (<=0: min(m,n))       */
+  int lookahead;       /* 1: overlap the next block's panel with the trailing
+                          update (results bitwise equal to 0)                */
 } hqrrp_options;
 
 /* Fills *opt with the reference defaults (64, 10, 1, FP64, seed 12). */
@@ -127,8 +129,9 @@ void hqrrp_reference_omega(int r, int m, double *G, unsigned seed);
  * (NoFLA_QRPmod_WY_unb_var4 with pivoting=1, num_stages=stages, on V_R,
  * NoFLA_HQRRP_WY_blk_var4.c:500-506 and 774-886).  dY: r x ncols (ld ldy) is
  * read, not modified.  dpiv[k] receives the position (0-based, relative) that
- * is swapped with position k at step k.  dmoves receives the composed
- * permutation: dmoves[0] = count, then (dst, src) position pairs. */
+ * is swapped with position k at step k.  dmoves (1 + 4*stages ints)
+ * receives the composed permutation: dmoves[0] = count (<= 2*stages), then
+ * (dst, src) position pairs.  r <= 80, stages <= min(r, ncols). */
 int hqrrp_sketch_qrcp_dev(hqrrp_handle_t h, int r, int ncols,
                           const double *dY, int ldy, int stages, int *dpiv,
                           int *dmoves);

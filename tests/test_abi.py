@@ -38,6 +38,7 @@ def test_default_options_are_reference_defaults():
     hqrrp_amd.lib().hqrrp_default_options(C.byref(o))
     assert (o.nb, o.pp, o.panel_pivoting, o.seed) == (64, 10, 1, 12)
     assert (o.sketch_prec, o.omega_kind, o.max_rank) == (0, 0, 0)
+    assert o.lookahead == 1
 
 
 def _call_dgeqp4(m, n, lda, lwork):

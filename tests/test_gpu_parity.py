@@ -20,7 +20,7 @@ torch = pytest.importorskip("torch")
 EPS = np.finfo(np.float64).eps
 
 
-def run_gpu(handle, A, nb, pp, pv):
+def run_gpu(handle, A, nb, pp, pv, **kw):
     import hqrrp_amd
 
     m, n = A.shape
@@ -28,7 +28,7 @@ def run_gpu(handle, A, nb, pp, pv):
     jp = torch.arange(1, n + 1, dtype=torch.int32, device="cuda")
     tau = torch.zeros(max(1, min(m, n)), dtype=torch.float64, device="cuda")
     hqrrp_amd.hqrrp_device(handle, dA, jp, tau, m, n, nb=nb, pp=pp,
-                           panel_pivoting=pv)
+                           panel_pivoting=pv, **kw)
     handle.synchronize()
     torch.cuda.synchronize()
     return (hqrrp_amd.from_colmajor(dA), jp.cpu().numpy(),
@@ -106,3 +106,26 @@ def test_deterministic(handle):
     a1, j1, t1 = run_gpu(handle, A, 64, 10, 1)
     a2, j2, t2 = run_gpu(handle, A, 64, 10, 1)
     assert np.array_equal(j1, j2) and np.array_equal(a1, a2)
+
+
+LOOKAHEAD_CASES = [
+    # (m, n, nb, pp, panel_pivoting, max_rank, seed)
+    (1300, 1300, 64, 10, 1, 0, 41),
+    (1000, 1500, 64, 10, 1, 0, 42),   # wide: last block ends at m
+    (1500, 1000, 32, 8, 1, 0, 43),    # tall
+    (1000, 1000, 16, 8, 0, 0, 44),    # no panel pivoting
+    (1100, 1030, 64, 10, 1, 0, 45),   # ragged last block
+    (1200, 1200, 64, 10, 1, 300, 46),  # truncated
+]
+
+
+@pytest.mark.parametrize("m,n,nb,pp,pv,kmax,seed", LOOKAHEAD_CASES)
+def test_lookahead_bitwise(handle, m, n, nb, pp, pv, kmax, seed):
+    """Overlapping the next panel with the trailing update changes nothing."""
+    print(f"lookahead {m}x{n} nb={nb} pv={pv} max_rank={kmax}", flush=True)
+    A = matgen.graded(m, n, seed)
+    a0, j0, t0 = run_gpu(handle, A, nb, pp, pv, max_rank=kmax, lookahead=0)
+    a1, j1, t1 = run_gpu(handle, A, nb, pp, pv, max_rank=kmax, lookahead=1)
+    assert np.array_equal(j0, j1)
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(a0, a1)
