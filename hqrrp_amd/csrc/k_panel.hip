@@ -10,24 +10,23 @@
 // Design (MI355X): one persistent launch per panel.  Workgroup w owns 256
 // rows of all b columns, resident in registers for the whole factorisation
 // (thread = one column x 32 consecutive rows), so the panel is read from and
-// written to HBM exactly once and every per-step loop runs on registers; LDS
-// only stages the coalesced load/store and broadcasts the reflector.  At
-// 16384 rows that is 64 workgroups; below 256 rows a single workgroup needs
-// no cross-workgroup exchange at all.  Each step needs two grid-wide
-// reductions, each a barrier-free exchange (common.h) through slots of its
-// own:
-//   A: the dot products of the reflector with every column (the dlarf
-//      vector w for the pending columns and, for the columns already
-//      factored, the inner products dlarft needs -- same pass, same barrier);
-//   B: after the rank-1 update, the squared norms of every pending column
-//      below rows k and k+1 (Drmac's recompute and the next dlarfg's ||x||)
-//      together with rows k and k+1 themselves.
-// Reductions are done in a fixed order by every workgroup, so all of them
-// hold bit-identical pivot/reflector state without further communication;
-// rows k and k+1 come from the slots of the workgroups that own them, in the
-// same memory round trip as the partial sums.
-// Columns stay with their threads; a position->column map records the
-// pivoting and the slab is written back in pivoted order.
+// written to HBM exactly once.  A step needs ONE grid-wide reduction (a
+// barrier-free exchange, common.h): with p the pivot column and
+// v = [1; scale * x_p(k+1:)] its reflector,
+//   v' x_c = x_c(k) + scale * G_c,   G_c = x_p(k+1:)' x_c(k+1:),
+// so every workgroup publishes its partial G_c for all columns (G_p is the
+// ||x||^2 that dlarfg needs) and the owner of row k publishes row k.  From
+// those every wavefront computes, redundantly and with identical bits, the
+// reflector, the dots (for the pending columns: dlarf's w; for the factored
+// columns: the inner products dlarft needs), row k of R (x_c(k) - tau * dot),
+// the norm downdate and the next pivot -- no workgroup barrier beyond the
+// exchange itself.  Only when a downdate loses accuracy (LAWN 176's
+// recompute test) is a second exchange run, for the fresh column norms.
+// The reflector entries reach the other columns' lanes by readlane from the
+// pivot column's lane (each wavefront holds all b columns of its rows).
+// Reductions are done in a fixed order everywhere.  Columns stay with their
+// threads; a position -> column map records the pivoting and the slab is
+// written back in pivoted order.
 #include "common.h"
 #include "kernels.h"
 
@@ -39,32 +38,32 @@ constexpr int kPT = 512;          // threads of the panel kernel
 constexpr int kPG = kPT / 64;     // row groups (one per wavefront)
 constexpr int kRT = 32;           // rows per thread (per row group)
 constexpr int kPR = kRT * kPG;    // panel rows held by one workgroup
-constexpr int kXS = 4 * kMaxB;    // exchange slot of one workgroup: A uses
-                                  // [0,64) (dots), B [0,128) (S1, S2 of
-                                  // column c at 2c, 2c+1), [128,192) row kk,
-                                  // [192,256) row kk+1
+constexpr int kXS = 2 * kMaxB;    // exchange slot of one workgroup: [0,64)
+                                  // partial sums per column, [64,128) row k
+                                  // (owner only)
 constexpr int kChunk = 8;         // workgroups' partials awaited together
 
-// Thread (c, g) = (t % 64, t / 64) holds column c of the 64 consecutive rows
-// row0 + 64g + i, i < 64, in registers x[i]; rows beyond mrows are zero.
+// Lane `lane`'s value of v (lane uniform across the wavefront).
+__device__ __forceinline__ double hq_readlane(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Thread (c, g) = (t % 64, t / 64) holds column c of the 32 consecutive rows
+// row0 + 32g + i, i < 32, in registers x[i]; rows beyond mrows are zero.
 // Relative to the current step k, a wavefront's rows are either all above
-// k (finished: skipped), all at or below k+2 (branch-free fast path), or they
-// straddle k, k+1 (masked path, at most two wavefronts per step).
+// k (finished: skipped), all below k (branch-free fast path), or contain row
+// k (masked path, one wavefront of one workgroup per step).
 __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   __shared__ double stage[kMaxB][129];  // staging of 128 rows x 64 columns
-  __shared__ double s_v[kPR];           // reflector on this WG's rows
-  // s_red*: this workgroup's per-group partials (combined by wavefront 0
-  // and published); s_acc*: per-group sums of the published partials.  Two
-  // sets, since wavefront 0 may still be combining the former when the other
-  // wavefronts already hold the exchange's results.
-  __shared__ double s_red[kPG][kMaxB], s_red2[kPG][kMaxB];
-  __shared__ double s_acc[kPG][kMaxB], s_acc2[kPG][kMaxB];
-  __shared__ double s_d[kMaxB], s_e[kMaxB], s_S2[kMaxB], s_rowk[kMaxB];
-  __shared__ double s_full[kMaxB], s_rk[kMaxB], s_rk1[kMaxB];
-  __shared__ double s_rkin[kMaxB], s_rk1in[kMaxB];  // rows kk, kk+1 (reduced)
-  __shared__ int s_phys[kMaxB], s_done[kMaxB], s_pos[kMaxB];
-  __shared__ double s_tau, s_beta, s_scale;
-  __shared__ int s_pc, s_abort;
+  __shared__ double s_red[kPG][kMaxB];  // this workgroup's per-group partials
+  __shared__ double s_acc[kPG][kMaxB];  // per-group sums of published ones
+  __shared__ double s_rowk[kMaxB];      // row k, staged by its owner group
+  __shared__ double s_rowin[kMaxB];     // row k, as received
+  __shared__ int s_pos[kMaxB];
+  __shared__ int s_abort;
 
   const int t = threadIdx.x, w = blockIdx.x;
   const int b = a.b, nw = a.nw;
@@ -72,12 +71,13 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   const int cc = t & 63, g = t >> 6;
   const bool col_ok = cc < b;
   const long glo = row0 + kRT * g;  // first global row of my group
-  // Position of global row kk inside my group, clamped to [-1, 64]:
-  // -1: kk is above my rows, 64: kk is below them.
+  // Position of global row kk inside my group, clamped to [-1, kRT]:
+  // -1: every row of mine is below kk, kRT: every row of mine is above it.
   auto rel = [&](long kk) -> int {
     long v = kk - glo;
     return (int)(v < -1 ? -1 : (v > kRT ? kRT : v));
   };
+  if (t == 0) s_abort = 0;
 
   // ---- load, 128 rows per round, coalesced along the rows of a column.
   double x[kRT];
@@ -96,257 +96,203 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
     __syncthreads();
   }
-  if (t < kMaxB) {
-    s_phys[t] = t;
-    s_done[t] = (t < b) ? 0 : 1;
-  }
-  if (t == 0) s_abort = 0;
-  __syncthreads();
 
-  // Exchange e's slot of workgroup q; e = 0: initial norms, then per step
-  // k: A at 1 + 2k, B at 2 + 2k.  Row group g reduces the partials of
-  // workgroups [q0, q1).
+  // Exchange e's slot of workgroup q.  e = 0: initial norms; step k: 1 + 2k
+  // (G and row k), 2 + 2k (recomputed norms, when needed).
   auto xslot = [&](int e, int q) -> double* {
     return a.xcur + ((long)e * nw + q) * kXS;
   };
   const int per = (nw + kPG - 1) / kPG;
   const int q0 = min(nw, g * per), q1 = min(nw, q0 + per);
-
-  // Squared norms of my column over rows >= kk+1 (S1) and >= kk+2 (S2), and
-  // rows kk, kk+1 (to s_rk / s_rk1); wavefront 0 combines and publishes.
-  auto norm_pass = [&](long kk, int e) {
-    double s1 = 0.0, s2 = 0.0;
-    const int r = rel(kk);
-    if (r < 0) {  // every row of mine is >= kk+1; S2 differs if kk+1 is mine
-      if (rel(kk + 1) < 0) {
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) s1 += x[i] * x[i];
-        s2 = s1;
-      } else {  // kk+1 is my first row
-#pragma unroll
-        for (int i = 1; i < kRT; ++i) s2 += x[i] * x[i];
-        s1 = s2 + x[0] * x[0];
-        s_rk1[cc] = x[0];
-      }
-    } else if (r < kRT) {  // straddle: masked path
-      double rk = 0.0, rk1 = 0.0;
-#pragma unroll
-      for (int i = 0; i < kRT; ++i) {
-        const double v2 = x[i] * x[i];
-        s1 += (i > r) ? v2 : 0.0;
-        s2 += (i > r + 1) ? v2 : 0.0;
-        rk = (i == r) ? x[i] : rk;
-        rk1 = (i == r + 1) ? x[i] : rk1;
-      }
-      s_rk[cc] = rk;
-      if (r + 1 < kRT) s_rk1[cc] = rk1;
-    }
-    s_red[g][cc] = s1;
-    s_red2[g][cc] = s2;
-    __syncthreads();
-    if (t < b) {
-      double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-      for (int q = 0; q < kPG; ++q) {
-        a1 += s_red[q][t];
-        a2 += s_red2[q][t];
-      }
-      double* my = xslot(e, w);
-      hq_publish(my + 2 * t, a1);
-      hq_publish(my + 2 * t + 1, a2);
-      const long lk = kk - row0, lk1 = kk + 1 - row0;
-      if (lk >= 0 && lk < kPR) hq_publish(my + 128 + t, s_rk[t]);
-      if (lk1 >= 0 && lk1 < kPR) hq_publish(my + 192 + t, s_rk1[t]);
-    }
-  };
-  // Full S1 / S2 of every column in a fixed order (s_full, s_S2) and rows
-  // kk, kk+1 (s_rkin, s_rk1in; row group 0 fetches them with its first
-  // chunk).  Returns false, in every thread, if the exchange failed.
-  auto reduce_B = [&](long kk, int e) -> bool {
-    bool ok = true;
-    if (col_ok) {
-      double a1 = 0.0, a2 = 0.0;
-      for (int base = q0; ok && base < q1; base += kChunk) {
-        const bool rows = g == 0 && base == q0;
-        double v[2 * kChunk + 2];
-        ok = hq_await(v, [&](int i) -> const double* {
-          if (i < 2 * kChunk) {
-            const int q = base + i / 2;
-            return q < q1 ? xslot(e, q) + 2 * cc + (i & 1) : nullptr;
-          }
-          if (!rows) return nullptr;
-          if (i == 2 * kChunk)
-            return kk >= 0 ? xslot(e, (int)(kk / kPR)) + 128 + cc : nullptr;
-          return kk + 1 < a.mrows ? xslot(e, (int)((kk + 1) / kPR)) + 192 + cc
-                                  : nullptr;
-        }, a.err);
-#pragma unroll
-        for (int i = 0; i < kChunk; ++i) {
-          a1 += v[2 * i];
-          a2 += v[2 * i + 1];
-        }
-        if (rows) {
-          s_rkin[cc] = v[2 * kChunk];
-          s_rk1in[cc] = v[2 * kChunk + 1];
-        }
-      }
-      s_acc[g][cc] = a1;
-      s_acc2[g][cc] = a2;
-    }
-    if (!ok) s_abort = 1;
-    __syncthreads();
-    if (s_abort) return false;
-    if (t < b) {
-      double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-      for (int q = 0; q < kPG; ++q) {
-        a1 += s_acc[q][t];
-        a2 += s_acc2[q][t];
-      }
-      s_full[t] = a1;
-      s_S2[t] = a2;
-    }
-    __syncthreads();
-    return true;
-  };
-
-  // Initial column norms (kk = -1): S1 = full squared norms, S2 = rows >= 1.
-  norm_pass(-1, 0);
-  if (!reduce_B(-1, 0)) return;
-  if (t < b) {
-    s_d[t] = sqrt(s_full[t]);
-    s_e[t] = s_d[t];
-    s_rowk[t] = s_rk1in[t];
-  }
-  __syncthreads();
-
-  for (int k = 0; k < b; ++k) {
-    // Phase 1: pivot (wavefront 0: largest partial norm, first position on
-    // ties, as idamax) and reflector; identical in every workgroup.
-    if (t < 64) {
-      int p = k;
-      if (a.pivoting) {
-        double v = (t >= k && t < b) ? s_d[s_phys[t]] : -1.0;
-        int q = (t >= k && t < b) ? t : 1 << 30;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          double ov = __shfl_xor(v, off, 64);
-          int oq = __shfl_xor(q, off, 64);
-          if (ov > v || (ov == v && oq < q)) {
-            v = ov;
-            q = oq;
-          }
-        }
-        p = q;
-      }
-      if (t == 0) {
-        int tmp = s_phys[k];
-        s_phys[k] = s_phys[p];
-        s_phys[p] = tmp;
-        const int pc = s_phys[k];
-        s_pc = pc;
-        s_done[pc] = 1;
-        hq_reflector h =
-            hq_dlarfg(s_rowk[pc], (a.mrows - k > 1) ? s_S2[pc] : 0.0);
-        s_tau = h.tau;
-        s_beta = h.beta;
-        s_scale = h.scale;
-        if (w == 0) a.tau[k] = h.tau;
-      }
-    }
-    __syncthreads();
-    const int pc = s_pc;
-    const double tau = s_tau;
-    const int r = rel(k);  // k relative to my group (-1: above, kRT: below)
-    const bool active = r < kRT;  // some of my rows are >= k
-    double* vg = s_v + kRT * g;
-    // Phase 2: the pivot column becomes the reflector; its values on my rows
-    // go to LDS (1 at row k, 0 above).
-    if (cc == pc && active) {
-      const double sc = s_scale, be = s_beta;
-      if (r < 0) {
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) {
-          x[i] *= sc;
-          vg[i] = x[i];
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) {
-          const bool below = i > r, diag = i == r;
-          x[i] = below ? x[i] * sc : (diag ? be : x[i]);
-          vg[i] = below ? x[i] : (diag ? 1.0 : 0.0);
-        }
-      }
-    }
-    __syncthreads();
-    // Phase 3: partial dots v'*col for every other column (rows >= k).
-    {
-      double s = 0.0;
-      if (active && col_ok && cc != pc) {
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) s += vg[i] * x[i];
-      }
-      s_red[g][cc] = s;
-    }
-    __syncthreads();
-    const int eA = 1 + 2 * k;
+  // Precondition: s_red holds this workgroup's per-group partials and, if
+  // this workgroup is `rowner`, s_rowk its row; all threads have synced.
+  // Wavefront 0 publishes; row group g sums the partials of workgroups
+  // [q0, q1) into s_acc[g]; group 0 also fetches the row of `rowner` (if
+  // >= 0) into s_rowin.  Returns false, in every thread, on failure.
+  auto exchange = [&](int e, int rowner) -> bool {
     if (t < b) {
       double a1 = 0.0;
 #pragma unroll
       for (int q = 0; q < kPG; ++q) a1 += s_red[q][t];
-      hq_publish(xslot(eA, w) + t, a1);
+      double* my = xslot(e, w);
+      hq_publish(my + t, a1);
+      if (rowner == w) hq_publish(my + kMaxB + t, s_rowk[t]);
     }
-    // Phase 4: full dots in a fixed order.
-    {
-      bool ok = true;
-      if (col_ok) {
-        double a1 = 0.0;
-        for (int base = q0; ok && base < q1; base += kChunk) {
-          double v[kChunk];
-          ok = hq_await(v, [&](int i) -> const double* {
+    bool ok = true;
+    if (col_ok) {
+      double acc = 0.0;
+      for (int base = q0; ok && base < q1; base += kChunk) {
+        const bool row = g == 0 && base == q0 && rowner >= 0;
+        double v[kChunk + 1];
+        ok = hq_await(v, [&](int i) -> const double* {
+          if (i < kChunk) {
             const int q = base + i;
-            return q < q1 ? xslot(eA, q) + cc : nullptr;
-          }, a.err);
+            return q < q1 ? xslot(e, q) + cc : nullptr;
+          }
+          return row ? xslot(e, rowner) + kMaxB + cc : nullptr;
+        }, a.err);
 #pragma unroll
-          for (int i = 0; i < kChunk; ++i) a1 += v[i];
-        }
-        s_acc[g][cc] = a1;
+        for (int i = 0; i < kChunk; ++i) acc += v[i];
+        if (row) s_rowin[cc] = v[kChunk];
       }
-      if (!ok) s_abort = 1;
+      s_acc[g][cc] = acc;
     }
+    if (!ok) s_abort = 1;
     __syncthreads();
-    if (s_abort) return;
-    if (t < b) {
-      double a1 = 0.0;
+    return s_abort == 0;
+  };
+  // Full sum for column c (every thread, the same order).
+  auto full = [&](int c) -> double {
+    double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < kPG; ++q) a1 += s_acc[q][t];
-      s_full[t] = a1;
-    }
-    __syncthreads();
-    if (w == 0 && t < k) a.gram[t + (long)k * b] = s_full[s_phys[t]];
-    // Phase 5: rank-1 update of the pending columns over rows >= k.
-    if (active && tau != 0.0 && col_ok && !s_done[cc]) {
-      const double f = -tau * s_full[cc];
+    for (int q = 0; q < kPG; ++q) s += s_acc[q][c];
+    return s;
+  };
+
+  // Per-column state, one copy per wavefront (lane c: column c): partial
+  // norm dn, its reference value en (LAWN 176's vn2), done flag; and the
+  // pivoting map ph = column at position c.
+  {
+    double s = 0.0;
 #pragma unroll
-      for (int i = 0; i < kRT; ++i) x[i] += vg[i] * f;
-    }
-    if (k + 1 < b) {
-      __syncthreads();
-      norm_pass(k, 2 + 2 * k);
-      if (!reduce_B(k, 2 + 2 * k)) return;
-      // Phase 6: norm downdate and the next step's alpha / ||x||^2.
-      if (t < b && !s_done[t]) {
-        const double wt = s_rkin[t];
-        if (a.pivoting)
-          hq_downdate_norm(s_d[t], s_e[t], wt, s_full[t], a.mrows - k - 1);
-        if (k + 1 < a.mrows) s_rowk[t] = s_rk1in[t];
-      }
-      __syncthreads();
-    }
+    for (int i = 0; i < kRT; ++i) s += x[i] * x[i];
+    s_red[g][cc] = s;
   }
-  // ---- write back in pivoted order (position q holds column s_phys[q]).
-  if (t < b) s_pos[s_phys[t]] = t;
+  __syncthreads();
+  if (!exchange(0, -1)) return;
+  double dn = col_ok ? sqrt(full(cc)) : 0.0, en = dn;
+  bool done = !col_ok;
+  int ph = cc;
+
+  // Pivot of step k (idamax over positions k..b-1: largest partial norm,
+  // first position on ties); swaps positions k and p in ph and returns the
+  // column now at position k.
+  auto choose = [&](int k) -> int {
+    int p = k;
+    if (a.pivoting) {
+      const double nv = __shfl(dn, ph, 64);
+      double key = (cc >= k && cc < b) ? nv : -1.0;
+      int q = (cc >= k && cc < b) ? cc : 1 << 30;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ok = __shfl_xor(key, off, 64);
+        const int oq = __shfl_xor(q, off, 64);
+        if (ok > key || (ok == key && oq < q)) {
+          key = ok;
+          q = oq;
+        }
+      }
+      p = q;
+    }
+    const int at_k = __shfl(ph, k, 64), at_p = __shfl(ph, p, 64);
+    if (cc == k)
+      ph = at_p;
+    else if (cc == p)
+      ph = at_k;
+    return at_p;
+  };
+
+  int pc = choose(0);
+  for (int k = 0; k < b; ++k) {
+    const int r = rel(k);
+    // 1. Partial G_c over my rows below k; the group holding row k stages it.
+    {
+      double s = 0.0;
+      if (r < 0) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) s += hq_readlane(x[i], pc) * x[i];
+      } else if (r < kRT) {
+        double rk = 0.0;
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) {
+          const double v = hq_readlane(x[i], pc);
+          s += (i > r) ? v * x[i] : 0.0;
+          rk = (i == r) ? x[i] : rk;
+        }
+        s_rowk[cc] = rk;
+      }
+      s_red[g][cc] = s;
+    }
+    __syncthreads();
+    if (!exchange(1 + 2 * k, (int)(k / kPR))) return;
+
+    // 2. Reflector, dots and gram entries (every thread, identical bits).
+    const hq_reflector h = hq_dlarfg(s_rowin[pc], full(pc));
+    if (w == 0 && t == 0) a.tau[k] = h.tau;
+    const double xk = col_ok ? s_rowin[cc] : 0.0;
+    const double dot = xk + h.scale * (col_ok ? full(cc) : 0.0);
+    {
+      const double dpos = __shfl(dot, ph, 64);  // dot of the column at my position
+      if (w == 0 && t < k) a.gram[t + (long)k * b] = dpos;
+    }
+    // 3. dlarf on my rows >= k: pending columns x += v * (-tau * dot), the
+    //    pivot column becomes (beta; v).
+    const double f = -h.tau * dot;
+    const bool upd = col_ok && !done && cc != pc && h.tau != 0.0;
+    if (r < 0) {
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) {
+        const double v = hq_readlane(x[i], pc) * h.scale;
+        if (upd)
+          x[i] += v * f;
+        else if (cc == pc)
+          x[i] = v;
+      }
+    } else if (r < kRT) {
+#pragma unroll
+      for (int i = 0; i < kRT; ++i) {
+        const double v = hq_readlane(x[i], pc) * h.scale;
+        if (i > r) {
+          if (upd)
+            x[i] += v * f;
+          else if (cc == pc)
+            x[i] = v;
+        } else if (i == r) {
+          if (upd)
+            x[i] += f;
+          else if (cc == pc)
+            x[i] = h.beta;
+        }
+      }
+    }
+    if (cc == pc) done = true;
+    if (k + 1 == b) break;
+
+    // 4. LAWN-176 downdate with row k of R (= the owner's updated x[r]).
+    bool need = false;
+    if (a.pivoting && !done && dn != 0.0) {
+      const double rk = (h.tau != 0.0) ? xk + f : xk;
+      double tt = fabs(rk) / dn;
+      const double u = (1.0 + tt) * (1.0 - tt);
+      tt = (0.0 > u) ? 0.0 : u;
+      const double ratio = dn / en;
+      if (tt * ratio * ratio <= HQ_TOL3Z)
+        need = true;
+      else
+        dn = dn * sqrt(tt);
+    }
+    if (__ballot(need)) {  // identical in every wavefront of every workgroup
+      double s = 0.0;
+      if (r < 0) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) s += x[i] * x[i];
+      } else if (r < kRT) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) s += (i > r) ? x[i] * x[i] : 0.0;
+      }
+      s_red[g][cc] = s;
+      __syncthreads();
+      if (!exchange(2 + 2 * k, -1)) return;
+      if (need) {
+        dn = (a.mrows - k - 1 > 0) ? sqrt(full(cc)) : 0.0;
+        en = dn;
+      }
+    }
+    // 5. Next pivot.
+    pc = choose(k + 1);
+  }
+  // ---- write back in pivoted order (position q holds column ph(q)).
+  if (g == 0 && col_ok) s_pos[ph] = cc;
   __syncthreads();
   const int myq = col_ok ? s_pos[cc] : 0;
 #pragma unroll
@@ -364,7 +310,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
     __syncthreads();
   }
-  if (w == 0 && t < b) a.perm[t] = s_phys[t];
+  if (w == 0 && g == 0 && col_ok) a.perm[cc] = ph;
   hq_xreset(a.xprev, a.xprev_len, w, nw, kPT);
 }
 
