@@ -166,6 +166,7 @@ struct hqrrp_handle_s {
   Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
   DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
   DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
+  DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf hA, hJ, hT;  // device copies used by the host entry points
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
@@ -390,11 +391,18 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   a.ldc = ldc;
   a.T = T;
   a.ldt = ldt;
-  // Split K so that >= 2 workgroups per CU are busy, slices >= 256 rows.
-  const long tiles = hq_cdiv(nc, 64);
+  // V'C: 256-column tiles (one workgroup per CU) when they can fill the
+  // CUs with K slices of >= 256 rows, else 64-column tiles (two per CU).
+  // Split K so that the grid fills but does not exceed those slots (a
+  // second, partial round of workgroups would double the time).
+  const long maxs = std::max(1L, (long)mrows / 256);
+  const long wtiles = hq_cdiv(nc, trail_vtc_cols());
+  a.wide = wtiles * maxs >= h->cu_count;
+  const long tiles = a.wide ? wtiles : hq_cdiv(nc, 64);
+  const long slots = a.wide ? h->cu_count : 2L * h->cu_count;
   const int kbk = trail_kbk();
-  long S = std::max(1L, (2L * h->cu_count + tiles - 1) / tiles);
-  S = std::min(S, std::max(1L, (long)mrows / 256));
+  long S = std::max(1L, slots / tiles);
+  S = std::min(S, maxs);
   long kc = (mrows + S - 1) / S;
   kc = (kc + kbk - 1) / kbk * kbk;
   S = (mrows + kc - 1) / kc;
@@ -523,19 +531,25 @@ int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
                     Y + (long)(j + b) * r, r, G + (long)j * r, r);
 }
 
-// The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.  With look-ahead the
-// pivot chain of block j+1 is computed from block j's W' = (T'V'C)' before
-// block j's trailing update has run:
-//   R12 rows: a copy of C's top b rows, updated by the same update kernel
-//             (bitwise the rows the full update will write);
-//   downdate and sketch pivots of block j+1 from them;
+// The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.
+//
+// Look-ahead: once block j's W' = (T'V'C)' is known, the R12 rows (a copy
+// of C's top b rows, updated by the same update kernel: bitwise the rows the
+// full update writes) and the sketch downdate are computed on the whole
+// chip; then the pivot chain of block j+1 runs on stream s_chain (a reserved
+// quarter of the CUs, so its persistent kernels are co-resident) while block
+// j's trailing update runs on s_upd (the other CUs):
+//   sketch pivots of block j+1;
 //   the b2 selected columns (rows below R12) copied out of the not yet
 //             updated C and updated by the same kernel into a panel buffer;
-// then block j's full update (stream s_upd) runs concurrently with the
-// panel QRCP of block j+1 on the panel buffer (stream s_chain), after which
-// the column gather, the panel write-back and the row permutation follow on
-// the caller's stream.  Every value is produced by the same kernel with the
-// same operands as without look-ahead, so the results are bitwise identical.
+//   panel QRCP and T of block j+1 on that buffer.
+// To let the chain read C while the update rewrites it, the update runs out
+// of place: the trailing matrix alternates between A and an internal buffer
+// of the same shape (X[0] = A, X[1]); every final value (R rows, panels) is
+// stored into A.  After both streams join, the column gather, the panel
+// write-back and the row permutation follow on the caller's stream.  Every
+// value is produced by the same kernel with the same operands as without
+// look-ahead, so the results are bitwise identical.
 int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
                   int* djpvt, double* dtau, const hqrrp_options& o) {
   const int nb = o.nb, pp = o.pp, r = nb + pp;
@@ -556,16 +570,22 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   HQ_RET(ensure(h->tmpA, sizeof(double) * (size_t)2 * nb * m));
   HQ_RET(ensure(h->tmpY, sizeof(double) * (size_t)2 * nb * r));
   HQ_RET(ensure(h->tmpJ, sizeof(int) * (size_t)2 * nb));
-  const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd;
+  const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd &&
+                      kmax > nb && n > nb;
   if (use_la) {
     HQ_RET(ensure(h->R12, sizeof(double) * (size_t)nb * n));
     HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
     HQ_RET(ensure(h->Wsel, sizeof(double) * (size_t)nb * nb));
+    HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
   }
   double* Y = h->Y.as<double>();
   double* G = h->G.as<double>();
   double* T = h->T.as<double>();
   hipStream_t main_s = h->stream;
+  // Trailing-matrix buffers; the trailing matrix lives in X[xc].
+  double* X[2] = {dA, use_la ? h->Xb.as<double>() : nullptr};
+  const long ldx[2] = {lda, (long)m};
+  int xc = 0;
 
   // Y = Omega * A (NoFLA_HQRRP_WY_blk_var4.c:407-409).
   {
@@ -580,7 +600,6 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     const bool last = (j + nb >= m) || (j + nb >= n);
     const int mrows = m - j, nc = n - j - b;
     double* Ajj = dA + j + (long)j * lda;
-    double* C = Ajj + (long)b * lda;
     if (!have_panel) {
       if (!last) {
         HQ_RET(block_sketch(h, r, j, n, b, Y));
@@ -593,15 +612,25 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     have_panel = false;
     if (nc <= 0) continue;
 
-    TrailArgs ta;
-    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, lda, &ta));
-    HQ_RET(trail_w(h, ta));
-
     const int jn = j + b;
-    int nw_next = 0;
-    const bool la = use_la && jn < kmax &&
-                    panel_geometry(h, m - jn, &nw_next) == HQRRP_OK &&
-                    nw_next <= h->chain_cus;
+    int nw_panel = 0;
+    const bool la =
+        use_la && jn < kmax &&
+        panel_geometry(h, m - jn, &nw_panel) == HQRRP_OK &&
+        nw_panel <= h->chain_cus &&
+        hq_cdiv(n - jn, sketch_cols_per_wg()) <= h->chain_cus;
+    if (!la && xc != 0) {
+      // The in-place update below needs the trailing matrix back in A.
+      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mrows * nc, 1);
+      HQ_TRY(launch_copy_block(dA + j + (long)jn * lda, lda,
+                               X[1] + j + (long)jn * ldx[1], ldx[1], mrows, nc,
+                               h->cur));
+      xc = 0;
+    }
+    double* C = X[xc] + j + (long)jn * ldx[xc];
+    TrailArgs ta;
+    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta));
+    HQ_RET(trail_w(h, ta));
     if (!la) {
       HQ_RET(trail_apply(h, ta));
       if (!last)
@@ -611,15 +640,17 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     }
 
     // jn < kmax implies !last and b == nb.
+    const int xn = xc ^ 1;
     double* R12 = h->R12.as<double>();
     double* Pn = h->Pn.as<double>();
     const int mnext = m - jn;
     const int b2 = std::min(nb, std::min(n - jn, m - jn));
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
+    // R12 rows and the sketch downdate on the whole chip (split-K GEMMs).
     {
       PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
                     16.0 * (double)b * nc, 2);
-      HQ_TRY(launch_copy_block(R12, b, C, lda, b, nc, h->cur));
+      HQ_TRY(launch_copy_block(R12, b, C, ldx[xc], b, nc, h->cur));
       TrailArgs tr = ta;
       tr.mrows = b;
       tr.C = R12;
@@ -627,25 +658,9 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
       HQ_TRY(launch_trail_update(tr, h->cur));
     }
     HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, R12, b, Y, G, nb));
-    if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
-    {
-      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
-                    16.0 * (double)mnext * b2, 2);
-      HQ_TRY(launch_select_cols(last_next ? nullptr : h->moves.as<int>(), b2, C,
-                                lda, b, mnext, ta.W, ta.ldw, b, Pn, mnext,
-                                h->Wsel.as<double>(), h->cur));
-      TrailArgs tp = ta;
-      tp.mrows = mnext;
-      tp.nc = b2;
-      tp.vmask = 0;
-      tp.V = Ajj + b;
-      tp.C = Pn;
-      tp.ldc = mnext;
-      tp.W = h->Wsel.as<double>();
-      tp.ldw = b2;
-      HQ_TRY(launch_trail_update(tp, h->cur));
-    }
-    // Fork: block j's update || block jn's panel QRCP and T factor.
+    ta.Cout = X[xn] + j + (long)jn * ldx[xn];
+    ta.ldco = ldx[xn];
+    // Fork: block j's update || block jn's pivot chain.
     HQ_TRY(hipEventRecord(h->ev_fork, main_s));
     HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_fork, 0));
     HQ_TRY(hipStreamWaitEvent(h->s_chain, h->ev_fork, 0));
@@ -654,23 +669,64 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     if (rc == HQRRP_OK && hipEventRecord(h->ev_upd, h->s_upd) != hipSuccess)
       rc = HQRRP_ERR_HIP;
     h->cur = h->s_chain;
-    if (rc == HQRRP_OK)
-      rc = run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
-                     h->perm.as<int>(), T, nb);
-    if (rc == HQRRP_OK && hipEventRecord(h->ev_chain, h->s_chain) != hipSuccess)
-      rc = HQRRP_ERR_HIP;
+    auto chain = [&]() -> int {
+      if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
+      {
+        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
+                      16.0 * (double)mnext * b2, 2);
+        HQ_TRY(launch_select_cols(last_next ? nullptr : h->moves.as<int>(), b2,
+                                  C, ldx[xc], b, mnext, ta.W, ta.ldw, b, Pn,
+                                  mnext, h->Wsel.as<double>(), h->cur));
+        TrailArgs tp = ta;
+        tp.mrows = mnext;
+        tp.nc = b2;
+        tp.vmask = 0;
+        tp.V = Ajj + b;
+        tp.C = Pn;
+        tp.ldc = mnext;
+        tp.Cout = nullptr;
+        tp.W = h->Wsel.as<double>();
+        tp.ldw = b2;
+        HQ_TRY(launch_trail_update(tp, h->cur));
+      }
+      HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
+                       h->perm.as<int>(), T, nb));
+      HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
+      return HQRRP_OK;
+    };
+    if (rc == HQRRP_OK) rc = chain();
     h->cur = main_s;
     HQ_RET(rc);
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
-    // Join: block jn's columns into place, then its factored panel over them.
-    if (!last_next) HQ_RET(block_gather(h, r, jn, m, b2, dA, lda, Y, djpvt));
+    // Join: the R12 rows into A (when the update wrote them elsewhere), block
+    // jn's columns into place (rows above jn in A, the rest in the trailing
+    // buffer), its factored panel over them, and the row permutation.
+    if (xn != 0) {
+      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)b * nc, 1);
+      HQ_TRY(launch_copy_block(dA + j + (long)jn * lda, lda, R12, b, b, nc,
+                               h->cur));
+    }
+    if (!last_next) {
+      if (xn == 0) {
+        HQ_RET(block_gather(h, r, jn, m, b2, dA, lda, Y, djpvt));
+      } else {
+        HQ_RET(block_gather(h, r, jn, jn, b2, dA, lda, Y, djpvt));
+        PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
+        HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b2,
+                                  X[1] + jn + (long)jn * ldx[1], ldx[1], mnext,
+                                  nullptr, 0, 0, nullptr, h->tmpA.as<double>(),
+                                  h->tmpY.as<double>(), h->tmpJ.as<int>(),
+                                  h->cur));
+      }
+    }
     {
       PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mnext * b2, 1);
       HQ_TRY(launch_copy_block(dA + jn + (long)jn * lda, lda, Pn, mnext, mnext,
                                b2, h->cur));
     }
     HQ_RET(block_permute(h, jn, b2, dA, lda, djpvt));
+    xc = xn;
     have_panel = true;
   }
   HQ_TRY(hipGetLastError());
@@ -780,7 +836,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
                     &h->hA,    &h->hJ,    &h->hT,     &h->R12,
-                    &h->Pn,    &h->Wsel};
+                    &h->Pn,    &h->Wsel,  &h->Xb};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(HQ_NT) void gather_out_kernel(
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < r; i += HQ_NT)
       tmpY[(long)e * r + i] = Y[src * ldy + i];
-    if (threadIdx.x == 0) tmpJ[e] = jpvt[src];
+    if (threadIdx.x == 0 && jpvt) tmpJ[e] = jpvt[src];
   }
 }
 
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(HQ_NT) void gather_in_kernel(
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < r; i += HQ_NT)
       Y[dst * ldy + i] = tmpY[(long)e * r + i];
-    if (threadIdx.x == 0) jpvt[dst] = tmpJ[e];
+    if (threadIdx.x == 0 && jpvt) jpvt[dst] = tmpJ[e];
   }
 }
 

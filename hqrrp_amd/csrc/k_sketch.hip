@@ -123,15 +123,22 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     {
-      double bk = s_key[0];
-      int bp = s_pos[0], bq = 0;
+      // Every wavefront combines the 8 wavefront winners by a butterfly
+      // over lane groups of 8 (identical result in every lane).
+      int bq = lane & 7;
+      double bk = s_key[bq];
+      int bp = s_pos[bq];
 #pragma unroll
-      for (int q = 1; q < 8; ++q)
-        if (s_key[q] > bk || (s_key[q] == bk && s_pos[q] < bp)) {
-          bk = s_key[q];
-          bp = s_pos[q];
-          bq = q;
+      for (int off = 4; off > 0; off >>= 1) {
+        const double ok = __shfl_xor(bk, off, 64);
+        const int op = __shfl_xor(bp, off, 64);
+        const int oq = __shfl_xor(bq, off, 64);
+        if (ok > bk || (ok == bk && op < bp)) {
+          bk = ok;
+          bp = op;
+          bq = oq;
         }
+      }
       if (bp == INT_MAX) {  // no candidate here
         if (t == 0) {
           hq_publish(mine + 0, -1.0);

@@ -22,13 +22,15 @@
 namespace {
 
 constexpr int kTile = 64;  // output tile edge (rows and columns)
-constexpr int kBK = 32;    // K rows staged per step in trail_vtc
+constexpr int kBK = 16;    // K rows staged per step in trail_vtc
+constexpr int kBK64 = 32;  // K rows staged per step in trail_vtc64
 
 __device__ __forceinline__ double vmask(double v, long i, long c) {
   return (i < c) ? 0.0 : ((i == c) ? 1.0 : v);
 }
 
-// C(mrows x nc) -= V(mrows x b) * W(b x nc) on 128x128 tiles, computed
+// C(mrows x nc) -= V(mrows x b) * W(b x nc) on 128x128 tiles (written to
+// Cout when given: the update then runs out of place), computed
 // transposed as C' := C' + W' (-V') so that the row-strided f64 accumulator
 // layout maps to 16 consecutive rows of one column of C per lane group:
 // every C load and store moves whole 128-byte lines.  The C tile is loaded
@@ -58,6 +60,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
   const bool interior = m0 + kUT <= a.mrows && n0 + kUT <= a.nc;
   const long lm = m0 + wm * 64 + l15, ln = n0 + wn * 64 + l4;
   double* cbase = a.C + lm + ln * a.ldc;
+  const long ldo = a.Cout ? a.ldco : a.ldc;
+  double* obase = (a.Cout ? a.Cout : a.C) + lm + ln * ldo;
   hq_d4 acc[4][4];
   if (interior) {
 #pragma unroll
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+        double* cc = obase + (long)(16 * i + 4 * r) * ldo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) cc[16 * j] = acc[i][j][r];
       }
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const bool nok = ln + 16 * i + 4 * r < a.nc;
-        double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+        double* cc = obase + (long)(16 * i + 4 * r) * ldo;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (nok && lm + 16 * j < a.mrows) cc[16 * j] = acc[i][j][r];
@@ -154,11 +158,108 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 
 // part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
 // computed transposed (C' V) so the accumulator stores whole columns of the
-// partial.  K rows are staged kBK at a time, k-contiguous in LDS (pitch kBK+4:
-// conflict-free stores, at most 2-way conflicts on fragment reads); the next
-// slice is loaded into registers while the matrix cores work.
-__global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
-  constexpr int P = kBK + 4;
+// partial.  A workgroup owns 256 columns of C (wavefront w: columns
+// 64w..64w+63, against all 64 reflectors: 4 x 4 MFMA blocks), so each V
+// slice staged in LDS serves 256 columns and C streams through once in
+// whole 128-byte lines.  K rows are staged kBK at a time (LDS pitch kBK+1:
+// conflict-free fragment reads); the next slice is loaded into registers
+// while the matrix cores work.
+constexpr int kVN = 256;       // columns of C per workgroup
+constexpr int kVP = kBK + 1;   // LDS pitch (doubles)
+__global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
+  __shared__ double Vs[kTile][kVP];  // Vs[p][k] = V(k0+k, p)
+  __shared__ double Cs[kVN][kVP];    // Cs[n][k] = C(k0+k, n0+n)
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const long n0 = (long)blockIdx.x * kVN;
+  const int s = blockIdx.y;
+  const long kbeg = (long)s * a.kchunk;
+  const long kend = min((long)a.mrows, kbeg + a.kchunk);
+  const int b = a.b;
+  constexpr int kPerV = kBK * kTile / HQ_NT;  // V elements per thread
+  constexpr int kPerC = kBK * kVN / HQ_NT;    // C elements per thread
+  const int l15 = lane & 15, l4 = lane >> 4;
+
+  hq_d4 acc[4][4];  // [n block][p block]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = hq_d4{0.0, 0.0, 0.0, 0.0};
+
+  double rv[kPerV], rc[kPerC];
+  auto load = [&](long k0) {
+#pragma unroll
+    for (int q = 0; q < kPerV; ++q) {
+      const int idx = t + HQ_NT * q, k = idx % kBK, p = idx / kBK;
+      const long gk = k0 + k;
+      const double vv = (gk < kend && p < b) ? a.V[gk + (long)p * a.ldv] : 0.0;
+      rv[q] = a.vmask ? vmask(vv, gk, p) : vv;
+    }
+#pragma unroll
+    for (int q = 0; q < kPerC; ++q) {
+      const int idx = t + HQ_NT * q, k = idx % kBK, n = idx / kBK;
+      const long gk = k0 + k, gn = n0 + n;
+      rc[q] = (gk < kend && gn < a.nc) ? a.C[gk + gn * a.ldc] : 0.0;
+    }
+  };
+  // The K slice is walked starting at a per-column-tile rotation, so that
+  // concurrently running workgroups read different rows (spreading the
+  // power-of-two-strided columns over HBM channels).
+  const long nsteps = (kend - kbeg + kBK - 1) / kBK;
+  const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
+  auto kstart = [&](long it) { return kbeg + ((it + rot) % nsteps) * kBK; };
+  if (nsteps > 0) load(kstart(0));
+  for (long it = 0; it < nsteps; ++it) {
+#pragma unroll
+    for (int q = 0; q < kPerV; ++q) {
+      const int idx = t + HQ_NT * q;
+      Vs[idx / kBK][idx % kBK] = rv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerC; ++q) {
+      const int idx = t + HQ_NT * q;
+      Cs[idx / kBK][idx % kBK] = rc[q];
+    }
+    __syncthreads();
+    if (it + 1 < nsteps) load(kstart(it + 1));
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 4) {
+      double af[4], bf[4];  // A = C' (n x k), B = V (k x p)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = Cs[wave * 64 + i * 16 + l15][kk + l4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Vs[j * 16 + l15][kk + l4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+                                                            acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
+  // n = n0 + 64*wave + 16i + l/16 + 4r.
+  double* out = a.part + (long)s * b * a.nc;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int p = j * 16 + l15;
+        const long gn = n0 + wave * 64 + i * 16 + l4 + 4 * r;
+        if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
+      }
+}
+
+// The same product on 64-column tiles (4 wavefronts of 32 x 32, two
+// workgroups per CU): more workgroups for the small trailing matrices of the
+// last blocks, where 256-column tiles leave CUs idle.  K rows are staged
+// kBK64 at a time, k-contiguous in LDS (pitch kBK64+4: conflict-free stores,
+// at most 2-way conflicts on fragment reads); the next slice is loaded into
+// registers while the matrix cores work.
+__global__ __launch_bounds__(HQ_NT, 2) void trail_vtc64_kernel(TrailArgs a) {
+  constexpr int P = kBK64 + 4;
   __shared__ double Vs[kTile][P];  // Vs[p][k] = V(k0+k, p)
   __shared__ double Cs[kTile][P];  // Cs[n][k] = C(k0+k, n0+n)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -168,7 +269,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
   const long kbeg = (long)s * a.kchunk;
   const long kend = min((long)a.mrows, kbeg + a.kchunk);
   const int b = a.b;
-  constexpr int kPer = kBK * kTile / HQ_NT;  // 8 elements per thread
+  constexpr int kPer = kBK64 * kTile / HQ_NT;  // 8 elements per thread
   const int l15 = lane & 15, l4 = lane >> 4;
 
   hq_d4 acc[2][2];
@@ -182,7 +283,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       int idx = t + HQ_NT * q;
-      int k = idx & (kBK - 1), p = idx / kBK;
+      int k = idx & (kBK64 - 1), p = idx / kBK64;
       long gk = k0 + k;
       double vv = (gk < kend && p < b) ? a.V[gk + (long)p * a.ldv] : 0.0;
       rv[q] = a.vmask ? vmask(vv, gk, p) : vv;
@@ -193,22 +294,22 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
   // The K slice is walked starting at a per-column-tile rotation, so that
   // concurrently running workgroups read different rows (spreading the
   // power-of-two-strided columns over HBM channels).
-  const long nsteps = (kend - kbeg + kBK - 1) / kBK;
+  const long nsteps = (kend - kbeg + kBK64 - 1) / kBK64;
   const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
-  auto kstart = [&](long it) { return kbeg + ((it + rot) % nsteps) * kBK; };
+  auto kstart = [&](long it) { return kbeg + ((it + rot) % nsteps) * kBK64; };
   if (nsteps > 0) load(kstart(0));
   for (long it = 0; it < nsteps; ++it) {
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       int idx = t + HQ_NT * q;
-      int k = idx & (kBK - 1), p = idx / kBK;
+      int k = idx & (kBK64 - 1), p = idx / kBK64;
       Vs[p][k] = rv[q];
       Cs[p][k] = rc[q];
     }
     __syncthreads();
     if (it + 1 < nsteps) load(kstart(it + 1));
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
+    for (int kk = 0; kk < kBK64; kk += 4) {
       double af[2], bf[2];  // A = C' (n x k), B = V (k x p)
 #pragma unroll
       for (int i = 0; i < 2; ++i) af[i] = Cs[wn * 32 + i * 16 + l15][kk + l4];
@@ -237,6 +338,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc_kernel(TrailArgs a) {
         if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
       }
 }
+
 
 // W(:, n0:n0+64) = T' * sum_s part[s](:, n0:n0+64), slices added in order,
 // stored transposed (W' is nc x b) for coalesced staging in trail_update.
@@ -272,8 +374,13 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
 }  // namespace
 
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
-  dim3 grid(hq_cdiv(a.nc, kTile), a.splits);
-  hipLaunchKernelGGL(trail_vtc_kernel, grid, dim3(HQ_NT), 0, s, a);
+  if (a.wide) {
+    dim3 grid(hq_cdiv(a.nc, kVN), a.splits);
+    hipLaunchKernelGGL(trail_vtc_kernel, grid, dim3(HQ_NT), 0, s, a);
+  } else {
+    dim3 grid(hq_cdiv(a.nc, kTile), a.splits);
+    hipLaunchKernelGGL(trail_vtc64_kernel, grid, dim3(HQ_NT), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -289,4 +396,6 @@ hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-int trail_kbk() { return kBK; }
+int trail_kbk() { return kBK64; }  // multiple of kBK too
+
+int trail_vtc_cols() { return kVN; }

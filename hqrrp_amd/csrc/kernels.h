@@ -51,9 +51,12 @@ struct GemmArgs {
 struct TrailArgs {
   int mrows, nc, b, splits;
   int vmask;        // 1: V's top b x b block is unit lower (masked)
+  int wide;         // trail_vtc: 256-column tiles (else 64)
   const double* V;  // mrows x b reflectors (unit lower trapezoidal), ld ldv
   long ldv;
   double* C;  // mrows x nc trailing matrix, ld ldc
+  double* Cout;  // trail_update's output (null: C in place), ld ldco
+  long ldco;
   long ldc;
   double* W;  // W' = (T' V' C)': nc x b, ld ldw
   long ldw;
@@ -68,6 +71,7 @@ hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
 int trail_kbk();
+int trail_vtc_cols();  // columns of C per wide trail_vtc workgroup
 
 // k_sketch.hip
 size_t sketch_xchg_doubles(int nw, int rmax, int stages);
@@ -99,6 +103,8 @@ hipError_t launch_select_cols(const int* moves, int b2, const double* C,
                               long ldc, int r0, int rows, const double* Wt,
                               long ldw, int b, double* P, long ldp,
                               double* Wsel, hipStream_t s);
+// Moves m rows of A's columns (A may point at any row), r rows of Y's
+// columns (r = 0: none) and jpvt (null: none).
 hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
                               long lda, int m, double* Y, long ldy, int r,
                               int* jpvt, double* tmpA, double* tmpY, int* tmpJ,
