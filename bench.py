@@ -10,10 +10,13 @@ the device-to-device copy that restores A is inside the timed step.
 metric: QRCP GFLOP/s with the flop count 2mn^2 - 2/3 n^3 (m >= n).
 
 Also reported:
-  roofline      -- the phase with the largest summed kernel time during the
-                   timed steps, its algorithmic flops (or bytes) per launch
-                   divided by its average launch duration, both from HIP
-                   events recorded on the library's stream;
+  phase_ms      -- per-phase kernel time of one extra, untimed step in which
+                   every phase is bracketed by HIP events;
+  roofline      -- the phase with the largest time in that step (the
+                   dominant kernel): during the timed steps only its launches
+                   are bracketed by HIP events (recorded on the stream they
+                   run on), and its algorithmic flops (or bytes) per launch
+                   are divided by its average launch duration;
   cpu_baseline  -- the oracle (a CPU restatement of the reference driver
                    making the same LAPACK calls, OpenBLAS) timed on the host
                    cores on the first --cpu-blocks blocks of the same matrix,
@@ -127,8 +130,16 @@ def main():
         h.synchronize()
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
 
+    # One untimed step with every phase timed: the breakdown, and the choice
+    # of the dominant phase (timing every phase costs ~10% of the step).
+    prof_all, dominant = {}, None
     if not args.no_profile:
         h.profile(True)
+        step()
+        h.synchronize()
+        prof_all = h.profile_read()
+        dominant = max(prof_all, key=lambda k: prof_all[k][0])
+        h.profile(True, phases=[dominant])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -148,19 +159,20 @@ def main():
     roof = None
     phase_ms = {}
     gemm_roof = {}
-    if prof:
+    if prof_all:
         # The trailing-update kernels carry ~97% of the flops; report their
-        # rooflines beside the dominant (largest-time) phase.
+        # rooflines (from the breakdown step) beside the dominant phase's.
         for name in ("trail_vtc", "trail_update"):
-            t_ms, launches, fl, by = prof[name]
+            t_ms, launches, fl, by = prof_all[name]
             if t_ms > 0:
                 gemm_roof[name] = {
                     "tflops": round(fl / (t_ms / 1e3) / 1e12, 2),
                     "frac_mfma": round(fl / (t_ms / 1e3) / PEAK_FP64_MFMA, 4),
                     "hbm_gbs": round(by / (t_ms / 1e3) / 1e9, 1),
                     "frac_hbm": round(by / (t_ms / 1e3) / PEAK_HBM, 4)}
-        phase_ms = {k: round(v[0] / args.steps, 3) for k, v in prof.items()}
-        name = max(prof, key=lambda k: prof[k][0])
+        phase_ms = {k: round(v[0], 3) for k, v in prof_all.items()}
+    if prof and dominant:
+        name = dominant
         t_ms, launches, fl, by = prof[name]
         bound = PHASE_BOUND[name]
         avg_s = t_ms / 1e3 / max(1, launches)

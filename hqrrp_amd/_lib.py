@@ -45,6 +45,7 @@ SIGNATURES = {
     "hqrrp_phase_count": (C.c_int, []),
     "hqrrp_phase_name": (C.c_char_p, [C.c_int]),
     "hqrrp_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "hqrrp_profile_phases": (C.c_int, [_vp, C.c_uint]),
     "hqrrp_profile_read": (C.c_int, [_vp, C.c_int, _d, C.POINTER(C.c_longlong),
                                      _d, _d]),
     "hqrrp_factor_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp,

@@ -103,9 +103,19 @@ class Handle:
     def synchronize(self) -> None:
         check(lib().hqrrp_synchronize(self._h), "hqrrp_synchronize")
 
-    def profile(self, on: bool = True) -> None:
-        """Start (and reset) or stop per-phase event timing."""
-        check(lib().hqrrp_profile_enable(self._h, 1 if on else 0),
+    def profile(self, on: bool = True, phases=None) -> None:
+        """Start (and reset) or stop per-phase event timing; `phases` (names)
+        limits the recording to those phases (default: all)."""
+        L = lib()
+        mask = 0xFFFFFFFF
+        if phases is not None:
+            names = [L.hqrrp_phase_name(i).decode()
+                     for i in range(L.hqrrp_phase_count())]
+            mask = 0
+            for p in phases:
+                mask |= 1 << names.index(p)
+        check(L.hqrrp_profile_phases(self._h, mask), "hqrrp_profile_phases")
+        check(L.hqrrp_profile_enable(self._h, 1 if on else 0),
               "hqrrp_profile_enable")
 
     def profile_read(self) -> dict:

@@ -97,6 +97,33 @@ __device__ __forceinline__ double hq_wave_sum(double v) {
   return v;
 }
 
+// Argmax over aligned groups of `width` lanes (a power of two <= 64): the
+// largest key, and among equal keys the smallest pos (idamax's first-index
+// rule); every lane of a group receives its group's (key, pos).  A max-only
+// butterfly on the key, then the position from the one lane holding the
+// maximum -- or, only if some group holds it in several lanes, a min
+// butterfly on the positions.  All 64 lanes must be active.
+__device__ __forceinline__ void hq_wave_argmax(double& key, int& pos,
+                                               int width) {
+  double m = key;
+  for (int off = width >> 1; off > 0; off >>= 1)
+    m = fmax(m, __shfl_xor(m, off, 64));
+  const int lane = threadIdx.x & 63;
+  const unsigned long long grp =
+      (width == 64) ? ~0ull
+                    : (((1ull << width) - 1ull) << (lane & ~(width - 1)));
+  const unsigned long long cand = __ballot(key == m) & grp;
+  if (__any(__popcll(cand) > 1)) {
+    int pp = (key == m) ? pos : 0x7fffffff;
+    for (int off = width >> 1; off > 0; off >>= 1)
+      pp = min(pp, __shfl_xor(pp, off, 64));
+    pos = pp;
+  } else {
+    pos = __shfl(pos, __builtin_ctzll(cand), 64);
+  }
+  key = m;
+}
+
 // LAPACK dlapy2: sqrt(x^2 + y^2) without destructive overflow.
 __device__ __forceinline__ double hq_dlapy2(double x, double y) {
   double xa = fabs(x), ya = fabs(y);

@@ -173,6 +173,7 @@ struct hqrrp_handle_s {
   std::mutex mu;
   // Optional per-phase timing with HIP events recorded on `stream`.
   bool prof = false;
+  unsigned prof_mask = ~0u;  // phases recorded when prof is on
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   struct Span {
@@ -222,7 +223,7 @@ struct PhaseScope {
   PhaseScope(hqrrp_handle_t h_, int ph_, double flops, double bytes,
              int launches)
       : h(h_), ph(ph_) {
-    if (!h->prof) return;
+    if (!h->prof || !((h->prof_mask >> ph) & 1u)) return;
     h->ph_flops[ph] += flops;
     h->ph_bytes[ph] += bytes;
     h->ph_launch[ph] += launches;
@@ -230,7 +231,7 @@ struct PhaseScope {
     if (a) (void)hipEventRecord(a, h->cur);
   }
   ~PhaseScope() {
-    if (!h->prof || !a) return;
+    if (!a) return;
     hipEvent_t b = next_event(h);
     if (!b) return;
     (void)hipEventRecord(b, h->cur);
@@ -866,6 +867,13 @@ extern "C" int hqrrp_profile_enable(hqrrp_handle_t h, int on) {
     h->ph_flops[p] = h->ph_bytes[p] = 0.0;
     h->ph_launch[p] = 0;
   }
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_profile_phases(hqrrp_handle_t h, unsigned mask) {
+  if (!h) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->prof_mask = mask;
   return HQRRP_OK;
 }
 

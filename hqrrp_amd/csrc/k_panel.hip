@@ -22,7 +22,7 @@
 // the norm downdate and the next pivot -- no workgroup barrier beyond the
 // exchange itself.  Only when a downdate loses accuracy (LAWN 176's
 // recompute test) is a second exchange run, for the fresh column norms.
-// The reflector entries reach the other columns' lanes by readlane from the
+// The reflector entries reach the other columns' lanes through LDS from the
 // pivot column's lane (each wavefront holds all b columns of its rows).
 // Reductions are done in a fixed order everywhere.  Columns stay with their
 // threads; a position -> column map records the pivoting and the slab is
@@ -43,14 +43,6 @@ constexpr int kXS = 2 * kMaxB;    // exchange slot of one workgroup: [0,64)
                                   // (owner only)
 constexpr int kChunk = 8;         // workgroups' partials awaited together
 
-// Lane `lane`'s value of v (lane uniform across the wavefront).
-__device__ __forceinline__ double hq_readlane(double v, int lane) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)bits, lane);
-  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // Thread (c, g) = (t % 64, t / 64) holds column c of the 32 consecutive rows
 // row0 + 32g + i, i < 32, in registers x[i]; rows beyond mrows are zero.
 // Relative to the current step k, a wavefront's rows are either all above
@@ -62,6 +54,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   __shared__ double s_acc[kPG][kMaxB];  // per-group sums of published ones
   __shared__ double s_rowk[kMaxB];      // row k, staged by its owner group
   __shared__ double s_rowin[kMaxB];     // row k, as received
+  __shared__ double s_pv[kPG][kRT];     // each group's rows of the pivot column
   __shared__ int s_pos[kMaxB];
   __shared__ int s_abort;
 
@@ -173,15 +166,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
       const double nv = __shfl(dn, ph, 64);
       double key = (cc >= k && cc < b) ? nv : -1.0;
       int q = (cc >= k && cc < b) ? cc : 1 << 30;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ok = __shfl_xor(key, off, 64);
-        const int oq = __shfl_xor(q, off, 64);
-        if (ok > key || (ok == key && oq < q)) {
-          key = ok;
-          q = oq;
-        }
-      }
+      hq_wave_argmax(key, q, 64);
       p = q;
     }
     const int at_k = __shfl(ph, k, 64), at_p = __shfl(ph, p, 64);
@@ -195,21 +180,36 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   int pc = choose(0);
   for (int k = 0; k < b; ++k) {
     const int r = rel(k);
-    // 1. Partial G_c over my rows below k; the group holding row k stages it.
+    const double* pv = s_pv[g];
+    // 1. The pivot column's lane stages its rows for its wavefront (read
+    //    back as LDS broadcasts: the LDS traffic stays inside the wavefront,
+    //    ordered by wavefront-scope fences); partial G_c over my rows below
+    //    k (four independent partial sums); the group holding row k stages
+    //    it.
     {
       double s = 0.0;
-      if (r < 0) {
+      if (r < kRT) {
+        if (cc == pc) {
 #pragma unroll
-        for (int i = 0; i < kRT; ++i) s += hq_readlane(x[i], pc) * x[i];
-      } else if (r < kRT) {
-        double rk = 0.0;
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) {
-          const double v = hq_readlane(x[i], pc);
-          s += (i > r) ? v * x[i] : 0.0;
-          rk = (i == r) ? x[i] : rk;
+          for (int i = 0; i < kRT; ++i) s_pv[g][i] = x[i];
         }
-        s_rowk[cc] = rk;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        if (r < 0) {
+#pragma unroll
+          for (int i = 0; i < kRT; ++i) s4[i & 3] += pv[i] * x[i];
+        } else {
+          double rk = 0.0;
+#pragma unroll
+          for (int i = 0; i < kRT; ++i) {
+            s4[i & 3] += (i > r) ? pv[i] * x[i] : 0.0;
+            rk = (i == r) ? x[i] : rk;
+          }
+          s_rowk[cc] = rk;
+        }
+        s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
       }
       s_red[g][cc] = s;
     }
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     if (r < 0) {
 #pragma unroll
       for (int i = 0; i < kRT; ++i) {
-        const double v = hq_readlane(x[i], pc) * h.scale;
+        const double v = pv[i] * h.scale;
         if (upd)
           x[i] += v * f;
         else if (cc == pc)
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     } else if (r < kRT) {
 #pragma unroll
       for (int i = 0; i < kRT; ++i) {
-        const double v = hq_readlane(x[i], pc) * h.scale;
+        const double v = pv[i] * h.scale;
         if (i > r) {
           if (upd)
             x[i] += v * f;

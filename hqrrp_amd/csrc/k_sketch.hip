@@ -107,15 +107,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     {
       double key = (pos >= 0) ? d : -1.0;
       int kp = (pos >= 0) ? pos : INT_MAX;
-#pragma unroll
-      for (int off = 16; off > 0; off >>= 1) {
-        double ok = __shfl_xor(key, off, 64);
-        int op = __shfl_xor(kp, off, 64);
-        if (ok > key || (ok == key && op < kp)) {
-          key = ok;
-          kp = op;
-        }
-      }
+      hq_wave_argmax(key, kp, 32);  // lanes l, l+32 hold the same column
       if (lane == 0) {
         s_key[wave] = key;
         s_pos[wave] = kp;
@@ -123,22 +115,13 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     {
-      // Every wavefront combines the 8 wavefront winners by a butterfly
-      // over lane groups of 8 (identical result in every lane).
-      int bq = lane & 7;
-      double bk = s_key[bq];
-      int bp = s_pos[bq];
-#pragma unroll
-      for (int off = 4; off > 0; off >>= 1) {
-        const double ok = __shfl_xor(bk, off, 64);
-        const int op = __shfl_xor(bp, off, 64);
-        const int oq = __shfl_xor(bq, off, 64);
-        if (ok > bk || (ok == bk && op < bp)) {
-          bk = ok;
-          bp = op;
-          bq = oq;
-        }
-      }
+      // Every wavefront combines the 8 wavefront winners over lane groups
+      // of 8 (identical result in every lane); bq: the winner's wavefront.
+      const int wq = s_pos[lane & 7];
+      double bk = s_key[lane & 7];
+      int bp = wq;
+      hq_wave_argmax(bk, bp, 8);
+      const int bq = __builtin_ctzll(__ballot(wq == bp) & 0xFFull);
       if (bp == INT_MAX) {  // no candidate here
         if (t == 0) {
           hq_publish(mine + 0, -1.0);
@@ -188,20 +171,13 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
           bs = hv[3];
         }
       }
-      int bl = lane;  // lane holding the winner's alpha and norm
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        double okk = __shfl_xor(bk, off, 64);
-        int op = __shfl_xor(bp, off, 64);
-        int ow = __shfl_xor(bw, off, 64);
-        int ol = __shfl_xor(bl, off, 64);
-        if (okk > bk || (okk == bk && op < bp)) {
-          bk = okk;
-          bp = op;
-          bw = ow;
-          bl = ol;
-        }
-      }
+      // The lane holding the winner (positions are unique) supplies its
+      // workgroup, alpha and norm.
+      const double lk = bk;
+      const int lp = bp;
+      hq_wave_argmax(bk, bp, 64);
+      const int bl = __builtin_ctzll(__ballot(lk == bk && lp == bp));
+      bw = __shfl(bw, bl, 64);
       ba = __shfl(ba, bl, 64);
       bs = __shfl(bs, bl, 64);
       if (lane == 0) {

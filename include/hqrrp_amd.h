@@ -102,13 +102,16 @@ int hqrrp_synchronize(hqrrp_handle_t h);
 
 /* Optional per-phase timing (no counterpart in the reference, which only has
  * compile-time wall-clock timers in its libflame variant): HIP events are
- * recorded on the handle's stream around the launches of each phase of the
- * block loop.  enable resets the counters; read synchronises the stream and
- * returns, per phase, the summed event time (ms), the number of kernel
- * launches, and the algorithmic flops and bytes of those launches. */
+ * recorded around the launches of each phase of the block loop, on the
+ * stream those launches go to.  enable resets the counters; phases limits
+ * the recording to the phases whose bit is set in mask (default: all; each
+ * event pair costs a few microseconds of GPU time); read synchronises the
+ * stream and returns, per phase, the summed event time (ms), the number of
+ * kernel launches, and the algorithmic flops and bytes of those launches. */
 int hqrrp_phase_count(void);
 const char *hqrrp_phase_name(int phase);
 int hqrrp_profile_enable(hqrrp_handle_t h, int on);
+int hqrrp_profile_phases(hqrrp_handle_t h, unsigned mask);
 int hqrrp_profile_read(hqrrp_handle_t h, int nphase, double *ms,
                        long long *launches, double *flops, double *bytes);
 
