@@ -34,6 +34,13 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 
+# HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and --pmc WRITE_SIZE on this workload (summary and commands in
+# profiles/trail_update_16384_hbm_counters.md), each counter scaled by what it
+# reads for a plain copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE
+# x1.0).  Keyed by (phase, m, n, nb); other shapes report null.
+MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64): round(4.658e11 / 765)}
+
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {
     "sketch_gemm": "mfma",
@@ -93,7 +100,28 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--lookahead", type=int, default=1,
                     help="overlap the next panel with the trailing update")
+    ap.add_argument("--input", choices=("graded", "gaussian"),
+                    default="graded",
+                    help="graded: the benchmark matrix; gaussian: same shape, "
+                         "no QR in its generation (for hardware-counter runs, "
+                         "whose bytes per launch depend only on the shapes)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel (read MIB MiB, write "
+                         "MIB MiB) three times: the HBM-counter calibration "
+                         "for the roofline traffic")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 0.0, out=y)
+        torch.cuda.synchronize()
+        print(f"# copy kernel: {n * 8} bytes read and written per launch",
+              flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: HQRRP is single-device in this "
@@ -109,9 +137,12 @@ def main():
 
     m, n, nb, pp = args.m, args.n, args.nb, args.pp
     dev = "cuda:0"
-    print(f"# generating graded {m}x{n} input on the GPU", file=sys.stderr,
-          flush=True)
-    A0 = matgen.graded_device(m, n, seed=0, device=dev)
+    print(f"# generating {args.input} {m}x{n} input on the GPU",
+          file=sys.stderr, flush=True)
+    if args.input == "graded":
+        A0 = matgen.graded_device(m, n, seed=0, device=dev)
+    else:
+        A0 = matgen.gaussian_device(m, n, seed=0, device=dev)
     A = torch.empty_like(A0)
     jp = torch.empty(n, dtype=torch.int32, device=dev)
     ident = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
@@ -176,18 +207,21 @@ def main():
         t_ms, launches, fl, by = prof[name]
         bound = PHASE_BOUND[name]
         avg_s = t_ms / 1e3 / max(1, launches)
+        traffic = MEASURED_TRAFFIC.get((name, m, n, nb))
         if bound == "mfma":
             ach = fl / max(1, launches) / avg_s / 1e12
             roof = {"bound": "mfma", "kernel": name, "achieved": round(ach, 3),
                     "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                     "frac": round(ach * 1e12 / PEAK_FP64_MFMA, 4),
-                    "traffic": None, "launches": launches,
+                    "traffic": traffic, "launches": launches,
                     "avg_launch_us": round(avg_s * 1e6, 2)}
         else:
             ach = by / max(1, launches) / avg_s / 1e9
             roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                    "frac": round(ach * 1e9 / PEAK_HBM, 4), "traffic": None,
+                    "frac": round(ach * 1e9 / PEAK_HBM, 4),
+                    "traffic": traffic,
+                    "algorithmic_bytes_per_launch": round(by / max(1, launches)),
                     "launches": launches,
                     "avg_launch_us": round(avg_s * 1e6, 2)}
 

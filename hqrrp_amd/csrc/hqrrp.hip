@@ -135,7 +135,7 @@ extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
 enum {
   PH_SKETCH_GEMM = 0,  // Y = Omega*A
   PH_SKETCH_QRCP,      // pivot selection on the sketch
-  PH_GATHER,           // A/Y/jpvt column gathers
+  PH_GATHER,           // A/Y/jpvt column gathers, look-ahead column copies
   PH_PANEL,            // panel QRCP kernel
   PH_PANEL_POST,       // T factor + permutation of the rows above the panel
   PH_TRAIL_VTC,        // trailing M = V'C (+ split-K reduction)
@@ -649,9 +649,12 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
     // R12 rows and the sketch downdate on the whole chip (split-K GEMMs).
     {
-      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
-                    16.0 * (double)b * nc, 2);
+      PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)b * nc, 1);
       HQ_TRY(launch_copy_block(R12, b, C, ldx[xc], b, nc, h->cur));
+    }
+    {
+      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
+                    16.0 * (double)b * nc, 1);
       TrailArgs tr = ta;
       tr.mrows = b;
       tr.C = R12;
@@ -673,11 +676,14 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
     auto chain = [&]() -> int {
       if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
       {
-        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
-                      16.0 * (double)mnext * b2, 2);
+        PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)mnext * b2, 1);
         HQ_TRY(launch_select_cols(last_next ? nullptr : h->moves.as<int>(), b2,
                                   C, ldx[xc], b, mnext, ta.W, ta.ldw, b, Pn,
                                   mnext, h->Wsel.as<double>(), h->cur));
+      }
+      {
+        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
+                      16.0 * (double)mnext * b2, 1);
         TrailArgs tp = ta;
         tp.mrows = mnext;
         tp.nc = b2;
