@@ -41,6 +41,7 @@ SIGNATURES = {
     "hqrrp_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "hqrrp_destroy": (C.c_int, [_vp]),
     "hqrrp_set_stream": (C.c_int, [_vp, _vp]),
+    "hqrrp_reset_stream": (C.c_int, [_vp]),
     "hqrrp_synchronize": (C.c_int, [_vp]),
     "hqrrp_phase_count": (C.c_int, []),
     "hqrrp_phase_name": (C.c_char_p, [C.c_int]),
@@ -63,6 +64,16 @@ SIGNATURES = {
     "hqrrp_dgemm_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_double, _vp, C.c_int, _vp,
                                   C.c_int, C.c_double, _vp, C.c_int]),
+    "hqrrp_geqrf_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    "hqrrp_orgqr_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
+                                  C.c_int, _vp, _vp, C.c_int]),
+    "hqrrp_qr_fixed_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
+                                     C.c_int, _vp, _vp, C.c_int, _vp,
+                                     C.c_int]),
+    "hqrrp_qr_fixed_adjoint_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int,
+                                             _vp, C.c_int, _vp, _vp, C.c_int,
+                                             _vp, C.c_int, _vp, C.c_int, _vp,
+                                             C.c_int, _vp, C.c_int]),
 }
 
 _LIB = None

@@ -95,10 +95,15 @@ class Handle:
         return self._h
 
     def use_stream(self, stream) -> None:
-        """Run this handle's work on a torch.cuda.Stream (None: own stream)."""
-        check(lib().hqrrp_set_stream(
-            self._h, None if stream is None else C.c_void_p(stream.cuda_stream)),
-            "hqrrp_set_stream")
+        """Run this handle's work on a torch.cuda.Stream, ordered with torch's
+        work on it (torch's default stream is the null stream); None: the
+        handle's own stream."""
+        if stream is None:
+            check(lib().hqrrp_reset_stream(self._h), "hqrrp_reset_stream")
+        else:
+            check(lib().hqrrp_set_stream(self._h,
+                                         C.c_void_p(stream.cuda_stream)),
+                  "hqrrp_set_stream")
 
     def synchronize(self) -> None:
         check(lib().hqrrp_synchronize(self._h), "hqrrp_synchronize")
@@ -171,3 +176,118 @@ def hqrrp_device(handle: Handle, At, jpvt_t, tau_t, m: int, n: int,
                                 C.c_void_p(jpvt_t.data_ptr()),
                                 C.c_void_p(tau_t.data_ptr()), C.byref(o))
     check(rc, "hqrrp_factor_dev")
+
+
+# ------------------------------------- Q and the fixed-pivot QR with adjoint
+def _vp(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def geqrf_device(handle: Handle, At, tau_t, m: int, n: int,
+                 lda: int | None = None) -> None:
+    """Householder QR without pivoting of the column-major m x n matrix held
+    by At (shape (n, lda)), in place; tau_t: float64 (min(m,n))."""
+    check(lib().hqrrp_geqrf_dev(handle.ptr, m, n, _vp(At),
+                                m if lda is None else lda, _vp(tau_t)),
+          "hqrrp_geqrf_dev")
+
+
+def orgqr_device(handle: Handle, At, tau_t, m: int, n: int, k: int,
+                 lda: int | None = None):
+    """Q (column-major m x n, returned as a torch (n, m) tensor) from the
+    first k reflectors of At (dorgqr semantics)."""
+    import torch
+
+    Q = torch.empty((n, m), dtype=torch.float64, device=At.device)
+    check(lib().hqrrp_orgqr_dev(handle.ptr, m, n, k, _vp(At),
+                                m if lda is None else lda, _vp(tau_t), _vp(Q),
+                                m), "hqrrp_orgqr_dev")
+    return Q
+
+
+def _colmajor_of(A):
+    """torch m x n (any layout) -> contiguous (n, m): column-major m x n."""
+    return A.detach().t().contiguous()
+
+
+def qr_fixed(handle: Handle, A, jpvt, k: int):
+    """QR of A[:, jpvt-1] truncated to k: returns Q (m x k) and R (k x n) as
+    ordinary torch tensors on A's device (jpvt: 1-based int32 on the device).
+    """
+    import torch
+
+    m, n = A.shape
+    Acm = _colmajor_of(A.to(torch.float64))
+    Qcm = torch.empty((k, m), dtype=torch.float64, device=A.device)
+    Rcm = torch.empty((n, k), dtype=torch.float64, device=A.device)
+    jp = jpvt.to(device=A.device, dtype=torch.int32).contiguous()
+    check(lib().hqrrp_qr_fixed_dev(handle.ptr, m, n, k, _vp(Acm), m, _vp(jp),
+                                   _vp(Qcm), m, _vp(Rcm), k),
+          "hqrrp_qr_fixed_dev")
+    return Qcm.t(), Rcm.t()
+
+
+def qr_fixed_adjoint(handle: Handle, A, jpvt, k: int, Q, R, gQ, gR):
+    """Cotangent of A for qr_fixed at fixed pivots (torch m x n)."""
+    import torch
+
+    m, n = A.shape
+    f64 = torch.float64
+    Acm = _colmajor_of(A.to(f64))
+    Qcm, Rcm = _colmajor_of(Q.to(f64)), _colmajor_of(R.to(f64))
+    z = lambda shape: torch.zeros(shape, dtype=f64, device=A.device)
+    gQcm = _colmajor_of(gQ.to(f64)) if gQ is not None else z((k, m))
+    gRcm = _colmajor_of(gR.to(f64)) if gR is not None else z((n, k))
+    gAcm = torch.empty((n, m), dtype=f64, device=A.device)
+    jp = jpvt.to(device=A.device, dtype=torch.int32).contiguous()
+    check(lib().hqrrp_qr_fixed_adjoint_dev(
+        handle.ptr, m, n, k, _vp(Acm), m, _vp(jp), _vp(Qcm), m, _vp(Rcm), k,
+        _vp(gQcm), m, _vp(gRcm), k, _vp(gAcm), m),
+        "hqrrp_qr_fixed_adjoint_dev")
+    return gAcm.t()
+
+
+_DEFAULT_HANDLES = {}
+
+
+def _default_handle(device) -> Handle:
+    import torch
+
+    idx = torch.device(device).index or 0
+    if idx not in _DEFAULT_HANDLES:
+        h = Handle(idx)
+        _DEFAULT_HANDLES[idx] = h
+    h = _DEFAULT_HANDLES[idx]
+    h.use_stream(torch.cuda.current_stream(idx))
+    return h
+
+
+def _fixed_qr_function():
+    import torch
+
+    class FixedPivotQR(torch.autograd.Function):
+        """(Q, R) = QR of A[:, jpvt-1] truncated to k columns of Q, with the
+        pivots held fixed; backward through the library's adjoint."""
+
+        @staticmethod
+        def forward(ctx, A, jpvt, k):
+            h = _default_handle(A.device)
+            Q, R = qr_fixed(h, A, jpvt, k)
+            ctx.save_for_backward(A, jpvt, Q, R)
+            ctx.k = k
+            return Q, R
+
+        @staticmethod
+        def backward(ctx, gQ, gR):
+            A, jpvt, Q, R = ctx.saved_tensors
+            h = _default_handle(A.device)
+            gA = qr_fixed_adjoint(h, A, jpvt, ctx.k, Q, R, gQ, gR)
+            return gA.to(A.dtype), None, None
+
+    return FixedPivotQR
+
+
+def fixed_pivot_qr(A, jpvt, k: int):
+    """Differentiable (Q, R) of A[:, jpvt-1] truncated to k (see
+    hqrrp_qr_fixed_dev); gradients by hqrrp_qr_fixed_adjoint_dev."""
+    return _fixed_qr_function().apply(A, jpvt, k)

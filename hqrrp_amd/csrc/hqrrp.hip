@@ -158,6 +158,10 @@ struct hqrrp_handle_s {
   hipStream_t s_chain = nullptr, s_upd = nullptr;
   int chain_cus = 0;
   hipEvent_t ev_fork = nullptr, ev_upd = nullptr, ev_chain = nullptr;
+  // Joins the caller's stream to the handle's own stream when the caller's
+  // is the null stream (whose implicit synchronisation with every blocking
+  // stream would slow the loop's many small launches).
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   int cu_count = 0;
   size_t lds_max = 0;
   int sketch_max_wg = 0, panel_max_wg = 0;
@@ -167,6 +171,7 @@ struct hqrrp_handle_s {
   DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
   DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
+  DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT;  // device copies used by the host entry points
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
@@ -236,6 +241,29 @@ struct PhaseScope {
     if (!b) return;
     (void)hipEventRecord(b, h->cur);
     h->spans.push_back({ph, a, b});
+  }
+};
+
+// Sets h->cur for an entry point: the caller's stream, or -- when that is
+// the null stream -- the handle's own stream joined to it by events at
+// entry and exit (the null stream's implicit synchronisation with every
+// blocking stream would slow a sequence of small launches).
+struct StreamScope {
+  hqrrp_handle_t h;
+  bool joined = false;
+  explicit StreamScope(hqrrp_handle_t h_) : h(h_) {
+    h->cur = h->stream;
+    if (!h->stream && hipEventRecord(h->ev_in, nullptr) == hipSuccess &&
+        hipStreamWaitEvent(h->own, h->ev_in, 0) == hipSuccess) {
+      h->cur = h->own;
+      joined = true;
+    }
+  }
+  ~StreamScope() {
+    if (!joined) return;
+    (void)hipEventRecord(h->ev_out, h->own);
+    (void)hipStreamWaitEvent(nullptr, h->ev_out, 0);
+    h->cur = h->stream;
   }
 };
 
@@ -551,6 +579,9 @@ int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
 // write-back and the row permutation follow on the caller's stream.  Every
 // value is produced by the same kernel with the same operands as without
 // look-ahead, so the results are bitwise identical.
+int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
+                long lda, int* djpvt, double* dtau, const hqrrp_options& o);
+
 int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
                   int* djpvt, double* dtau, const hqrrp_options& o) {
   const int nb = o.nb, pp = o.pp, r = nb + pp;
@@ -562,6 +593,13 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   if (sketch_rmax(r) < 0 || nb > 64) return HQRRP_ERR_UNSUPPORTED;
 
   HQ_TRY(hipSetDevice(h->device));
+  StreamScope ss(h);
+  return factor_loop(h, m, n, kmax, dA, lda, djpvt, dtau, o);
+}
+
+int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
+                long lda, int* djpvt, double* dtau, const hqrrp_options& o) {
+  const int nb = o.nb, pp = o.pp, r = nb + pp;
   HQ_RET(load_omega(h, r, m, o));
   HQ_RET(ensure(h->Y, sizeof(double) * (size_t)r * n));
   HQ_RET(ensure(h->moves, sizeof(int) * (size_t)(1 + 4 * nb)));
@@ -582,7 +620,7 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   double* Y = h->Y.as<double>();
   double* G = h->G.as<double>();
   double* T = h->T.as<double>();
-  hipStream_t main_s = h->stream;
+  hipStream_t main_s = h->cur;
   // Trailing-matrix buffers; the trailing matrix lives in X[xc].
   double* X[2] = {dA, use_la ? h->Xb.as<double>() : nullptr};
   const long ldx[2] = {lda, (long)m};
@@ -740,6 +778,128 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   return HQRRP_OK;
 }
 
+// Householder QR without pivoting of the first kmax columns of A (m x n),
+// each block's reflectors applied to all columns to its right (LAPACK
+// dgeqrf; the reference calls it for dgeqp4's fixed columns, WY.c:221-225):
+// panel (pivoting off) + T, then the trailing dlarfb.
+int run_geqrf(hqrrp_handle_t h, int m, int n, int kmax, double* dA, long lda,
+              double* dtau) {
+  const int nb = 64;
+  HQ_RET(ensure(h->T, sizeof(double) * nb * nb));
+  HQ_RET(ensure(h->perm, sizeof(int) * 64));
+  for (int j = 0; j < kmax; j += nb) {
+    const int b = std::min(nb, std::min(kmax - j, m - j));
+    double* Ajj = dA + j + (long)j * lda;
+    HQ_RET(run_panel(h, m - j, b, Ajj, lda, 0, dtau + j, h->perm.as<int>(),
+                     h->T.as<double>(), nb));
+    if (j + b < n)
+      HQ_RET(apply_qt_left(h, m - j, n - j - b, b, Ajj, lda, h->T.as<double>(),
+                           nb, Ajj + (long)b * lda, lda));
+  }
+  return HQRRP_OK;
+}
+
+// Q (m x n) = the first n columns of H_1 ... H_k, the reflectors held in the
+// first k columns of A below the diagonal (LAPACK dorgqr, written to Q):
+// Q = [I; 0], then block reflectors from the last to the first applied to
+// Q(j:m, j:n) -- (I - V T V') C with T rebuilt from V'V (same larft kernel).
+int run_orgqr(hqrrp_handle_t h, int m, int n, int k, const double* dA,
+              long lda, const double* dtau, double* dQ, long ldq) {
+  const int nb = 64;
+  HQ_RET(ensure(h->T, sizeof(double) * nb * nb));
+  HQ_RET(ensure(h->gram, sizeof(double) * 64 * 64));
+  HQ_TRY(launch_fill_eye(dQ, ldq, m, n, 1.0, h->cur));
+  if (k <= 0) return HQRRP_OK;
+  for (int j = ((k - 1) / nb) * nb; j >= 0; j -= nb) {
+    const int b = std::min(nb, k - j), mrows = m - j;
+    const double* V = dA + j + (long)j * lda;
+    double* gram = h->gram.as<double>();
+    HQ_RET(gemm(h, true, false, b, b, mrows, 1.0, V, lda, 1, V, lda, 1, 0.0,
+                gram, b));
+    HQ_TRY(launch_larft(b, gram, dtau + j, h->T.as<double>(), nb, h->cur));
+    TrailArgs ta;
+    HQ_RET(trail_prepare(h, mrows, n - j, b, V, lda, h->T.as<double>(), nb,
+                         dQ + j + (long)j * ldq, ldq, &ta));
+    ta.tnotrans = 1;
+    HQ_RET(trail_w(h, ta));
+    HQ_RET(trail_apply(h, ta));
+  }
+  return HQRRP_OK;
+}
+
+// QR of AP = A(:, jpvt-1) truncated to k columns of Q: AP(:, 0:k) = Q R11,
+// R = [R11 R12] = Q' AP(:, 0:n) restricted to its first k rows.
+int run_qr_fixed(hqrrp_handle_t h, int m, int n, int k, const double* dA,
+                 long lda, const int* djpvt, double* dQ, long ldq, double* dR,
+                 long ldr) {
+  HQ_RET(ensure(h->adjAP, sizeof(double) * (size_t)m * n));
+  HQ_RET(ensure(h->adjTau, sizeof(double) * (size_t)std::max(1, k)));
+  double* AP = h->adjAP.as<double>();
+  HQ_TRY(launch_permute_cols(dA, lda, m, n, djpvt, 0, AP, m, h->cur));
+  HQ_RET(run_geqrf(h, m, n, k, AP, m, h->adjTau.as<double>()));
+  HQ_TRY(launch_copy_r(AP, m, k, n, dR, ldr, h->cur));
+  return run_orgqr(h, m, k, k, AP, m, h->adjTau.as<double>(), dQ, ldq);
+}
+
+// Reverse-mode derivative of run_qr_fixed (pivots held fixed).  With
+// X = AP(:, :k), Y = AP(:, k:), dQ' = dQ + Y dR12', M = R11 dR11' - dQ'' Q:
+//   dX = (dQ' + Q copyltu(M)) R11^-T,   dY = Q dR12,   dA(:, p) = [dX dY].
+int run_qr_fixed_adjoint(hqrrp_handle_t h, int m, int n, int k,
+                         const double* dA, long lda, const int* djpvt,
+                         const double* dQ, long ldq, const double* dR,
+                         long ldr, const double* gQ, long ldgq,
+                         const double* gR, long ldgr, double* gA, long ldga) {
+  HQ_RET(ensure(h->adjAP, sizeof(double) * (size_t)m * n));
+  HQ_RET(ensure(h->adjK, sizeof(double) * (size_t)3 * k * k));
+  double* G = h->adjAP.as<double>();  // m x n: [Z | Y], later [dX | dY]
+  double* Mk = h->adjK.as<double>();  // k x k each: M, copyltu(M), R11^-1
+  double* Bk = Mk + (size_t)k * k;
+  double* Ri = Bk + (size_t)k * k;
+  double* Z = G;
+  double* Y = G + (size_t)k * m;
+  const int nr = n - k;
+  // Z = dQ, Y = AP(:, k:)
+  HQ_TRY(launch_copy_block(Z, m, gQ, ldgq, m, k, h->cur));
+  if (nr > 0) {
+    HQ_TRY(launch_permute_cols(dA, lda, m, nr, djpvt + k, 0, Y, m, h->cur));
+    // Z += Y dR12'
+    HQ_RET(gemm(h, false, true, m, k, nr, 1.0, Y, m, 0, gR + (long)k * ldgr,
+                ldgr, 0, 1.0, Z, m));
+  }
+  // M = R11 dR11' - Z' Q
+  HQ_RET(gemm(h, false, true, k, k, k, 1.0, dR, ldr, 0, gR, ldgr, 0, 0.0, Mk,
+              k));
+  HQ_RET(gemm(h, true, false, k, k, m, -1.0, Z, m, 0, dQ, ldq, 0, 1.0, Mk, k));
+  HQ_TRY(launch_copyltu(k, Mk, k, Bk, k, h->cur));
+  // Z += Q copyltu(M)
+  HQ_RET(gemm(h, false, false, m, k, k, 1.0, dQ, ldq, 0, Bk, k, 0, 1.0, Z, m));
+  // R11^-1: diagonal 64-blocks in LDS, then doubling:
+  //   Ri(B1,B2) = -Ri(B1,B1) R(B1,B2) Ri(B2,B2)
+  HQ_TRY(launch_fill_eye(Ri, k, k, k, 0.0, h->cur));
+  HQ_TRY(launch_tri_inv_diag(k, dR, ldr, Ri, k, h->cur));
+  for (int s = tri_inv_block(); s < k; s *= 2) {
+    for (int i0 = 0; i0 + s < k; i0 += 2 * s) {
+      const int i1 = i0 + s, s2 = std::min(s, k - i1);
+      // Bk(0:s, 0:s2) = Ri(B1,B1) R(B1,B2); Ri(B1,B2) = -Bk Ri(B2,B2)
+      HQ_RET(gemm(h, false, false, s, s2, s, 1.0, Ri + i0 + (long)i0 * k, k, 0,
+                  dR + i0 + (long)i1 * ldr, ldr, 0, 0.0, Bk, k));
+      HQ_RET(gemm(h, false, false, s, s2, s2, -1.0, Bk, k, 0,
+                  Ri + i1 + (long)i1 * k, k, 0, 0.0, Ri + i0 + (long)i1 * k, k));
+    }
+  }
+  // dY = Q dR12 (over Y), dX = Z R11^-T (then over Z)
+  if (nr > 0)
+    HQ_RET(gemm(h, false, false, m, nr, k, 1.0, dQ, ldq, 0,
+                gR + (long)k * ldgr, ldgr, 0, 0.0, Y, m));
+  HQ_RET(ensure(h->adjX, sizeof(double) * (size_t)m * k));
+  HQ_RET(gemm(h, false, true, m, k, k, 1.0, Z, m, 0, Ri, k, 0, 0.0,
+              h->adjX.as<double>(), m));
+  HQ_TRY(launch_copy_block(Z, m, h->adjX.as<double>(), m, m, k, h->cur));
+  // dA(:, p[c]-1) = [dX dY](:, c)
+  HQ_TRY(launch_permute_cols(G, m, m, n, djpvt, 1, gA, ldga, h->cur));
+  return HQRRP_OK;
+}
+
 std::mutex g_default_mu;
 hqrrp_handle_t g_default[64];
 
@@ -820,6 +980,8 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
       h->s_chain = h->s_upd = nullptr;
       h->chain_cus = 0;
     }
+    HQ_TRY(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    HQ_TRY(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
     if (h->s_chain) {
       HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, hipEventDisableTiming));
@@ -843,11 +1005,12 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
                     &h->hA,    &h->hJ,    &h->hT,     &h->R12,
-                    &h->Pn,    &h->Wsel,  &h->Xb};
+                    &h->Pn,    &h->Wsel,  &h->Xb,     &h->adjAP,
+                    &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain})
+  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain, h->ev_in, h->ev_out})
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t st : {h->s_chain, h->s_upd})
     if (st) (void)hipStreamDestroy(st);
@@ -905,7 +1068,15 @@ extern "C" int hqrrp_profile_read(hqrrp_handle_t h, int nphase, double* ms,
 
 extern "C" int hqrrp_set_stream(hqrrp_handle_t h, void* stream) {
   if (!h) return HQRRP_ERR_ARG;
-  h->stream = stream ? (hipStream_t)stream : h->own;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->stream = (hipStream_t)stream;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_reset_stream(hqrrp_handle_t h) {
+  if (!h) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->stream = h->own;
   return HQRRP_OK;
 }
 
@@ -984,6 +1155,51 @@ extern "C" int hqrrp_dgemm_dev(hqrrp_handle_t h, int ta, int tb, int M, int N,
   h->cur = h->stream;
   return gemm(h, ta != 0, tb != 0, M, N, K, alpha, dA, lda, 0, dB, ldb, 0,
               beta, dC, ldc);
+}
+
+extern "C" int hqrrp_geqrf_dev(hqrrp_handle_t h, int m, int n, double* dA,
+                               int lda, double* dtau) {
+  if (!h || m < 0 || n < 0 || lda < std::max(1, m)) return HQRRP_ERR_ARG;
+  if (std::min(m, n) == 0) return HQRRP_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_geqrf(h, m, n, std::min(m, n), dA, lda, dtau);
+}
+
+extern "C" int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k,
+                               const double* dA, int lda, const double* dtau,
+                               double* dQ, int ldq) {
+  if (!h || m < 0 || n < 0 || n > m || k < 0 || k > n ||
+      lda < std::max(1, m) || ldq < std::max(1, m))
+    return HQRRP_ERR_ARG;
+  if (n == 0) return HQRRP_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_orgqr(h, m, n, k, dA, lda, dtau, dQ, ldq);
+}
+
+extern "C" int hqrrp_qr_fixed_dev(hqrrp_handle_t h, int m, int n, int k,
+                                  const double* dA, int lda, const int* djpvt,
+                                  double* dQ, int ldq, double* dR, int ldr) {
+  if (!h || k < 1 || n < k || m < k || lda < m || ldq < m || ldr < k)
+    return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_qr_fixed(h, m, n, k, dA, lda, djpvt, dQ, ldq, dR, ldr);
+}
+
+extern "C" int hqrrp_qr_fixed_adjoint_dev(
+    hqrrp_handle_t h, int m, int n, int k, const double* dA, int lda,
+    const int* djpvt, const double* dQ, int ldq, const double* dR, int ldr,
+    const double* gQ, int ldgq, const double* gR, int ldgr, double* gA,
+    int ldga) {
+  if (!h || k < 1 || n < k || m < k || lda < m || ldq < m || ldr < k ||
+      ldgq < m || ldgr < k || ldga < m)
+    return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_qr_fixed_adjoint(h, m, n, k, dA, lda, djpvt, dQ, ldq, dR, ldr, gQ,
+                              ldgq, gR, ldgr, gA, ldga);
 }
 
 // --------------------------------------------------- drop-in host entries

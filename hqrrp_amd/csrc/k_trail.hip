@@ -340,8 +340,9 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc64_kernel(TrailArgs a) {
 }
 
 
-// W(:, n0:n0+64) = T' * sum_s part[s](:, n0:n0+64), slices added in order,
-// stored transposed (W' is nc x b) for coalesced staging in trail_update.
+// W(:, n0:n0+64) = T' * sum_s part[s](:, n0:n0+64) (or T * ..., to apply Q
+// instead of Q'), slices added in order, stored transposed (W' is nc x b) for
+// coalesced staging in trail_update.
 __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
   __shared__ double Ms[kTile][kTile + 1];  // Ms[p][n]
   __shared__ double Ts[kTile][kTile + 1];  // Ts[p][q] = T(p, q)
@@ -366,7 +367,11 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
     long gn = n0 + n;
     if (gn >= a.nc) continue;
     double s = 0.0;
-    for (int p = 0; p <= q; ++p) s += Ts[p][q] * Ms[p][n];
+    if (a.tnotrans) {
+      for (int p = q; p < b; ++p) s += Ts[q][p] * Ms[p][n];  // (T M)(q, n)
+    } else {
+      for (int p = 0; p <= q; ++p) s += Ts[p][q] * Ms[p][n];  // (T' M)(q, n)
+    }
     a.W[gn + (long)q * a.ldw] = s;  // W' (nc x b), ld ldw
   }
 }

@@ -52,6 +52,7 @@ struct TrailArgs {
   int mrows, nc, b, splits;
   int vmask;        // 1: V's top b x b block is unit lower (masked)
   int wide;         // trail_vtc: 256-column tiles (else 64)
+  int tnotrans;     // trail_tw: W = T M (apply Q) instead of T' M (apply Q')
   const double* V;  // mrows x b reflectors (unit lower trapezoidal), ld ldv
   long ldv;
   double* C;  // mrows x nc trailing matrix, ld ldc
@@ -109,3 +110,18 @@ hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
                               long lda, int m, double* Y, long ldy, int r,
                               int* jpvt, double* tmpA, double* tmpY, int* tmpJ,
                               hipStream_t s);
+
+// k_adj.hip
+hipError_t launch_fill_eye(double* Q, long ldq, int m, int n, double diag,
+                           hipStream_t s);
+hipError_t launch_copy_r(const double* A, long lda, int k, int n, double* R,
+                         long ldr, hipStream_t s);
+hipError_t launch_tri_inv_diag(int k, const double* R, long ldr, double* X,
+                               long ldx, hipStream_t s);
+int tri_inv_block();
+hipError_t launch_copyltu(int k, const double* M, long ldm, double* B, long ldb,
+                          hipStream_t s);
+// scatter = 0: dst(:, c) = src(:, p[c]-1); 1: dst(:, p[c]-1) = src(:, c).
+hipError_t launch_permute_cols(const double* src, long lds, int m, int ncols,
+                               const int* p, int scatter, double* dst,
+                               long ldd, hipStream_t s);

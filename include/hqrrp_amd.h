@@ -95,9 +95,11 @@ void hqrrp_default_options(hqrrp_options *opt);
 /* Handle: owns device workspace, the grid-barrier words and a stream. */
 int hqrrp_create(hqrrp_handle_t *h, int device);
 int hqrrp_destroy(hqrrp_handle_t h);
-/* Use `stream` (a hipStream_t) for all work of this handle; NULL restores
- * the handle's own stream. */
+/* Use `stream` (a hipStream_t) for all work of this handle; NULL is the
+ * default (null) stream, as in hipBLAS/cuBLAS.  A new handle uses a stream
+ * of its own; reset_stream returns to it. */
 int hqrrp_set_stream(hqrrp_handle_t h, void *stream);
+int hqrrp_reset_stream(hqrrp_handle_t h);
 int hqrrp_synchronize(hqrrp_handle_t h);
 
 /* Optional per-phase timing (no counterpart in the reference, which only has
@@ -125,6 +127,44 @@ int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
  * the stream srand(seed) + polar method, without touching the caller's
  * rand() state.  (NoFLA_HQRRP_WY_blk_var4.c:380, 575-614) */
 void hqrrp_reference_omega(int r, int m, double *G, unsigned seed);
+
+/* ------------------------------- Q, fixed-pivot QR and its adjoint (device)
+ * Device pointers, stream-ordered on the handle's stream. */
+
+/* Householder QR without pivoting, LAPACK dgeqrf semantics: A (m x n, ld
+ * lda) factored in place, tau (min(m,n)).  Replaces the dgeqrf_ call the
+ * reference makes for dgeqp4's fixed columns (NoFLA_HQRRP_WY_blk_var4.c:
+ * 221-225). */
+int hqrrp_geqrf_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
+                    double *dtau);
+
+/* Q (m x n) = the first n columns of H_1 ... H_k, with the reflectors in the
+ * first k columns of A below the diagonal and tau (LAPACK dorgqr semantics,
+ * written out of place to Q); k <= n <= m.  Replaces the dorgqr_ call of
+ * the reference's test driver (WYtest.c:89-93). */
+int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k, const double *dA,
+                    int lda, const double *dtau, double *dQ, int ldq);
+
+/* QR at fixed pivots, truncated to k (no counterpart in the reference):
+ * with AP = A(:, jpvt-1) (jpvt 1-based, n entries), AP(:, 0:k) = Q R11 and
+ * R = [R11 R12] = the first k rows of the R factor of AP.  Q: m x k (ld
+ * ldq), R: k x n (ld ldr, zero below the diagonal).  1 <= k <= min(m,n). */
+int hqrrp_qr_fixed_dev(hqrrp_handle_t h, int m, int n, int k, const double *dA,
+                       int lda, const int *djpvt, double *dQ, int ldq,
+                       double *dR, int ldr);
+
+/* Reverse-mode derivative of hqrrp_qr_fixed_dev with the pivots held fixed
+ * (no counterpart in the reference): given A, jpvt, the forward's Q and R,
+ * and the cotangents gQ (m x k), gR (k x n), writes gA (m x n):
+ *   X = AP(:, :k), Y = AP(:, k:), gQ' = gQ + Y gR12',
+ *   M = R11 gR11' - gQ''Q, gX = (gQ' + Q copyltu(M)) R11^-T, gY = Q gR12,
+ *   gA(:, jpvt-1) = [gX gY]. */
+int hqrrp_qr_fixed_adjoint_dev(hqrrp_handle_t h, int m, int n, int k,
+                               const double *dA, int lda, const int *djpvt,
+                               const double *dQ, int ldq, const double *dR,
+                               int ldr, const double *gQ, int ldgq,
+                               const double *gR, int ldgr, double *gA,
+                               int ldga);
 
 /* ------------------------------------- component entry points (device) */
 
