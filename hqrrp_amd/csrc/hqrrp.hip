@@ -172,7 +172,7 @@ struct hqrrp_handle_s {
   DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
-  DevBuf hA, hJ, hT;  // device copies used by the host entry points
+  DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
   std::mutex mu;
@@ -1004,7 +1004,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
-                    &h->hA,    &h->hJ,    &h->hT,     &h->R12,
+                    &h->hA,    &h->hJ,    &h->hT,     &h->hW,     &h->R12,
                     &h->Pn,    &h->Wsel,  &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
@@ -1235,6 +1235,70 @@ int host_hqrrp(int m, int n, double* A, int lda, int* jpvt, double* tau,
   return HQRRP_OK;
 }
 
+// dgeqp4 with nf > 0 fixed columns already moved to the front
+// (NoFLA_HQRRP_WY_blk_var4.c:220-292): QR of the fixed columns with the
+// reflectors applied to the rest (the reference's dgeqrf + dormqr), HQRRP of
+// the free block A(nf:, nf:) permuting jpvt(nf:), then the rows above the
+// free block follow its pivots -- one gather by the inverse of the previous
+// jpvt instead of the reference's O(n^2) search (same columns end up in the
+// same places).
+int host_dgeqp4_fixed(int m, int n, double* A, int lda, int* jpvt, double* tau,
+                      int nf, const hqrrp_options& o) {
+  hqrrp_handle_t h = nullptr;
+  HQ_RET(default_handle(&h));
+  std::lock_guard<std::mutex> lk(h->mu);
+  const int mn = std::min(m, n);
+  HQ_RET(ensure(h->hA, sizeof(double) * (size_t)std::max(1, m) * n));
+  HQ_RET(ensure(h->hJ, sizeof(int) * (size_t)std::max(1, n)));
+  HQ_RET(ensure(h->hT, sizeof(double) * (size_t)std::max(1, mn)));
+  double* dA = h->hA.as<double>();
+  int* dJ = h->hJ.as<int>();
+  double* dT = h->hT.as<double>();
+  h->cur = h->stream;
+  HQ_TRY(hipMemcpy2DAsync(dA, sizeof(double) * m, A, sizeof(double) * lda,
+                          sizeof(double) * m, n, hipMemcpyHostToDevice,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(dJ, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
+                        h->stream));
+  HQ_RET(run_geqrf(h, m, n, nf, dA, m, dT));
+  if (nf < mn) {
+    std::vector<int> prev(jpvt, jpvt + n);
+    HQ_RET(factor_locked(h, m - nf, n - nf, dA + nf + (long)nf * m, m, dJ + nf,
+                         dT + nf, o));
+    HQ_RET(check_err(h));
+    HQ_TRY(hipMemcpyAsync(jpvt, dJ, sizeof(int) * n, hipMemcpyDeviceToHost,
+                          h->stream));
+    HQ_TRY(hipStreamSynchronize(h->stream));
+    // Column j >= nf of the top block comes from the position that held
+    // column id jpvt[j] before the HQRRP.
+    std::vector<int> pos(n + 1, -1), src(n - nf);
+    for (int k = nf; k < n; ++k) pos[prev[k]] = k;
+    for (int j = nf; j < n; ++j) {
+      const int v = jpvt[j];
+      if (v < 1 || v > n || pos[v] < nf) return HQRRP_ERR_KERNEL;
+      src[j - nf] = pos[v] - nf + 1;
+    }
+    HQ_RET(ensure(h->hW, sizeof(double) * (size_t)nf * (n - nf) +
+                             sizeof(int) * (size_t)(n - nf)));
+    double* top = h->hW.as<double>();
+    int* dsrc = reinterpret_cast<int*>(top + (size_t)nf * (n - nf));
+    HQ_TRY(hipMemcpyAsync(dsrc, src.data(), sizeof(int) * (n - nf),
+                          hipMemcpyHostToDevice, h->stream));
+    HQ_TRY(launch_permute_cols(dA + (long)nf * m, m, nf, n - nf, dsrc, 0, top,
+                               nf, h->stream));
+    HQ_TRY(launch_copy_block(dA + (long)nf * m, m, top, nf, nf, n - nf,
+                             h->stream));
+  }
+  HQ_RET(check_err(h));
+  HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
+                          sizeof(double) * m, n, hipMemcpyDeviceToHost,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(tau, dT, sizeof(double) * mn, hipMemcpyDeviceToHost,
+                        h->stream));
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  return HQRRP_OK;
+}
+
 }  // namespace
 
 extern "C" int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double* buff_A,
@@ -1320,16 +1384,10 @@ extern "C" void dgeqp4(int* m, int* n, double* A, int* lda, int* jpvt,
     }
   }
   const int nf = std::min(mA, nfixed);
-  if (nf > 0) {
-    // Fixed-column factorisation (dgeqrf + dormqr in the reference) is not
-    // implemented on the GPU yet.
-    fprintf(stderr, "dgeqp4: fixed columns are not supported by this build\n");
-    *info = -5;
-    return;
-  }
   hqrrp_options o;
   hqrrp_default_options(&o);
-  int rc = host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
+  int rc = nf > 0 ? host_dgeqp4_fixed(mA, nA, A, ld, jpvt, tau, nf, o)
+                  : host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
   if (rc != HQRRP_OK) {
     fprintf(stderr, "dgeqp4: device factorisation failed (%d)\n", rc);
     *info = rc;

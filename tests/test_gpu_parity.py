@@ -101,6 +101,30 @@ def test_dgeqp4_front_end(handle):
     assert res <= 50 * 1200 * EPS and orth <= 50 * 1200 * EPS
 
 
+@pytest.mark.parametrize("fixed", [[3, 17, 400, 999], list(range(0, 80)),
+                                   list(range(500, 1000, 7))])
+def test_dgeqp4_fixed_columns(handle, fixed):
+    """Fixed leading columns (jpvt != 0): the GPU path (QR of the fixed
+    columns, HQRRP of the free block, re-pivoted top rows) gives the oracle's
+    pivots and a valid factorization (NoFLA_HQRRP_WY_blk_var4.c:201-292)."""
+    import hqrrp_amd
+
+    m, n = 1200, 1000
+    A = matgen.gaussian(m, n, 23)
+    jin = np.zeros(n, dtype=np.int32)
+    jin[fixed] = 1
+    Ag, jg, tg, info, _ = hqrrp_amd.dgeqp4(A, jpvt=jin)
+    Ar, jr, tr, info_r, _ = oracle.dgeqp4(A, jpvt=jin)
+    assert info == 0 and info_r == 0
+    nf = len(fixed)
+    assert sorted(jg[:nf].tolist()) == [c + 1 for c in fixed]
+    assert np.array_equal(jg, jr)
+    res, orth = oracle.residuals(A, Ag, jg, tg)
+    assert res <= 50 * m * EPS and orth <= 50 * m * EPS
+    dg, dr = np.abs(np.diag(Ag)), np.abs(np.diag(Ar))
+    assert np.abs(dg - dr).max() <= 1e-10 * dr.max()
+
+
 def test_deterministic(handle):
     A = matgen.graded(1300, 1300, 31)
     a1, j1, t1 = run_gpu(handle, A, 64, 10, 1)
