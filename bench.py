@@ -100,11 +100,17 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--lookahead", type=int, default=1,
                     help="overlap the next panel with the trailing update")
-    ap.add_argument("--input", choices=("graded", "gaussian"),
+    ap.add_argument("--input", choices=("graded", "gaussian", "lowrank"),
                     default="graded",
                     help="graded: the benchmark matrix; gaussian: same shape, "
                          "no QR in its generation (for hardware-counter runs, "
-                         "whose bytes per launch depend only on the shapes)")
+                         "whose bytes per launch depend only on the shapes); "
+                         "lowrank: rank 512 (Gaussian factors, sigma_i = 1/i) "
+                         "+ 1e-8 N(0,1), the mixed-precision workload")
+    ap.add_argument("--sketch", choices=("fp64", "fp32", "bf16"),
+                    default="fp64",
+                    help="sketch precision (panel, T and trailing update are "
+                         "FP64 in every case)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel (read MIB MiB, write "
                          "MIB MiB) three times: the HBM-counter calibration "
@@ -141,8 +147,11 @@ def main():
           file=sys.stderr, flush=True)
     if args.input == "graded":
         A0 = matgen.graded_device(m, n, seed=0, device=dev)
+    elif args.input == "lowrank":
+        A0 = matgen.lowrank_device(m, n, rank=512, seed=0, device=dev)
     else:
         A0 = matgen.gaussian_device(m, n, seed=0, device=dev)
+    sketch_prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.sketch]
     A = torch.empty_like(A0)
     jp = torch.empty(n, dtype=torch.int32, device=dev)
     ident = torch.arange(1, n + 1, dtype=torch.int32, device=dev)
@@ -154,7 +163,8 @@ def main():
         A.copy_(A0)
         jp.copy_(ident)
         hqrrp_amd.hqrrp_device(h, A, jp, tau, m, n, nb=nb, pp=pp,
-                               panel_pivoting=1, lookahead=args.lookahead)
+                               panel_pivoting=1, lookahead=args.lookahead,
+                               sketch_prec=sketch_prec)
 
     for i in range(args.warmup):
         step()
@@ -235,16 +245,21 @@ def main():
 
     h.close()
     line = {
-        "metric": "QRCP GFLOP/s (2mn^2-2/3n^3), plain FP64 sketch",
+        "metric": ("QRCP GFLOP/s (2mn^2-2/3n^3), plain FP64 sketch"
+                   if args.sketch == "fp64" else
+                   f"QRCP GFLOP/s (2mn^2-2/3n^3), mixed {args.sketch} sketch"),
         "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"HQRRP plain FP64 {m}x{n} graded "
-                               f"(Haar U,V, sigma_i=10^(-12i/n))",
+        "config": {"workload": (
+                       f"HQRRP plain FP64 {m}x{n} graded "
+                       f"(Haar U,V, sigma_i=10^(-12i/n))"
+                       if args.input == "graded" and args.sketch == "fp64" else
+                       f"HQRRP {args.sketch} sketch {m}x{n} {args.input}"),
                    "m": m, "n": n, "nb": nb, "pp": pp, "panel_pivoting": 1,
-                   "sketch": "fp64", "omega": "reference stream seed 12",
+                   "sketch": args.sketch, "omega": "reference stream seed 12",
                    "lookahead": args.lookahead},
         "roofline": roof, "cpu_baseline": cpu, "phase_ms": phase_ms,
         "gemm_roofline": gemm_roof,

@@ -124,61 +124,93 @@ __device__ __forceinline__ void hq_wave_argmax(double& key, int& pos,
   key = m;
 }
 
-// LAPACK dlapy2: sqrt(x^2 + y^2) without destructive overflow.
-__device__ __forceinline__ double hq_dlapy2(double x, double y) {
-  double xa = fabs(x), ya = fabs(y);
-  double w = fmax(xa, ya), z = fmin(xa, ya);
-  if (z == 0.0 || w > 1.79769313486231571e308) return w;
-  double q = z / w;
-  return w * sqrt(1.0 + q * q);
+// Largest finite value of T (LAPACK's overflow threshold in dlapy2/slapy2).
+template <class T>
+__device__ __forceinline__ T hq_huge();
+template <>
+__device__ __forceinline__ double hq_huge<double>() {
+  return 1.79769313486231571e308;
+}
+template <>
+__device__ __forceinline__ float hq_huge<float>() {
+  return 3.40282347e38f;
+}
+
+// LAPACK dlapy2 / slapy2: sqrt(x^2 + y^2) without destructive overflow.
+template <class T>
+__device__ __forceinline__ T hq_dlapy2(T x, T y) {
+  T xa = fabs(x), ya = fabs(y);
+  T w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == T(0) || w > hq_huge<T>()) return w;
+  T q = z / w;
+  return w * sqrt(T(1) + q * q);
 }
 
 // Householder generation with LAPACK's conventions (dlarfg): given alpha and
 // the squared 2-norm of x, returns beta = -sign(alpha)*||(alpha,x)||,
 // tau = (beta-alpha)/beta and the scale 1/(alpha-beta) applied to x.  When
 // x is zero tau = 0, beta = alpha and x is left unscaled (scale = 1).
-struct hq_reflector {
-  double beta, tau, scale;
+template <class T>
+struct hq_reflector_t {
+  T beta, tau, scale;
 };
-__device__ __forceinline__ hq_reflector hq_dlarfg(double alpha, double xnorm2) {
-  hq_reflector h;
-  double xnorm = sqrt(xnorm2);
-  if (xnorm == 0.0) {
+typedef hq_reflector_t<double> hq_reflector;
+// In the arithmetic of T (dlarfg for double, slarfg for float).
+template <class T>
+__device__ __forceinline__ hq_reflector_t<T> hq_larfg(T alpha, T xnorm2) {
+  hq_reflector_t<T> h;
+  T xnorm = sqrt(xnorm2);
+  if (xnorm == T(0)) {
     h.beta = alpha;
-    h.tau = 0.0;
-    h.scale = 1.0;
+    h.tau = T(0);
+    h.scale = T(1);
     return h;
   }
-  double beta = -copysign(hq_dlapy2(alpha, xnorm), alpha);
+  T beta = -copysign(hq_dlapy2<T>(alpha, xnorm), alpha);
   h.beta = beta;
   h.tau = (beta - alpha) / beta;
-  h.scale = 1.0 / (alpha - beta);
+  h.scale = T(1) / (alpha - beta);
   return h;
 }
+__device__ __forceinline__ hq_reflector hq_dlarfg(double alpha, double xnorm2) {
+  return hq_larfg<double>(alpha, xnorm2);
+}
 
-// sqrt(eps) with eps = 2^-53 (LAPACK dlamch('E')): Drmac's recompute bound.
+// sqrt(eps) with eps = 2^-53 (LAPACK dlamch('E')): Drmac's recompute bound;
+// for float, sqrt(slamch('E')) = sqrt(2^-24).
 #define HQ_TOL3Z 1.0536712127723509e-08
+#define HQ_TOL3Z_F32 2.44140625e-04f
+template <class T>
+__device__ __forceinline__ T hq_tol3z();
+template <>
+__device__ __forceinline__ double hq_tol3z<double>() {
+  return HQ_TOL3Z;
+}
+template <>
+__device__ __forceinline__ float hq_tol3z<float>() {
+  return HQ_TOL3Z_F32;
+}
 
 // LAWN 176 partial-norm downdate for one column.  `wt` is the entry of the
 // column in the row just eliminated, `below2` the squared norm of the part of
 // the column under that row (used when the downdated value is not trusted),
 // `mbelow` its length.
-__device__ __forceinline__ void hq_downdate_norm(double& d, double& e,
-                                                 double wt, double below2,
+template <class T>
+__device__ __forceinline__ void hq_downdate_norm(T& d, T& e, T wt, T below2,
                                                  int mbelow) {
-  if (d != 0.0) {
-    double t = fabs(wt) / d;
-    double u = (1.0 + t) * (1.0 - t);
-    t = (0.0 > u) ? 0.0 : u;
-    double ratio = d / e;
-    double t2 = t * ratio * ratio;
-    if (t2 <= HQ_TOL3Z) {
+  if (d != T(0)) {
+    T t = fabs(wt) / d;
+    T u = (T(1) + t) * (T(1) - t);
+    t = (T(0) > u) ? T(0) : u;
+    T ratio = d / e;
+    T t2 = t * ratio * ratio;
+    if (t2 <= hq_tol3z<T>()) {
       if (mbelow > 0) {
         d = sqrt(below2);
         e = d;
       } else {
-        d = 0.0;
-        e = 0.0;
+        d = T(0);
+        e = T(0);
       }
     } else {
       d = d * sqrt(t);

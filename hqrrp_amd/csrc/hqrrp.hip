@@ -169,6 +169,9 @@ struct hqrrp_handle_s {
   DevBuf Y, G, Gcache, moves, piv, gram, T, perm;
   Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
   DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
+  // Sketch precision of the running factorisation: Y and G hold floats
+  // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
+  int sk32 = 0, skbf = 0;
   DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
@@ -343,6 +346,47 @@ void xchg_end(Xchg& x, size_t used) {
   x.par ^= 1;
 }
 
+// C (float) := alpha op(A) op(B) + beta C in FP32 arithmetic; each operand
+// FP64 (x64 = 1) or FP32; automatic split-K (k_gemm32.hip).
+int gemm32(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K,
+           float alpha, const void* A, long lda, int a64, int maskA,
+           const void* B, long ldb, int b64, int maskB, float beta, float* C,
+           long ldc, int bf16, bool allow_split = true) {
+  if (M <= 0 || N <= 0) return HQRRP_OK;
+  Gemm32Args g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.a64 = a64;
+  g.B = B;
+  g.ldb = ldb;
+  g.b64 = b64;
+  g.C = C;
+  g.ldc = ldc;
+  g.maskA = maskA;
+  g.maskB = maskB;
+  g.bf16 = bf16;
+  long tiles = (long)hq_cdiv(M, 64) * hq_cdiv(N, 64);
+  long kc = K > 0 ? K : 16;
+  int S = 1;
+  if (allow_split && K > 0) S = choose_splits(h, tiles, K, &kc);
+  g.kchunk = kc;
+  if (S > 1) {
+    HQ_RET(ensure(h->split, sizeof(float) * (size_t)S * M * N));
+    g.part = h->split.as<float>();
+    HQ_TRY(launch_gemm32(g, ta, tb, S, h->cur));
+    HQ_TRY(launch_reduce32(M, N, S, g.part, alpha, beta, C, ldc, h->cur));
+  } else {
+    g.part = nullptr;
+    HQ_TRY(launch_gemm32(g, ta, tb, 1, h->cur));
+  }
+  return HQRRP_OK;
+}
+
 int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
   int n = hq_cdiv(mrows, panel_rows_per_wg());
   if (n > h->panel_max_wg) return HQRRP_ERR_UNSUPPORTED;
@@ -350,13 +394,14 @@ int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
   return HQRRP_OK;
 }
 
-int run_sketch(hqrrp_handle_t h, int r, int ncols, const double* dY, long ldy,
-               int stages, int* dpiv, int* dmoves) {
+int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
+               int stages, int* dpiv, int* dmoves, int f32 = 0) {
   int rmax = sketch_rmax(r);
   if (rmax < 0) return HQRRP_ERR_UNSUPPORTED;
   SketchArgs a{};
   a.Y = dY;
   a.ldy = ldy;
+  a.f32 = f32;
   a.r = r;
   a.ncols = ncols;
   a.stages = stages;
@@ -497,10 +542,36 @@ int downdate_y(hqrrp_handle_t h, int r, int b, int mU, int nY2, const double* U,
   return HQRRP_OK;
 }
 
+// downdate_y with G and Y2 in FP32 (the mixed-precision sketch): the
+// same four products in FP32 arithmetic on the FP64 reflectors, T and R12.
+int downdate_y32(hqrrp_handle_t h, int r, int b, int mU, int nY2,
+                 const double* U, long ldu, const double* R12, long ldr,
+                 const double* T, long ldt, float* Y2, long ldy, float* G,
+                 long ldg) {
+  HQ_RET(ensure(h->Pp, sizeof(float) * (size_t)r * b));
+  HQ_RET(ensure(h->Qm, sizeof(float) * (size_t)r * b));
+  float* P = h->Pp.as<float>();
+  float* Q = h->Qm.as<float>();
+  HQ_RET(gemm32(h, false, false, r, b, mU, 1.0f, G, ldg, 0, 0, U, ldu, 1, 1,
+                0.0f, P, r, 0));
+  HQ_RET(gemm32(h, false, false, r, b, b, 1.0f, P, r, 0, 0, T, ldt, 1, 0,
+                0.0f, Q, r, 0, false));
+  HQ_RET(gemm32(h, false, true, r, mU, b, -1.0f, Q, r, 0, 0, U, ldu, 1, 1,
+                1.0f, G, ldg, 0, false));
+  if (nY2 > 0)
+    HQ_RET(gemm32(h, false, false, r, nY2, b, -1.0f, G, ldg, 0, 0, R12, ldr, 1,
+                  0, 1.0f, Y2, ldy, 0, false));
+  return HQRRP_OK;
+}
+
 int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
   HQ_RET(ensure(h->G, sizeof(double) * (size_t)r * m));
   if (o.omega_kind == HQRRP_OMEGA_USER) {
     if (!o.omega) return HQRRP_ERR_ARG;
+    if (h->sk32) {
+      HQ_TRY(launch_cast_f32(o.omega, h->G.as<float>(), (long)r * m, h->cur));
+      return HQRRP_OK;
+    }
     HQ_TRY(hipMemcpyAsync(h->G.p, o.omega, sizeof(double) * (size_t)r * m,
                           hipMemcpyDeviceToDevice, h->cur));
     return HQRRP_OK;
@@ -515,6 +586,11 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
     h->om_m = m;
     h->om_seed = o.seed;
   }
+  if (h->sk32) {
+    HQ_TRY(launch_cast_f32(h->Gcache.as<double>(), h->G.as<float>(),
+                           (long)r * m, h->cur));
+    return HQRRP_OK;
+  }
   HQ_TRY(hipMemcpyAsync(h->G.p, h->Gcache.p, sizeof(double) * (size_t)r * m,
                         hipMemcpyDeviceToDevice, h->cur));
   return HQRRP_OK;
@@ -522,22 +598,27 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
 
 // Sketch pivots of the block starting at column j (b stages over the n - j
 // remaining sketch columns) into h->piv / h->moves.
-int block_sketch(hqrrp_handle_t h, int r, int j, int n, int b, double* Y) {
+// Element j*r of the sketch (FP64 or FP32 per h->sk32).
+void* sk_at(hqrrp_handle_t h, void* base, long off) {
+  return static_cast<char*>(base) + off * (h->sk32 ? 4 : 8);
+}
+
+int block_sketch(hqrrp_handle_t h, int r, int j, int n, int b, void* Y) {
   PhaseScope ps(h, PH_SKETCH_QRCP, 4.0 * r * (double)(n - j) * b,
-                8.0 * r * (double)(n - j), 1);
-  return run_sketch(h, r, n - j, Y + (long)j * r, r, b, h->piv.as<int>(),
-                    h->moves.as<int>());
+                (h->sk32 ? 4.0 : 8.0) * r * (double)(n - j), 1);
+  return run_sketch(h, r, n - j, sk_at(h, Y, (long)j * r), r, b,
+                    h->piv.as<int>(), h->moves.as<int>(), h->sk32);
 }
 
 // Applies the composed moves of the last sketch to columns j.. of A (all m
 // rows), of Y and of jpvt.
 int block_gather(hqrrp_handle_t h, int r, int j, int m, int b, double* dA,
-                 long lda, double* Y, int* djpvt) {
+                 long lda, void* Y, int* djpvt) {
   PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
   HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b, dA + (long)j * lda, lda,
-                            m, Y + (long)j * r, r, r, djpvt + j,
-                            h->tmpA.as<double>(), h->tmpY.as<double>(),
-                            h->tmpJ.as<int>(), h->cur));
+                            m, sk_at(h, Y, (long)j * r), r, r,
+                            h->sk32 ? 4 : 8, djpvt + j, h->tmpA.as<double>(),
+                            h->tmpY.as<double>(), h->tmpJ.as<int>(), h->cur));
   return HQRRP_OK;
 }
 
@@ -552,12 +633,18 @@ int block_permute(hqrrp_handle_t h, int j, int b, double* dA, long lda,
 
 int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
                    const double* V, long ldv, const double* R12, long ldr,
-                   double* Y, double* G, int nb) {
+                   void* Y, void* G, int nb) {
   PhaseScope ps(h, PH_DOWNDATE,
                 4.0 * r * b * (double)mrows + 2.0 * r * b * nc,
-                8.0 * (2.0 * r * mrows + 2.0 * r * nc), 4);
+                (h->sk32 ? 4.0 : 8.0) * (2.0 * r * mrows + 2.0 * r * nc), 4);
+  if (h->sk32)
+    return downdate_y32(h, r, b, mrows, nc, V, ldv, R12, ldr,
+                        h->T.as<double>(), nb,
+                        static_cast<float*>(sk_at(h, Y, (long)(j + b) * r)), r,
+                        static_cast<float*>(sk_at(h, G, (long)j * r)), r);
   return downdate_y(h, r, b, mrows, nc, V, ldv, R12, ldr, h->T.as<double>(), nb,
-                    Y + (long)(j + b) * r, r, G + (long)j * r, r);
+                    static_cast<double*>(Y) + (long)(j + b) * r, r,
+                    static_cast<double*>(G) + (long)j * r, r);
 }
 
 // The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.
@@ -589,10 +676,13 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   h->cur = h->stream;
   if (mn == 0) return HQRRP_OK;
   const int kmax = (o.max_rank > 0) ? std::min(mn, o.max_rank) : mn;
-  if (o.sketch_prec != HQRRP_SKETCH_FP64) return HQRRP_ERR_UNSUPPORTED;
+  if (o.sketch_prec < HQRRP_SKETCH_FP64 || o.sketch_prec > HQRRP_SKETCH_BF16)
+    return HQRRP_ERR_ARG;
   if (sketch_rmax(r) < 0 || nb > 64) return HQRRP_ERR_UNSUPPORTED;
 
   HQ_TRY(hipSetDevice(h->device));
+  h->sk32 = o.sketch_prec != HQRRP_SKETCH_FP64;
+  h->skbf = o.sketch_prec == HQRRP_SKETCH_BF16;
   StreamScope ss(h);
   return factor_loop(h, m, n, kmax, dA, lda, djpvt, dtau, o);
 }
@@ -617,8 +707,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     HQ_RET(ensure(h->Wsel, sizeof(double) * (size_t)nb * nb));
     HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
   }
-  double* Y = h->Y.as<double>();
-  double* G = h->G.as<double>();
+  void* Y = h->Y.p;  // FP64 or FP32 (h->sk32)
+  void* G = h->G.p;
   double* T = h->T.as<double>();
   hipStream_t main_s = h->cur;
   // Trailing-matrix buffers; the trailing matrix lives in X[xc].
@@ -630,7 +720,12 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   {
     PhaseScope ps(h, PH_SKETCH_GEMM, 2.0 * r * (double)m * n,
                   8.0 * ((double)m * n + (double)r * (m + n)), 1);
-    HQ_RET(gemm(h, false, false, r, n, m, 1.0, G, r, 0, dA, lda, 0, 0.0, Y, r));
+    if (h->sk32)
+      HQ_RET(gemm32(h, false, false, r, n, m, 1.0f, G, r, 0, 0, dA, lda, 1, 0,
+                    0.0f, static_cast<float*>(Y), r, h->skbf));
+    else
+      HQ_RET(gemm(h, false, false, r, n, m, 1.0, static_cast<double*>(G), r, 0,
+                  dA, lda, 0, 0.0, static_cast<double*>(Y), r));
   }
 
   bool have_panel = false;  // block j's panel already factored by look-ahead
@@ -760,7 +855,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
         PhaseScope ps(h, PH_GATHER, 0.0, 0.0, 2);
         HQ_TRY(launch_gather_cols(h->moves.as<int>(), 2 * b2,
                                   X[1] + jn + (long)jn * ldx[1], ldx[1], mnext,
-                                  nullptr, 0, 0, nullptr, h->tmpA.as<double>(),
+                                  nullptr, 0, 0, 8, nullptr, h->tmpA.as<double>(),
                                   h->tmpY.as<double>(), h->tmpJ.as<int>(),
                                   h->cur));
       }

@@ -13,9 +13,11 @@ namespace {
 
 constexpr int kRowsPerBlock = 4 * HQ_NT;
 
+// Y's columns move as r*yw 32-bit words (yw = 1 for float, 2 for double).
 __global__ __launch_bounds__(HQ_NT) void gather_out_kernel(
-    const int* moves, const double* A, long lda, int m, const double* Y,
-    long ldy, int r, const int* jpvt, double* tmpA, double* tmpY, int* tmpJ) {
+    const int* moves, const double* A, long lda, int m, const unsigned* Y,
+    long ldy, int r, const int* jpvt, double* tmpA, unsigned* tmpY,
+    int* tmpJ) {
   const int e = blockIdx.y;
   if (e >= moves[0]) return;
   const long src = moves[2 + 2 * e];
@@ -35,8 +37,8 @@ __global__ __launch_bounds__(HQ_NT) void gather_out_kernel(
 }
 
 __global__ __launch_bounds__(HQ_NT) void gather_in_kernel(
-    const int* moves, double* A, long lda, int m, double* Y, long ldy, int r,
-    int* jpvt, const double* tmpA, const double* tmpY, const int* tmpJ) {
+    const int* moves, double* A, long lda, int m, unsigned* Y, long ldy, int r,
+    int* jpvt, const double* tmpA, const unsigned* tmpY, const int* tmpJ) {
   const int e = blockIdx.y;
   if (e >= moves[0]) return;
   const long dst = moves[1 + 2 * e];
@@ -140,16 +142,20 @@ hipError_t launch_select_cols(const int* moves, int b2, const double* C,
 }
 
 hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
-                              long lda, int m, double* Y, long ldy, int r,
-                              int* jpvt, double* tmpA, double* tmpY, int* tmpJ,
-                              hipStream_t s) {
+                              long lda, int m, void* Y, long ldy, int r,
+                              int ysz, int* jpvt, double* tmpA, double* tmpY,
+                              int* tmpJ, hipStream_t s) {
   if (max_moves <= 0) return hipSuccess;
+  if (r > 0 && ysz != 4 && ysz != 8) return hipErrorInvalidValue;
+  const int yw = ysz / 4;  // 32-bit words per Y element
+  unsigned* Yw = static_cast<unsigned*>(Y);
+  unsigned* tY = reinterpret_cast<unsigned*>(tmpY);
   dim3 grid(m > 0 ? hq_cdiv(m, kRowsPerBlock) : 1, max_moves);
   hipLaunchKernelGGL(gather_out_kernel, grid, dim3(HQ_NT), 0, s, moves, A, lda,
-                     m, Y, ldy, r, jpvt, tmpA, tmpY, tmpJ);
+                     m, Yw, ldy * yw, r * yw, jpvt, tmpA, tY, tmpJ);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(gather_in_kernel, grid, dim3(HQ_NT), 0, s, moves, A, lda,
-                     m, Y, ldy, r, jpvt, tmpA, tmpY, tmpJ);
+                     m, Yw, ldy * yw, r * yw, jpvt, tmpA, tY, tmpJ);
   return hipGetLastError();
 }

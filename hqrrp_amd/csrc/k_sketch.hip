@@ -50,7 +50,10 @@ constexpr int kMaxStages = 80;
 // row k + i.  Every workgroup polls the headers, picks the same winner,
 // builds the reflector from the winner's alpha and norm, and reads the
 // winner's rows.
-template <int RMAX>
+//
+// T is the sketch's arithmetic: double, or float for the mixed-precision
+// sketch (values travel through the exchange as doubles, exactly).
+template <class T, int RMAX>
 __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   constexpr int H = RMAX / 2;
   const int t = threadIdx.x, w = blockIdx.x;
@@ -62,31 +65,32 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   const int ibase = half * H;  // first active row (relative) of this lane
   const long slot = kSlotHead + RMAX;
 
-  __shared__ double s_v[RMAX];  // reflector on the active rows (relative)
+  __shared__ T s_v[RMAX];  // reflector on the active rows (relative)
   __shared__ double s_slot[kSlotHead + RMAX];  // this workgroup's slot
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
   __shared__ int s_win_pos, s_win_wg;
-  __shared__ double s_tau, s_scale;
+  __shared__ T s_tau, s_scale;
   __shared__ int s_piv[kMaxStages];
   __shared__ int s_abort;
 
-  double x[H];
+  const T* Yp = reinterpret_cast<const T*>(a.Y);
+  T x[H];
 #pragma unroll
   for (int q = 0; q < H; ++q) {
     const int i = ibase + q;
-    x[q] = (own && i < r) ? a.Y[(long)c * a.ldy + i] : 0.0;
+    x[q] = (own && i < r) ? Yp[(long)c * a.ldy + i] : T(0);
   }
-  double ss = 0.0;
+  T ss = 0;
 #pragma unroll
   for (int q = 0; q < H; ++q) ss += x[q] * x[q];
   ss += __shfl_xor(ss, 32, 64);
-  double d = sqrt(ss), e = d;
+  T d = sqrt(ss), e = d;
   // Squared norm of the rows below the first active one (the ||x||^2 of
   // this column's reflector, should it win).
-  double sqn;
+  T sqn;
   {
-    double s1 = 0.0;
+    T s1 = 0;
 #pragma unroll
     for (int q = 1; q < H; ++q) s1 += x[q] * x[q];
     s1 += half ? x[0] * x[0] : 0.0;
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    thread combines the 8 wavefront winners identically and the
     //    winning column's lanes publish the slot.
     {
-      double key = (pos >= 0) ? d : -1.0;
+      double key = (pos >= 0) ? (double)d : -1.0;
       int kp = (pos >= 0) ? pos : INT_MAX;
       hq_wave_argmax(key, kp, 32);  // lanes l, l+32 hold the same column
       if (lane == 0) {
@@ -139,8 +143,8 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
           if (half == 0) {
             s_slot[0] = bk;
             s_slot[1] = (double)bp;
-            s_slot[2] = x[0];
-            s_slot[3] = sqn;
+            s_slot[2] = (double)x[0];
+            s_slot[3] = (double)sqn;
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       ba = __shfl(ba, bl, 64);
       bs = __shfl(bs, bl, 64);
       if (lane == 0) {
-        hq_reflector h = hq_dlarfg(ba, bs);
+        const hq_reflector_t<T> h = hq_larfg<T>((T)ba, (T)bs);
         s_tau = h.tau;
         s_scale = h.scale;
         s_win_pos = bp;
@@ -197,14 +201,14 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     // 3. the winner's rows below k, scaled, as the reflector (relative rows,
     //    1 at row k, zero beyond span).
     if (t < RMAX) {
-      double v = 0.0;
+      T v = 0;
       if (t > 0 && t < span) {
         double cv[1];
         const double* src = cbuf + (long)s_win_wg * slot + kSlotHead + t;
         ok = hq_await(cv, [&](int) -> const double* { return src; }, a.err);
-        v = cv[0] * s_scale;
+        v = (T)cv[0] * s_scale;
       }
-      s_v[t] = (t == 0) ? 1.0 : v;
+      s_v[t] = (t == 0) ? T(1) : v;
       if (!ok) s_abort = 1;
     }
     __syncthreads();
@@ -221,34 +225,34 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    active rows up by one (row k leaves).  Both lanes of a column take
     //    the same branches (pos is per column).
     if (pos >= 0) {
-      const double tau = s_tau;
-      if (tau != 0.0) {
-        double wd = 0.0;
+      const T tau = s_tau;
+      if (tau != T(0)) {
+        T wd = 0;
 #pragma unroll
         for (int q = 0; q < H; ++q) wd += s_v[ibase + q] * x[q];
         wd += __shfl_xor(wd, 32, 64);
-        const double f = -tau * wd;
+        const T f = -tau * wd;
 #pragma unroll
         for (int q = 0; q < H; ++q) x[q] += f * s_v[ibase + q];
       }
       // b2: rows below k (Drmac's recompute); s3: rows below k+1 (the next
       // step's ||x||^2).
-      double b2 = 0.0, s3 = 0.0;
+      T b2 = 0, s3 = 0;
 #pragma unroll
       for (int q = 1; q < H; ++q) b2 += x[q] * x[q];
 #pragma unroll
       for (int q = 2; q < H; ++q) s3 += x[q] * x[q];
-      const double x0 = x[0];
-      const double up = __shfl_xor(x0, 32, 64);  // partner's first row
-      b2 += half ? x0 * x0 : 0.0;
-      s3 += half ? x[1] * x[1] + x0 * x0 : 0.0;
+      const T x0 = x[0];
+      const T up = __shfl_xor(x0, 32, 64);  // partner's first row
+      b2 += half ? x0 * x0 : T(0);
+      s3 += half ? x[1] * x[1] + x0 * x0 : T(0);
       b2 += __shfl_xor(b2, 32, 64);
       sqn = s3 + __shfl_xor(s3, 32, 64);
-      const double wt = half ? up : x0;  // row k
-      hq_downdate_norm(d, e, wt, b2, r - k - 1);
+      const T wt = half ? up : x0;  // row k
+      hq_downdate_norm<T>(d, e, wt, b2, r - k - 1);
 #pragma unroll
       for (int q = 0; q < H - 1; ++q) x[q] = x[q + 1];
-      x[H - 1] = half ? 0.0 : up;
+      x[H - 1] = half ? T(0) : up;
     }
   }
   // Composed permutation (workgroup 0, wavefront 0): replay the swaps of
@@ -330,11 +334,23 @@ hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax,
   if (a.nw < 1 || a.nw > kMaxWg || a.stages < 1 || a.stages > kMaxStages)
     return hipErrorInvalidValue;
   dim3 grid(a.nw), block(kThreads);
-  if (rmax == 32)
-    hipLaunchKernelGGL(sketch_qrcp_kernel<32>, grid, block, 0, stream, a);
-  else if (rmax == 80)
-    hipLaunchKernelGGL(sketch_qrcp_kernel<80>, grid, block, 0, stream, a);
-  else
+  if (a.f32) {
+    if (rmax == 32)
+      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 32>), grid, block, 0,
+                         stream, a);
+    else if (rmax == 80)
+      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 80>), grid, block, 0,
+                         stream, a);
+    else
+      return hipErrorInvalidValue;
+  } else if (rmax == 32) {
+    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 32>), grid, block, 0,
+                       stream, a);
+  } else if (rmax == 80) {
+    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 80>), grid, block, 0,
+                       stream, a);
+  } else {
     return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }

@@ -5,8 +5,9 @@
 #include <stddef.h>
 
 struct SketchArgs {
-  const double* Y;  // sketch columns to pivot: r x ncols, ld ldy
+  const void* Y;  // sketch columns to pivot: r x ncols, ld ldy (elements)
   long ldy;
+  int f32;        // Y holds floats (the pivoting then runs in float)
   int r, ncols, stages, nw;
   int* piv;    // stages: position swapped with position k at step k
   int* moves;  // [0] = count, then (dst, src) position pairs
@@ -105,11 +106,12 @@ hipError_t launch_select_cols(const int* moves, int b2, const double* C,
                               long ldw, int b, double* P, long ldp,
                               double* Wsel, hipStream_t s);
 // Moves m rows of A's columns (A may point at any row), r rows of Y's
-// columns (r = 0: none) and jpvt (null: none).
+// columns (r = 0: none; Y's elements are ysz = 4 or 8 bytes, ldy in
+// elements) and jpvt (null: none).
 hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
-                              long lda, int m, double* Y, long ldy, int r,
-                              int* jpvt, double* tmpA, double* tmpY, int* tmpJ,
-                              hipStream_t s);
+                              long lda, int m, void* Y, long ldy, int r,
+                              int ysz, int* jpvt, double* tmpA, double* tmpY,
+                              int* tmpJ, hipStream_t s);
 
 // k_adj.hip
 hipError_t launch_fill_eye(double* Q, long ldq, int m, int n, double diag,
@@ -125,3 +127,27 @@ hipError_t launch_copyltu(int k, const double* M, long ldm, double* B, long ldb,
 hipError_t launch_permute_cols(const double* src, long lds, int m, int ncols,
                                const int* p, int scatter, double* dst,
                                long ldd, hipStream_t s);
+
+// k_gemm32.hip
+struct Gemm32Args {
+  int M, N, K;
+  float alpha, beta;
+  const void* A;  // FP64 (a64 = 1) or FP32 elements
+  long lda;
+  int a64;
+  const void* B;
+  long ldb;
+  int b64;
+  float* C;
+  long ldc;
+  int maskA, maskB;  // read the stored operand as unit lower trapezoidal
+  int bf16;          // round both operands to bfloat16 when staged
+  long kchunk;       // K range of one split (blockIdx.z)
+  float* part;       // split-K output slices (M x N each), or null
+};
+hipError_t launch_gemm32(const Gemm32Args& g, bool ta, bool tb, int splits,
+                         hipStream_t s);
+hipError_t launch_reduce32(int M, int N, int S, const float* part, float alpha,
+                           float beta, float* C, long ldc, hipStream_t s);
+hipError_t launch_cast_f32(const double* src, float* dst, long n,
+                           hipStream_t s);

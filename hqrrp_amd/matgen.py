@@ -73,3 +73,21 @@ def gaussian_device(m: int, n: int, seed: int = 0, device: str = "cuda"):
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     return torch.randn(n, m, generator=g, device=device, dtype=torch.float64)
+
+
+def lowrank_device(m: int, n: int, rank: int = 512, seed: int = 0,
+                   noise: float = 1e-8, device: str = "cuda"):
+    """Same family as :func:`lowrank_noise` (rank `rank`, sigma_i = 1/i,
+    plus `noise` N(0,1)), built with torch on the GPU; returned column-major
+    as a (n, m) tensor like :func:`gaussian_device`."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = torch.float64
+    L = torch.randn(m, rank, generator=g, device=device, dtype=f64) / m ** 0.5
+    R = torch.randn(n, rank, generator=g, device=device, dtype=f64) / n ** 0.5
+    sig = 1.0 / torch.arange(1, rank + 1, device=device, dtype=f64)
+    At = (R * sig) @ L.t()  # (n, m): A' = R diag(sig) L'
+    At += noise * torch.randn(n, m, generator=g, device=device, dtype=f64)
+    return At
