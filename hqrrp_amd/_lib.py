@@ -70,6 +70,11 @@ SIGNATURES = {
     "hqrrp_qr_fixed_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
                                      C.c_int, _vp, _vp, C.c_int, _vp,
                                      C.c_int]),
+    "hqrrp_batch_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
+                                     C.c_int, C.POINTER(_vp)]),
+    "hqrrp_batch_get_nb": (C.c_int, [_vp, _vp]),
+    "hqrrp_batch_factor_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "hqrrp_batch_destroy": (C.c_int, [_vp]),
     "hqrrp_qr_fixed_adjoint_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int,
                                              _vp, C.c_int, _vp, _vp, C.c_int,
                                              _vp, C.c_int, _vp, C.c_int, _vp,

@@ -291,3 +291,139 @@ def fixed_pivot_qr(A, jpvt, k: int):
     """Differentiable (Q, R) of A[:, jpvt-1] truncated to k (see
     hqrrp_qr_fixed_dev); gradients by hqrrp_qr_fixed_adjoint_dev."""
     return _fixed_qr_function().apply(A, jpvt, k)
+
+
+# ------------------------------------------------------------ batched path
+class BatchPlan:
+    """A prepared batch of small matrices (m, n <= 512) for the batched
+    persistent launch: shapes, per-matrix block size (nb: None = chosen per
+    shape, an int for all, or one per matrix), oversampling pp, queue order
+    and workspace.  See hqrrp_batch_create in include/hqrrp_amd.h."""
+
+    def __init__(self, handle: Handle, m, n, nb=None, pp: int = 8,
+                 panel_pivoting: int = 1, lda=None):
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        if m.shape != n.shape:
+            raise ValueError("m and n must have the same length")
+        self.count = int(m.size)
+        self.m, self.n = m, n
+        nb_a = None
+        if nb is not None:
+            nb_a = np.ascontiguousarray(
+                np.full(self.count, nb) if np.isscalar(nb) else nb,
+                dtype=np.int32)
+        lda_a = None if lda is None else np.ascontiguousarray(lda, np.int32)
+        self._p = C.c_void_p()
+        self._keep = (m, n, nb_a, lda_a)
+        check(lib().hqrrp_batch_create(
+            handle.ptr, self.count, _ptr(m), _ptr(n),
+            None if lda_a is None else _ptr(lda_a),
+            None if nb_a is None else _ptr(nb_a), pp, panel_pivoting,
+            C.byref(self._p)), "hqrrp_batch_create")
+        self.pp, self.panel_pivoting = pp, panel_pivoting
+
+    @property
+    def nb(self) -> np.ndarray:
+        out = np.zeros(max(1, self.count), dtype=np.int32)
+        check(lib().hqrrp_batch_get_nb(self._p, _ptr(out)),
+              "hqrrp_batch_get_nb")
+        return out[:self.count]
+
+    def factor(self, handle: Handle, A_ptrs, jpvt_ptrs, tau_ptrs) -> None:
+        """Factor in place; the arguments are int64 torch tensors on the
+        device holding the matrices', pivot vectors' and tau vectors'
+        device addresses."""
+        check(lib().hqrrp_batch_factor_dev(handle.ptr, self._p,
+                                           _vp(A_ptrs), _vp(jpvt_ptrs),
+                                           _vp(tau_ptrs)),
+              "hqrrp_batch_factor_dev")
+
+    def close(self) -> None:
+        if self._p:
+            lib().hqrrp_batch_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PackedBatch:
+    """A batch packed in three device buffers (matrices column-major with
+    ld = m, pivots initialised to 1..n, tau) and the device pointer arrays
+    BatchPlan.factor takes."""
+
+    def __init__(self, m, n, device="cuda"):
+        import torch
+
+        self.m = np.asarray(m, dtype=np.int64)
+        self.n = np.asarray(n, dtype=np.int64)
+        sizes = self.m * self.n
+        self.a_off = np.concatenate([[0], np.cumsum(sizes)])
+        self.j_off = np.concatenate([[0], np.cumsum(self.n)])
+        k = np.minimum(self.m, self.n)
+        self.t_off = np.concatenate([[0], np.cumsum(np.maximum(k, 1))])
+        self.A = torch.zeros(int(max(1, self.a_off[-1])), dtype=torch.float64,
+                             device=device)
+        self.jpvt = torch.zeros(int(max(1, self.j_off[-1])), dtype=torch.int32,
+                                device=device)
+        self.tau = torch.zeros(int(max(1, self.t_off[-1])),
+                               dtype=torch.float64, device=device)
+        self.reset_pivots()
+        base = lambda t, off, es: torch.as_tensor(  # noqa: E731
+            t.data_ptr() + off[:-1] * es, dtype=torch.int64).to(device)
+        self.A_ptrs = base(self.A, self.a_off, 8)
+        self.jpvt_ptrs = base(self.jpvt, self.j_off, 4)
+        self.tau_ptrs = base(self.tau, self.t_off, 8)
+
+    def reset_pivots(self) -> None:
+        import torch
+
+        host = np.concatenate([np.arange(1, int(c) + 1, dtype=np.int32)
+                               for c in self.n]) if self.n.size else \
+            np.zeros(0, np.int32)
+        if host.size:
+            self.jpvt.copy_(torch.as_tensor(host))
+
+    def load(self, mats) -> None:
+        """Copy host matrices (numpy, shape m_i x n_i) into the buffer."""
+        import torch
+
+        flat = np.concatenate([np.asarray(a, dtype=np.float64).ravel(order="F")
+                               for a in mats]) if mats else np.zeros(0)
+        if flat.size:
+            self.A[:flat.size].copy_(torch.as_tensor(flat))
+
+    def unpack(self):
+        """[(A_i factored, jpvt_i, tau_i)] as numpy arrays."""
+        A = self.A.cpu().numpy()
+        J = self.jpvt.cpu().numpy()
+        T = self.tau.cpu().numpy()
+        out = []
+        for i in range(self.m.size):
+            m, n = int(self.m[i]), int(self.n[i])
+            a = A[self.a_off[i]:self.a_off[i + 1]].reshape((m, n), order="F")
+            out.append((np.asfortranarray(a),
+                        J[self.j_off[i]:self.j_off[i + 1]].copy(),
+                        T[self.t_off[i]:self.t_off[i] + min(m, n)].copy()))
+        return out
+
+
+def hqrrp_batched(mats, nb=None, pp: int = 8, panel_pivoting: int = 1,
+                  handle: Handle | None = None):
+    """NoFLA_HQRRP_WY_blk_var4 on copies of many small host matrices in one
+    batched launch; returns [(A, jpvt, tau)] like :func:`hqrrp`."""
+    h = handle if handle is not None else _default_handle(0)
+    m = [np.shape(a)[0] for a in mats]
+    n = [np.shape(a)[1] for a in mats]
+    pk = PackedBatch(m, n)
+    pk.load(mats)
+    plan = BatchPlan(h, m, n, nb=nb, pp=pp, panel_pivoting=panel_pivoting)
+    plan.factor(h, pk.A_ptrs, pk.jpvt_ptrs, pk.tau_ptrs)
+    h.synchronize()
+    out = pk.unpack()
+    plan.close()
+    return out

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -178,6 +179,10 @@ struct hqrrp_handle_s {
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
+  // Batched path: block size per 64 x 64 shape bucket chosen by
+  // hqrrp_batch_tune for (bt_pp, bt_piv); 0 = not tuned.
+  int bt_nb[8][8] = {};
+  int bt_pp = -1, bt_piv = -1;
   std::mutex mu;
   // Optional per-phase timing with HIP events recorded on `stream`.
   bool prof = false;
@@ -1295,6 +1300,142 @@ extern "C" int hqrrp_qr_fixed_adjoint_dev(
   StreamScope ss(h);
   return run_qr_fixed_adjoint(h, m, n, k, dA, lda, djpvt, dQ, ldq, dR, ldr, gQ,
                               ldgq, gR, ldgr, gA, ldga);
+}
+
+// ------------------------------------------------------------ batched path
+
+// A prepared batch of small matrices (shapes, block sizes, queue order,
+// shared Omega, per-workgroup workspace).  Factorisations reuse it.
+struct hqrrp_batch_s {
+  int device = 0;
+  int count = 0, pp = 8, pivoting = 1, wgs = 0;
+  std::vector<int> m, n, lda, nb;
+  DevBuf items, omega, work, counter;
+};
+
+namespace {
+
+// Block size of the batched path for an m x n matrix when the caller gives
+// none: the handle's tuned table (hqrrp_batch_tune) if it holds an entry for
+// the shape's 64 x 64 bucket, else 16 below 64 rows or columns and 32 above.
+int batch_pick_nb(hqrrp_handle_t h, int pp, int pivoting, int m, int n) {
+  const int bm = std::max(0, std::min(7, (m - 1) / 64));
+  const int bn = std::max(0, std::min(7, (n - 1) / 64));
+  if (h->bt_pp == pp && h->bt_piv == pivoting && h->bt_nb[bm][bn] > 0)
+    return h->bt_nb[bm][bn];
+  return std::min(m, n) < 64 ? 16 : 32;
+}
+
+// Relative cost of one matrix for the queue order (largest first): the
+// factorisation's flops plus a latency term per pivot stage.
+double batch_cost(int m, int n) {
+  const double a = std::max(m, n), b = std::min(m, n);
+  return 2.0 * a * b * b - 2.0 / 3.0 * b * b * b + 4.0e5 * b;
+}
+
+}  // namespace
+
+extern "C" int hqrrp_batch_create(hqrrp_handle_t h, int count, const int* m,
+                                  const int* n, const int* lda, const int* nb,
+                                  int pp, int panel_pivoting,
+                                  hqrrp_batch_t* out) {
+  if (!h || !out || count < 0 || pp < 0 || pp > HQB_PP_MAX) return HQRRP_ERR_ARG;
+  if (count > 0 && (!m || !n)) return HQRRP_ERR_ARG;
+  *out = nullptr;
+  hqrrp_batch_t b = new (std::nothrow) hqrrp_batch_s();
+  if (!b) return HQRRP_ERR_ALLOC;
+  std::unique_ptr<hqrrp_batch_s> guard(b);
+  b->device = h->device;
+  b->count = count;
+  b->pp = pp;
+  b->pivoting = panel_pivoting ? 1 : 0;
+  b->m.assign(m, m + count);
+  b->n.assign(n, n + count);
+  b->lda.resize(count);
+  b->nb.resize(count);
+  long omega_len = 1;
+  for (int i = 0; i < count; ++i) {
+    const int mi = m[i], ni = n[i];
+    if (mi < 0 || ni < 0 || mi > HQB_DIM_MAX || ni > HQB_DIM_MAX)
+      return HQRRP_ERR_UNSUPPORTED;
+    b->lda[i] = lda ? lda[i] : std::max(1, mi);
+    if (b->lda[i] < std::max(1, mi)) return HQRRP_ERR_ARG;
+    b->nb[i] = nb ? nb[i] : batch_pick_nb(h, pp, b->pivoting, mi, ni);
+    if (b->nb[i] < 1 || b->nb[i] > 64) return HQRRP_ERR_UNSUPPORTED;
+    omega_len = std::max(omega_len, (long)(b->nb[i] + pp) * std::max(1, mi));
+  }
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return batch_cost(b->m[x], b->n[x]) > batch_cost(b->m[y], b->n[y]);
+  });
+  std::vector<BatchItem> items(std::max(1, count));
+  for (int q = 0; q < count; ++q) {
+    const int i = order[q];
+    BatchItem& it = items[q];
+    memset(&it, 0, sizeof(it));
+    it.m = b->m[i];
+    it.n = b->n[i];
+    it.lda = b->lda[i];
+    it.nb = b->nb[i];
+    it.idx = i;
+  }
+  HQ_TRY(hipSetDevice(h->device));
+  HQ_RET(ensure(b->items, sizeof(BatchItem) * items.size()));
+  HQ_TRY(hipMemcpy(b->items.p, items.data(), sizeof(BatchItem) * items.size(),
+                   hipMemcpyHostToDevice));
+  // Omega of an r x m matrix is the first r*m normals of the reference's
+  // sequence (seed 12) in column order: one sequence serves every shape.
+  std::vector<double> om(omega_len);
+  hqrrp_reference_omega(1, (int)omega_len, om.data(), 12u);
+  HQ_RET(ensure(b->omega, sizeof(double) * om.size()));
+  HQ_TRY(hipMemcpy(b->omega.p, om.data(), sizeof(double) * om.size(),
+                   hipMemcpyHostToDevice));
+  b->wgs = std::max(1, std::min(count, h->cu_count));
+  HQ_RET(ensure(b->work, sizeof(double) * (size_t)batch_work_doubles() * b->wgs));
+  HQ_TRY(hipMemset(b->work.p, 0, b->work.bytes));
+  HQ_RET(ensure(b->counter, 256));
+  HQ_TRY(hipMemset(b->counter.p, 0, b->counter.bytes));
+  *out = guard.release();
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_batch_get_nb(hqrrp_batch_t b, int* nb) {
+  if (!b || (!nb && b->count > 0)) return HQRRP_ERR_ARG;
+  for (int i = 0; i < b->count; ++i) nb[i] = b->nb[i];
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_batch_factor_dev(hqrrp_handle_t h, hqrrp_batch_t b,
+                                      double* const* dA, int* const* djpvt,
+                                      double* const* dtau) {
+  if (!h || !b || b->device != h->device) return HQRRP_ERR_ARG;
+  if (b->count == 0) return HQRRP_OK;
+  if (!dA || !djpvt || !dtau) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  BatchArgs a{};
+  a.items = b->items.as<BatchItem>();
+  a.count = b->count;
+  a.pp = b->pp;
+  a.pivoting = b->pivoting;
+  a.A = dA;
+  a.jpvt = djpvt;
+  a.tau = dtau;
+  a.omega = b->omega.as<double>();
+  a.work = b->work.as<double>();
+  a.counter = b->counter.as<int>();
+  HQ_TRY(launch_batch_hqrrp(a, b->wgs, h->cur));
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_batch_destroy(hqrrp_batch_t b) {
+  if (!b) return HQRRP_OK;
+  (void)hipSetDevice(b->device);
+  for (DevBuf* d : {&b->items, &b->omega, &b->work, &b->counter})
+    if (d->p) (void)hipFree(d->p);
+  delete b;
+  return HQRRP_OK;
 }
 
 // --------------------------------------------------- drop-in host entries

@@ -68,6 +68,29 @@ struct TrailArgs {
   long kchunk;   // rows of C per split
 };
 
+// k_batch.hip: batched HQRRP of small matrices (m, n <= HQB_DIM_MAX;
+// nb <= 64; pp <= HQB_PP_MAX).
+#define HQB_PP_MAX 10
+#define HQB_DIM_MAX 512
+struct BatchItem {
+  int m, n, lda, nb;
+  int idx;  // index into the pointer arrays
+  int pad[3];
+};
+struct BatchArgs {
+  const BatchItem* items;  // queue order (largest first)
+  int count;
+  int pp, pivoting;
+  double* const* A;  // per-matrix device pointers, indexed by BatchItem::idx
+  int* const* jpvt;
+  double* const* tau;
+  const double* omega;  // normals: Omega of r x m is omega[0, r*m), ld r
+  double* work;         // batch_work_doubles() per workgroup
+  int* counter;         // queue head, reset by launch_batch_hqrrp
+};
+long batch_work_doubles();
+hipError_t launch_batch_hqrrp(const BatchArgs& a, int wgs, hipStream_t s);
+
 // k_trail.hip
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);

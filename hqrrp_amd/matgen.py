@@ -91,3 +91,41 @@ def lowrank_device(m: int, n: int, rank: int = 512, seed: int = 0,
     At = (R * sig) @ L.t()  # (n, m): A' = R diag(sig) L'
     At += noise * torch.randn(n, m, generator=g, device=device, dtype=f64)
     return At
+
+
+# ---------------------------------------------------------------- batches
+def batch_shapes(count: int, seed: int = 0, m_range=(64, 512),
+                 n_min: int = 32):
+    """Shapes of the batched workload: m ~ U[m_range], n ~ U[n_min, m]."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(m_range[0], m_range[1] + 1, size=count)
+    u = rng.random(count)
+    n = n_min + np.floor(u * (m - n_min + 1)).astype(np.int64)
+    return m.astype(np.int32), np.minimum(n, m).astype(np.int32)
+
+
+def _batch_one(args):
+    m, n, seed, decades = args
+    return graded(int(m), int(n), int(seed), decades)
+
+
+def _one_thread():
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:
+        pass
+
+
+def batch_matrices(m, n, seed: int = 0, decades: float = 8.0,
+                   workers: int = 1):
+    """Matrix i of a batch: N(0,1)-distributed singular vectors (Haar U, V)
+    with singular values rescaled to 10^(-decades*i/k), seeded by (seed, i).
+    `workers` > 1 generates them in a process pool."""
+    jobs = [(m[i], n[i], seed * 1000003 + i, decades) for i in range(len(m))]
+    if workers <= 1 or len(jobs) < 64:
+        return [_batch_one(j) for j in jobs]
+    import multiprocessing as mp
+
+    with mp.get_context("fork").Pool(workers, initializer=_one_thread) as p:
+        return p.map(_batch_one, jobs, chunksize=32)

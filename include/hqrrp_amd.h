@@ -166,6 +166,36 @@ int hqrrp_qr_fixed_adjoint_dev(hqrrp_handle_t h, int m, int n, int k,
                                const double *gR, int ldgr, double *gA,
                                int ldga);
 
+/* ------------------------------------------- batched small matrices */
+
+/* Many independent small QRCPs in one persistent launch (no counterpart in
+ * the reference, whose NoFLA_HQRRP_WY_blk_var4 factors one matrix per call,
+ * NoFLA_HQRRP_WY_blk_var4.c:303-571).  Each matrix i (m[i] x n[i], ld
+ * lda[i], m, n <= 512) is factored exactly as NoFLA_HQRRP_WY_blk_var4(m[i],
+ * n[i], A_i, lda[i], jpvt_i, tau_i, nb[i], pp, panel_pivoting) would: Omega
+ * from the reference's generator with seed 12, jpvt_i permuted in place
+ * (callers pass 1..n[i] for the reference's usual call), tau_i of length
+ * min(m[i], n[i]).
+ *
+ * A plan holds the shapes, the block sizes (nb == NULL: chosen per shape,
+ * from the table of hqrrp_batch_tune when it has run on this handle), the
+ * queue order and a workspace; nb[i] in 1..64, pp in 0..10.  Returns
+ * HQRRP_ERR_UNSUPPORTED for shapes or block sizes outside that range. */
+typedef struct hqrrp_batch_s *hqrrp_batch_t;
+int hqrrp_batch_create(hqrrp_handle_t h, int count, const int *m, const int *n,
+                       const int *lda, const int *nb, int pp,
+                       int panel_pivoting, hqrrp_batch_t *plan);
+/* The block size used for each matrix (count ints). */
+int hqrrp_batch_get_nb(hqrrp_batch_t plan, int *nb);
+/* Factors the plan's matrices: dA, djpvt, dtau are DEVICE arrays of count
+ * device pointers (matrix i at dA[i], ...).  Stream-ordered on the handle's
+ * stream, one launch, no allocation and no host synchronisation (so it can
+ * be captured in a HIP graph).  Calls sharing a plan must not overlap. */
+int hqrrp_batch_factor_dev(hqrrp_handle_t h, hqrrp_batch_t plan,
+                           double *const *dA, int *const *djpvt,
+                           double *const *dtau);
+int hqrrp_batch_destroy(hqrrp_batch_t plan);
+
 /* ------------------------------------- component entry points (device) */
 
 /* Greedy QRCP of the sketch for `stages` steps, choosing pivots only
