@@ -9,6 +9,12 @@ the device-to-device copy that restores A is inside the timed step.
 
 metric: QRCP GFLOP/s with the flop count 2mn^2 - 2/3 n^3 (m >= n).
 
+--workload batched: 8192 independent matrices (m ~ U[64,512], n ~ U[32,m],
+singular values 10^(-8 i/n) with Haar singular vectors) factored in one
+persistent launch; a step restores the packed inputs (device copy) and
+factors all of them; metric: matrices/s, with the summed flops of the
+realised shapes beside it.
+
 Also reported:
   phase_ms      -- per-phase kernel time of one extra, untimed step in which
                    every phase is bracketed by HIP events;
@@ -86,6 +92,148 @@ def cpu_baseline(A_host, nb, pp, blocks, threads):
                        f"matrix: {dt:.2f} s for {fl:.3e} flops")}
 
 
+def _oracle_slice(job):
+    import oracle
+
+    oracle.set_threads(1)
+    mats, nbs, pp = job
+    for A, nb in zip(mats, nbs):
+        oracle.hqrrp(A, int(nb), pp, 1)
+    return len(mats)
+
+
+def cpu_baseline_batched(mats, nbs, pp, workers, budget_s=15.0):
+    """The oracle on the host cores (one OpenBLAS thread per worker process)
+    over a sample of the batch sized to about budget_s seconds."""
+    import multiprocessing as mp
+
+    import oracle
+
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    probe = 0
+    while probe < len(mats) and time.perf_counter() - t0 < 1.0:
+        oracle.hqrrp(mats[probe], int(nbs[probe]), pp, 1)
+        probe += 1
+    per = (time.perf_counter() - t0) / max(1, probe)
+    count = int(min(len(mats), max(workers, budget_s * workers / per)))
+    chunks = [(mats[i:count:workers], nbs[i:count:workers], pp)
+              for i in range(workers)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        done = sum(pool.map(_oracle_slice, chunks))
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 1), "unit": "matrices/s",
+            "cores": workers, "kind": "port",
+            "sample": (f"oracle restatement of NoFLA_HQRRP_WY_blk_var4 "
+                       f"(OpenBLAS, 1 thread per process, {workers} "
+                       f"processes) on the first {done} matrices of the "
+                       f"batch with the same nb and pp: {dt:.2f} s")}
+
+
+def bench_batched(args):
+    import numpy as np
+    import torch
+
+    import hqrrp_amd
+    import oracle
+    from hqrrp_amd import matgen
+
+    count, pp = args.count, 8
+    workers = min(16, os.cpu_count() or 1)
+    print(f"# generating {count} matrices ({workers} processes)",
+          file=sys.stderr, flush=True)
+    m, n = matgen.batch_shapes(count, seed=0)
+    mats = matgen.batch_matrices(m, n, seed=0, workers=workers)
+    h = hqrrp_amd.Handle(0)
+    h.use_stream(torch.cuda.current_stream())
+    plan = hqrrp_amd.BatchPlan(h, m, n, nb=args.batch_nb, pp=pp)
+    nbs = plan.nb
+    src = hqrrp_amd.PackedBatch(m, n)
+    src.load(mats)
+    work = hqrrp_amd.PackedBatch(m, n)
+    ident = src.jpvt.clone()
+
+    def step():
+        work.A.copy_(src.A)
+        work.jpvt.copy_(ident)
+        plan.factor(h, work.A_ptrs, work.jpvt_ptrs, work.tau_ptrs)
+
+    for i in range(args.warmup):
+        step()
+        h.synchronize()
+        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+    # one untimed step with the kernel's per-phase cycle counts
+    phase_share = {}
+    if not args.no_profile:
+        plan.profile(True)
+        step()
+        h.synchronize()
+        cyc = plan.profile_read()
+        plan.profile(False)
+        tot = sum(cyc.values()) or 1.0
+        phase_share = {k: round(v / tot, 4) for k, v in cyc.items()}
+        print(f"# phase shares {phase_share}", file=sys.stderr, flush=True)
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        work.A.copy_(src.A)
+        work.jpvt.copy_(ident)
+        ev[i][0].record()
+        plan.factor(h, work.A_ptrs, work.jpvt_ptrs, work.tau_ptrs)
+        ev[i][1].record()
+    h.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    kern_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    print(f"# timed {args.steps} steps: {ms:.2f} ms/step, kernel "
+          f"{kern_ms:.2f} ms", file=sys.stderr, flush=True)
+    flops = float(sum(qr_flops(int(a), int(b)) for a, b in zip(m, n)))
+    # backward error of a sample of the factorisations
+    out = work.unpack()
+    worst = 0.0
+    for i in range(0, count, max(1, count // 64)):
+        res, orth = oracle.residuals(mats[i], *out[i])
+        worst = max(worst, res / (max(m[i], n[i]) * np.finfo(float).eps),
+                    orth / (max(m[i], n[i]) * np.finfo(float).eps))
+    ach = flops / (kern_ms / 1e3)
+    roof = {"bound": "mfma", "kernel": "batch_hqrrp_kernel",
+            "achieved": round(ach / 1e12, 3), "peak": PEAK_FP64_MFMA / 1e12,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_MFMA, 4),
+            "traffic": None, "launches": args.steps,
+            "avg_launch_us": round(kern_ms * 1e3, 1),
+            "algorithmic_flops_per_launch": flops}
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_batched(mats, nbs, pp, workers)
+    plan.close()
+    h.close()
+    line = {
+        "metric": "batched QRCP/s (matrices per second)",
+        "value": round(count / (ms / 1e3), 1), "unit": "matrices/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"batched HQRRP {count} matrices, m~U[64,512], "
+                                f"n~U[32,m], sigma_i=10^(-8i/n)"),
+                   "count": count, "pp": pp, "panel_pivoting": 1,
+                   "nb_counts": {int(k): int(v) for k, v in
+                                 zip(*np.unique(nbs, return_counts=True))},
+                   "omega": "reference stream seed 12"},
+        "gflops_realised": round(flops / (ms / 1e3) / 1e9, 1),
+        "kernel_ms": round(kern_ms, 3),
+        "max_backward_error_over_n_eps": round(worst, 3),
+        "phase_cycle_share": phase_share,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +259,15 @@ def main():
                     default="fp64",
                     help="sketch precision (panel, T and trailing update are "
                          "FP64 in every case)")
+    ap.add_argument("--workload", choices=("single", "batched"),
+                    default="single",
+                    help="single: one large HQRRP (the headline); batched: "
+                         "many small matrices in one launch")
+    ap.add_argument("--count", type=int, default=8192,
+                    help="matrices of the batched workload")
+    ap.add_argument("--batch-nb", type=int, default=None,
+                    help="block size of every batched matrix (default: "
+                         "chosen per shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel (read MIB MiB, write "
                          "MIB MiB) three times: the HBM-counter calibration "
@@ -134,6 +291,10 @@ def main():
               "build (multi-GPU sharding is a design note only); run with "
               "--gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import numpy as np
     import torch

@@ -339,6 +339,21 @@ class BatchPlan:
                                            _vp(tau_ptrs)),
               "hqrrp_batch_factor_dev")
 
+    def profile(self, on: bool = True) -> None:
+        """Start (and reset) or stop the kernel's per-phase cycle counts."""
+        check(lib().hqrrp_batch_profile(self._p, 1 if on else 0),
+              "hqrrp_batch_profile")
+
+    def profile_read(self) -> dict:
+        """{phase: shader-clock cycles summed over workgroups}."""
+        L = lib()
+        k = L.hqrrp_batch_phase_count()
+        out = (C.c_double * k)()
+        check(L.hqrrp_batch_profile_read(self._p, k, out),
+              "hqrrp_batch_profile_read")
+        return {L.hqrrp_batch_phase_name(i).decode(): out[i]
+                for i in range(k)}
+
     def close(self) -> None:
         if self._p:
             lib().hqrrp_batch_destroy(self._p)

@@ -1308,9 +1308,15 @@ extern "C" int hqrrp_qr_fixed_adjoint_dev(
 // shared Omega, per-workgroup workspace).  Factorisations reuse it.
 struct hqrrp_batch_s {
   int device = 0;
-  int count = 0, pp = 8, pivoting = 1, wgs = 0;
+  int count = 0, pp = 8, pivoting = 1;
+  // Items are grouped by kernel variant (block sizes up to 16, 32, 64):
+  // class c holds items [first[c], first[c] + cnt[c]), launched with wgs[c]
+  // workgroups on queue counter c.
+  int first[3] = {0, 0, 0}, cnt[3] = {0, 0, 0}, wgs[3] = {0, 0, 0};
   std::vector<int> m, n, lda, nb;
   DevBuf items, omega, work, counter;
+  DevBuf prof;  // per-phase cycle sums when profiling is on
+  bool prof_on = false;
 };
 
 namespace {
@@ -1364,11 +1370,24 @@ extern "C" int hqrrp_batch_create(hqrrp_handle_t h, int count, const int* m,
     if (b->nb[i] < 1 || b->nb[i] > 64) return HQRRP_ERR_UNSUPPORTED;
     omega_len = std::max(omega_len, (long)(b->nb[i] + pp) * std::max(1, mi));
   }
+  auto cls = [&](int i) {
+    const int c = batch_class_nb(b->nb[i]);
+    return c == 16 ? 0 : (c == 32 ? 1 : 2);
+  };
   std::vector<int> order(count);
   for (int i = 0; i < count; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (cls(x) != cls(y)) return cls(x) < cls(y);
     return batch_cost(b->m[x], b->n[x]) > batch_cost(b->m[y], b->n[y]);
   });
+  for (int q = 0; q < count; ++q) ++b->cnt[cls(order[q])];
+  b->first[1] = b->cnt[0];
+  b->first[2] = b->cnt[0] + b->cnt[1];
+  int max_wgs = 1;
+  for (int c = 0; c < 3; ++c) {
+    b->wgs[c] = std::min(b->cnt[c], batch_wgs_per_cu(16 << c) * h->cu_count);
+    max_wgs = std::max(max_wgs, b->wgs[c]);
+  }
   std::vector<BatchItem> items(std::max(1, count));
   for (int q = 0; q < count; ++q) {
     const int i = order[q];
@@ -1391,8 +1410,7 @@ extern "C" int hqrrp_batch_create(hqrrp_handle_t h, int count, const int* m,
   HQ_RET(ensure(b->omega, sizeof(double) * om.size()));
   HQ_TRY(hipMemcpy(b->omega.p, om.data(), sizeof(double) * om.size(),
                    hipMemcpyHostToDevice));
-  b->wgs = std::max(1, std::min(count, h->cu_count));
-  HQ_RET(ensure(b->work, sizeof(double) * (size_t)batch_work_doubles() * b->wgs));
+  HQ_RET(ensure(b->work, sizeof(double) * (size_t)batch_work_doubles() * max_wgs));
   HQ_TRY(hipMemset(b->work.p, 0, b->work.bytes));
   HQ_RET(ensure(b->counter, 256));
   HQ_TRY(hipMemset(b->counter.p, 0, b->counter.bytes));
@@ -1414,25 +1432,64 @@ extern "C" int hqrrp_batch_factor_dev(hqrrp_handle_t h, hqrrp_batch_t b,
   if (!dA || !djpvt || !dtau) return HQRRP_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
-  BatchArgs a{};
-  a.items = b->items.as<BatchItem>();
-  a.count = b->count;
-  a.pp = b->pp;
-  a.pivoting = b->pivoting;
-  a.A = dA;
-  a.jpvt = djpvt;
-  a.tau = dtau;
-  a.omega = b->omega.as<double>();
-  a.work = b->work.as<double>();
-  a.counter = b->counter.as<int>();
-  HQ_TRY(launch_batch_hqrrp(a, b->wgs, h->cur));
+  HQ_TRY(launch_batch_reset(b->counter.as<int>(), 3, h->cur));
+  for (int c = 0; c < 3; ++c) {
+    if (b->cnt[c] == 0) continue;
+    BatchArgs a{};
+    a.items = b->items.as<BatchItem>() + b->first[c];
+    a.count = b->cnt[c];
+    a.pp = b->pp;
+    a.pivoting = b->pivoting;
+    a.A = dA;
+    a.jpvt = djpvt;
+    a.tau = dtau;
+    a.omega = b->omega.as<double>();
+    a.work = b->work.as<double>();
+    a.counter = b->counter.as<int>() + c;
+    a.prof = b->prof_on ? b->prof.as<unsigned long long>() : nullptr;
+    HQ_TRY(launch_batch_hqrrp(a, 16 << c, b->wgs[c], h->cur));
+  }
+  return HQRRP_OK;
+}
+
+static const char* const kBatchPhaseNames[HQB_PHASES] = {
+    "idle", "sketch_gemm", "sketch_qrcp", "panel", "t_factor",
+    "trailing_update", "downdate", "finish_permutation"};
+
+extern "C" int hqrrp_batch_phase_count(void) { return HQB_PHASES; }
+
+extern "C" const char* hqrrp_batch_phase_name(int phase) {
+  return (phase >= 0 && phase < HQB_PHASES) ? kBatchPhaseNames[phase] : "";
+}
+
+extern "C" int hqrrp_batch_profile(hqrrp_batch_t b, int on) {
+  if (!b) return HQRRP_ERR_ARG;
+  HQ_TRY(hipSetDevice(b->device));
+  if (on) {
+    HQ_RET(ensure(b->prof, sizeof(unsigned long long) * HQB_PHASES));
+    HQ_TRY(hipMemset(b->prof.p, 0, b->prof.bytes));
+  }
+  b->prof_on = on != 0;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_batch_profile_read(hqrrp_batch_t b, int nphase,
+                                        double* cycles) {
+  if (!b || nphase < HQB_PHASES || !cycles) return HQRRP_ERR_ARG;
+  unsigned long long v[HQB_PHASES] = {};
+  if (b->prof.p) {
+    HQ_TRY(hipSetDevice(b->device));
+    HQ_TRY(hipDeviceSynchronize());
+    HQ_TRY(hipMemcpy(v, b->prof.p, sizeof(v), hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < HQB_PHASES; ++i) cycles[i] = (double)v[i];
   return HQRRP_OK;
 }
 
 extern "C" int hqrrp_batch_destroy(hqrrp_batch_t b) {
   if (!b) return HQRRP_OK;
   (void)hipSetDevice(b->device);
-  for (DevBuf* d : {&b->items, &b->omega, &b->work, &b->counter})
+  for (DevBuf* d : {&b->items, &b->omega, &b->work, &b->counter, &b->prof})
     if (d->p) (void)hipFree(d->p);
   delete b;
   return HQRRP_OK;

@@ -42,7 +42,7 @@ constexpr int kDim = HQB_DIM_MAX;
 constexpr int kBT = 512;            // threads per workgroup (one row/column each)
 constexpr int kWaves = kBT / 64;
 constexpr int kLdv = kDim;          // leading dimension of V T' (workspace)
-constexpr int kPermRows = 8;        // rows per pass of the final permutation
+constexpr int kPermRows = 4;        // rows per pass of the final permutation
 constexpr int kPermPitch = kDim + 1;
 
 // Workspace of one workgroup (doubles): Y (r x n), G (r x m), V' (nb x R,
@@ -59,27 +59,32 @@ struct Cfg {
   static constexpr int CPT = 1;                // sketch columns per thread
   static constexpr int RPT = 1;                // panel rows per thread
   static constexpr int IT = NB / 16;           // 16-row tiles of V'
+  static constexpr int SB = 1;  // strips per round (more spill at 128 VGPRs)
   static constexpr int RT = (RMAX + 15) / 16;  // 16-row tiles of Y
   static constexpr int SH = NB == 64 ? 0 : (NB == 32 ? 1 : 2);
 };
 
+template <int NB>
 struct Lds {
+  static constexpr int RMAX = NB + kPPMax;
+  static constexpr int IT = NB / 16;
   int map[kDim];       // logical column -> physical column of A (and Y)
-  int slot_col[64];    // panel slot -> physical column
-  int slot_step[64];   // panel slot -> elimination step
+  int slot_col[NB];    // panel slot -> physical column
+  int slot_step[NB];   // panel slot -> elimination step
   int item;
-  double tau[64];
+  unsigned long long clk[HQB_PHASES + 1];  // optional phase cycle counts
+  double tau[NB];
   double onehot[192];  // onehot[64] = 1, zeros elsewhere
   double step_gt[192]; // step_gt[i] = (i > 64)
-  double pn_scale[64]; // panel: dlarfg scale of each slot's reflector
-  double sk_hdr[2][kWaves][4];          // sketch: wave's (key, pos, tau, alpha)
-  double sk_x[2][kWaves][64 + kPPMax];  // sketch: wave's candidate reflector
-  double pn_part[2][kWaves][64];        // panel: per-wave partial sums per slot
-  double pn_rowk[2][64];                // panel: row k of every slot
-  double pn_bc[kWaves][2][64];          // panel: per-wave broadcast (f, row k)
+  double pn_scale[NB]; // panel: dlarfg scale of each slot's reflector
+  double sk_hdr[2][kWaves][4];       // sketch: wave's (key, pos, tau, alpha)
+  double sk_x[2][kWaves][RMAX];      // sketch: wave's candidate reflector
+  double pn_part[2][kWaves][NB];     // panel: per-wave partial sums per slot
+  double pn_rowk[2][NB];             // panel: row k of every slot
+  double pn_bc[kWaves][2][NB];       // panel: per-wave broadcast (f, row k)
   union {
-    double T[64][65];  // T upper triangle; S(l,k) = v_l'v_k at T[k][l], l < k
-    double W[kWaves * 4 * 4 * 64];  // strip partials [wave][IT][4][64]
+    double T[NB][NB + 1];  // T upper triangle; S(l,k) = v_l'v_k at T[k][l]
+    double W[kWaves * Cfg<NB>::SB * IT * 4 * 64];  // [wave][SB][IT][4][64]
     double perm[kPermRows * kPermPitch];
   } u;
 };
@@ -88,14 +93,36 @@ __device__ __forceinline__ hq_d4 mfma4(double a, double b, hq_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// *p if c, else 0.  The load is issued whatever c is (p must be a valid
-// address): the empty asm consumes it, so the compiler cannot move it under
-// a branch on c, which would serialise it with the matrix-core work.
-__device__ __forceinline__ double ld_keep(const double* p, bool c) {
-  double v = *p;
-  asm volatile("" : "+v"(v));
-  return c ? v : 0.0;
-}
+// Operands outside a matrix are read from here (a load from a selected
+// address, never a load under a branch, so loads stay in flight together).
+__device__ const double kZeros[4] = {0.0, 0.0, 0.0, 0.0};
+
+// Optional per-phase cycle counts (BatchArgs::prof): thread 0 of each
+// workgroup times the phases of its matrices (shader clock, after the
+// barrier that ends each phase) into LDS and adds them up at exit.
+struct PhaseClock {
+  unsigned long long* acc;  // LDS, HQB_PHASES + 1 (the last: phase start)
+  bool on;
+  __device__ void start(unsigned long long* lds, bool enable) {
+    acc = lds;
+    on = enable;
+    if (on && threadIdx.x == 0) {
+      for (int i = 0; i < HQB_PHASES; ++i) acc[i] = 0;
+      acc[HQB_PHASES] = clock64();
+    }
+  }
+  __device__ __forceinline__ void mark(int ph) {
+    if (on && threadIdx.x == 0) {
+      const unsigned long long t = clock64();
+      acc[ph] += t - acc[HQB_PHASES];
+      acc[HQB_PHASES] = t;
+    }
+  }
+  __device__ void flush(unsigned long long* out) {
+    if (on && threadIdx.x == 0)
+      for (int i = 0; i < HQB_PHASES; ++i) atomicAdd(out + i, acc[i]);
+  }
+};
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -161,7 +188,7 @@ __device__ __forceinline__ double wave_reduce_scatter(Val val) {
 // registers of the reduce-scatter); one barrier; buffers alternate with
 // `buf`.
 template <int NB, class Val>
-__device__ __forceinline__ double pn_exchange(Lds& L, Val val, int buf) {
+__device__ __forceinline__ double pn_exchange(Lds<NB>& L, Val val, int buf) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int CK = NB < 32 ? NB : 32;
   constexpr int SH = Cfg<CK>::SH;
@@ -204,13 +231,14 @@ __device__ __forceinline__ void sketch_gemm(const double* __restrict__ G, int r,
     for (int k0 = 0; k0 < m; k0 += 4) {
       const int k = k0 + l4;
       const bool kok = k < m;
+      // k >= m reads zero for the A operand; rows of Y >= r are not stored
+      const double bv = *(kok ? Ac + k : kZeros);
       const int kc = kok ? k : m - 1;
-      const double bv = ld_keep(Ac + kc, cok && kok);
 #pragma unroll
       for (int i = 0; i < RT; ++i) {
         if (i < rtn) {
           const int row = i * 16 + l15;
-          const double av = ld_keep(G + min(row, r - 1) + kc * r, row < r && kok);
+          const double av = G[min(row, r - 1) + kc * r];
           acc[i] = mfma4(av, bv, acc[i]);
         }
       }
@@ -236,7 +264,7 @@ __device__ __forceinline__ void sketch_gemm(const double* __restrict__ G, int r,
 // a one-hot vector, so no register is ever indexed by k.  Only the pivots are
 // kept: the logical -> physical map takes each swap.
 template <int NB>
-__device__ __forceinline__ void sketch_qrcp(Lds& L, const double* __restrict__ Y, int r, int j,
+__device__ __forceinline__ void sketch_qrcp(Lds<NB>& L, const double* __restrict__ Y, int r, int j,
                             int N, int b) {
   constexpr int RMAX = Cfg<NB>::RMAX, CPT = Cfg<NB>::CPT;
   constexpr int CH = 8, NCH = (RMAX + CH - 1) / CH;
@@ -276,21 +304,24 @@ __device__ __forceinline__ void sketch_qrcp(Lds& L, const double* __restrict__ Y
     hq_wave_argmax(wk, wp, 64);
     double* xw = L.sk_x[buf][wave];
     double* hd = L.sk_hdr[buf][wave];
+    const double* oh = L.onehot + 64 - k;   // oh[i] = (i == k)
+    const double* gt = L.step_gt + 64 - k;  // gt[i] = (i > k)
     if (kc >= 0 && kp == wp) {
-      // the wave's candidate forms its reflector (dlarfg) in LDS
-      double xn = 0.0;
+      // the wave's candidate forms its reflector (dlarfg) in LDS; rows past
+      // r hold zeros, so u_i = (i > k) scale x_i + (i == k) needs no test
+      double alpha = 0.0, xn = 0.0;
 #pragma unroll
       for (int i = 0; i < RMAX; ++i) {
         const double xv = (CPT == 1 || kc == 0) ? v[0][i] : v[CPT - 1][i];
-        if (i == k) hd[3] = xv;
-        if (i > k) xn += xv * xv;
+        alpha = fma(oh[i], xv, alpha);
+        const double xg = gt[i] * xv;
+        xn = fma(xg, xg, xn);
       }
-      wave_lds_sync();
-      const hq_reflector h = hq_dlarfg(hd[3], xn);
+      const hq_reflector h = hq_dlarfg(alpha, xn);
 #pragma unroll
       for (int i = 0; i < RMAX; ++i) {
         const double xv = (CPT == 1 || kc == 0) ? v[0][i] : v[CPT - 1][i];
-        xw[i] = (i > k && i < r) ? h.scale * xv : (i == k ? 1.0 : 0.0);
+        xw[i] = fma(gt[i] * h.scale, xv, oh[i]);
       }
       hd[0] = wk;
       hd[1] = (double)wp;
@@ -316,7 +347,6 @@ __device__ __forceinline__ void sketch_qrcp(Lds& L, const double* __restrict__ Y
     if (bp < k || bp >= N) bp = k;  // only reachable with non-finite norms
     const double tauk = L.sk_hdr[buf][ww][2];
     const double* us = L.sk_x[buf][ww];
-    const double* oh = L.onehot + 64 - k;  // oh[i] = (i == k)
     // w = u'v and the old row k of every column held
     double w[CPT], vk[CPT];
 #pragma unroll
@@ -367,7 +397,6 @@ __device__ __forceinline__ void sketch_qrcp(Lds& L, const double* __restrict__ Y
         tt = (0.0 > uu) ? 0.0 : uu;
         const double ratio = d[c] / e[c];
         if (tt * ratio * ratio <= HQ_TOL3Z) {
-          const double* gt = L.step_gt + 64 - k;  // gt[i] = (i > k)
           double s = 0.0;
 #pragma unroll
           for (int i = 0; i < RMAX; ++i) {
@@ -411,7 +440,7 @@ __device__ __forceinline__ void sketch_qrcp(Lds& L, const double* __restrict__ Y
 // applied on the way out).  A second exchange recomputes partial norms when
 // the LAWN test asks for it.
 template <int NB>
-__device__ __forceinline__ void panel_qrcp(Lds& L, double* __restrict__ A, long lda, int R,
+__device__ __forceinline__ void panel_qrcp(Lds<NB>& L, double* __restrict__ A, long lda, int R,
                            int j, int b, int pivoting,
                            double* __restrict__ tau, double* __restrict__ Vt) {
   constexpr int RPT = Cfg<NB>::RPT;
@@ -509,7 +538,7 @@ __device__ __forceinline__ void panel_qrcp(Lds& L, double* __restrict__ A, long 
     if (wave == 0 && sdone && lane < b) L.u.T[k][L.slot_step[lane]] = ssc * w;
     if (lane < NB) {
       bc[lane] = f;
-      bc[64 + lane] = rk;
+      bc[NB + lane] = rk;
     }
     wave_lds_sync();
     // 6. update my rows of every slot (f = 0 but for the live ones); the
@@ -524,7 +553,7 @@ __device__ __forceinline__ void panel_qrcp(Lds& L, double* __restrict__ A, long 
     if (owner) {
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
-        const double rc = bc[64 + c];
+        const double rc = bc[NB + c];
         if (RPT == 1 || sk == 0)
           P[0][c] = rc;
         else
@@ -584,6 +613,16 @@ __device__ __forceinline__ void panel_qrcp(Lds& L, double* __restrict__ A, long 
   }
   if (wave == 0 && lane < b) L.pn_scale[lane] = ssc;
   __syncthreads();
+  // V' rows (steps) b..NB-1 and columns (rows of the panel) R..16*ceil(R/16)
+  // hold zeros, so that the products read V' unmasked
+  {
+    const int row = t;
+    if (row < ((R + 15) & ~15)) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c >= b || row >= R) Vt[c + (long)row * NB] = 0.0;
+    }
+  }
   // write the panel back (slot c -> its physical column) and V'
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
@@ -608,7 +647,7 @@ __device__ __forceinline__ void panel_qrcp(Lds& L, double* __restrict__ A, long 
 // T(k,k) = tau_k, T(0:k,k) = -tau_k T(0:k,0:k) S(0:k,k).  Wave 0, lane i owns
 // row i of T.
 template <int NB>
-__device__ __forceinline__ void build_t(Lds& L, int b) {
+__device__ __forceinline__ void build_t(Lds<NB>& L, int b) {
   const int lane = threadIdx.x & 63;
   if ((threadIdx.x >> 6) != 0) return;
   const int i = lane;
@@ -628,7 +667,7 @@ __device__ __forceinline__ void build_t(Lds& L, int b) {
 
 // -V T' (R x b, ld kLdv) from V' and T, stored negated for the update.
 template <int NB>
-__device__ __forceinline__ void form_vt(Lds& L, const double* __restrict__ Vt,
+__device__ __forceinline__ void form_vt(Lds<NB>& L, const double* __restrict__ Vt,
                         double* __restrict__ VTc, int R, int b) {
   constexpr int IT = Cfg<NB>::IT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -643,7 +682,7 @@ __device__ __forceinline__ void form_vt(Lds& L, const double* __restrict__ Vt,
     for (int k0 = 0; k0 < NB; k0 += 4) {
       if (k0 < b) {
         const int k = k0 + l4;
-        const double av = ld_keep(Vt + k + row * NB, row < R && k < b);
+        const double av = Vt[k + row * NB];  // zero-padded
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
           const int i = it * 16 + l15;
@@ -658,96 +697,118 @@ __device__ __forceinline__ void form_vt(Lds& L, const double* __restrict__ Vt,
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int rr = g * 16 + l4 + 4 * e, i = it * 16 + l15;
-        if (rr < R && i < b) VTc[rr + (long)i * kLdv] = -acc[it][e];
+        VTc[rr + (long)i * kLdv] = -acc[it][e];  // zeros outside R x b
       }
   }
 }
 
 // C (R x ncols) := (I - V T V')' C = C - (V T') (V' C), C(row, col) read
-// at cin(col)[row * rs] and written to cout(col)[row * rs].  Strips of 16
-// columns: the waves split the rows (16-row groups dealt round-robin), form
-// their part of V'C, sum the parts through LDS in a fixed order, then
-// re-read their C tiles into the MFMA accumulators and apply the rank-b
-// update (the second read of a strip hits L2).
+// at cin(col)[row * rs] and written to cout(col)[row * rs].  Rounds of SB
+// strips of 16 columns: the waves split the rows (16-row groups dealt
+// round-robin), form their part of V'C for every strip of the round, sum the
+// parts through LDS in a fixed order (two barriers per round), then re-read
+// their C tiles into the MFMA accumulators and apply the rank-b update (the
+// second read hits L2).  SB strips keep SB times the loads in flight.
 template <int NB, class CIn, class COut>
-__device__ __forceinline__ void apply_qt(Lds& L, const double* __restrict__ Vt,
+__device__ __forceinline__ void apply_qt(Lds<NB>& L, const double* __restrict__ Vt,
                          const double* __restrict__ VTc, int R, int b,
                          int ncols, long rs, CIn cin, COut cout) {
-  constexpr int IT = Cfg<NB>::IT;
+  constexpr int IT = Cfg<NB>::IT, SB = Cfg<NB>::SB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l15 = lane & 15, l4 = lane >> 4;
   const int groups = (R + 15) / 16;
   double* Wl = L.u.W;
-  constexpr int WS = IT * 4 * 64;  // doubles per wave in Wl
-  for (int c0 = 0; c0 < ncols; c0 += 16) {
-    const int col = c0 + l15;
-    const bool cok = col < ncols;
-    const double* cp = cin(cok ? col : 0);
-    hq_d4 wacc[IT];
+  constexpr int WS = SB * IT * 4 * 64;  // doubles per wave in Wl
+  for (int c0 = 0; c0 < ncols; c0 += 16 * SB) {
+    const double* cp[SB];
+    bool cok[SB];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) wacc[it] = hq_d4{0.0, 0.0, 0.0, 0.0};
+    for (int sb = 0; sb < SB; ++sb) {
+      const int col = c0 + 16 * sb + l15;
+      cok[sb] = col < ncols;
+      cp[sb] = cin(cok[sb] ? col : 0);
+    }
+    hq_d4 wacc[SB][IT];
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb)
+#pragma unroll
+      for (int it = 0; it < IT; ++it) wacc[sb][it] = hq_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
     for (int g = wave; g < groups; g += kWaves) {
-      hq_d4 cg;
+      hq_d4 cg[SB];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = g * 16 + l4 + 4 * e;
-        cg[e] = ld_keep(cp + (long)min(row, R - 1) * rs, row < R && cok);
-      }
+      for (int sb = 0; sb < SB; ++sb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // rows past R repeat row R-1; V' is zero there
+          const int row = g * 16 + l4 + 4 * e;
+          cg[sb][e] = cp[sb][(long)min(row, R - 1) * rs];
+        }
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
         const int row = g * 16 + 4 * s4 + l4;
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
-          const int i = it * 16 + l15;
-          const double av = ld_keep(Vt + i + row * NB, i < b && row < R);
-          wacc[it] = mfma4(av, cg[s4], wacc[it]);
+          const double av = Vt[it * 16 + l15 + row * NB];  // zero-padded
+#pragma unroll
+          for (int sb = 0; sb < SB; ++sb)
+            wacc[sb][it] = mfma4(av, cg[sb][s4], wacc[sb][it]);
         }
       }
     }
 #pragma unroll
-    for (int it = 0; it < IT; ++it)
+    for (int sb = 0; sb < SB; ++sb)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        Wl[wave * WS + (it * 4 + e) * 64 + lane] = wacc[it][e];
+      for (int it = 0; it < IT; ++it)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          Wl[wave * WS + ((sb * IT + it) * 4 + e) * 64 + lane] = wacc[sb][it][e];
     __syncthreads();
-    hq_d4 ws[IT];
+    hq_d4 ws[SB][IT];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
+    for (int sb = 0; sb < SB; ++sb)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int o = (it * 4 + e) * 64 + lane;
-        double sum = 0.0;
+      for (int it = 0; it < IT; ++it) {
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) sum += Wl[w * WS + o];
-        ws[it][e] = sum;
+        for (int e = 0; e < 4; ++e) {
+          const int o = ((sb * IT + it) * 4 + e) * 64 + lane;
+          double sum = 0.0;
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w) sum += Wl[w * WS + o];
+          ws[sb][it][e] = sum;
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
-    }
     __syncthreads();
-    double* op = cout(cok ? col : 0);
 #pragma unroll 1
     for (int g = wave; g < groups; g += kWaves) {
-      hq_d4 cg;
+      hq_d4 cg[SB];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = g * 16 + l4 + 4 * e;
-        cg[e] = ld_keep(cp + (long)min(row, R - 1) * rs, row < R && cok);
-      }
+      for (int sb = 0; sb < SB; ++sb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = g * 16 + l4 + 4 * e;
+          cg[sb][e] = cp[sb][(long)min(row, R - 1) * rs];  // rows past R unused
+        }
       const int row = g * 16 + l15;
 #pragma unroll
       for (int ks = 0; ks < NB / 4; ++ks) {
         if (ks * 4 < b) {
-          const int k = ks * 4 + l4;
-          const double av = ld_keep(VTc + row + k * kLdv, row < R && k < b);
-          cg = mfma4(av, ws[ks / 4][ks % 4], cg);
+          const double av = VTc[row + (ks * 4 + l4) * kLdv];  // zero-padded
+#pragma unroll
+          for (int sb = 0; sb < SB; ++sb)
+            cg[sb] = mfma4(av, ws[sb][ks / 4][ks % 4], cg[sb]);
         }
       }
-      if (cok) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int rr = g * 16 + l4 + 4 * e;
-          if (rr < R) op[(long)rr * rs] = cg[e];
+      for (int sb = 0; sb < SB; ++sb) {
+        if (cok[sb]) {
+          double* op = cout(c0 + 16 * sb + l15);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int rr = g * 16 + l4 + 4 * e;
+            if (rr < R) op[(long)rr * rs] = cg[sb][e];
+          }
         }
       }
     }
@@ -757,7 +818,7 @@ __device__ __forceinline__ void apply_qt(Lds& L, const double* __restrict__ Vt,
 // Y(:, c) -= G(:, j:j+b) R12(:, c) for the logical columns c >= j + b
 // (R12 = A(j:j+b, c)).  Work item: a 16-column tile with all row tiles.
 template <int NB>
-__device__ __forceinline__ void y2_update(Lds& L, double* __restrict__ Y,
+__device__ __forceinline__ void y2_update(Lds<NB>& L, double* __restrict__ Y,
                           const double* __restrict__ G, int r,
                           const double* __restrict__ A, long lda, int j, int b,
                           int nrest) {
@@ -777,20 +838,19 @@ __device__ __forceinline__ void y2_update(Lds& L, double* __restrict__ Y,
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = i * 16 + l4 + 4 * e;
-        acc[i][e] = ld_keep(Yc + min(row, r - 1), i < rtn && row < r && cok);
+        acc[i][e] = Yc[min(row, r - 1)];  // rows past r are not stored
       }
 #pragma unroll
     for (int k0 = 0; k0 < NB; k0 += 4) {
       if (k0 < b) {
         const int k = k0 + l4;
         const int kc = min(k, b - 1);
-        const double bv = ld_keep(Ac + kc, k < b && cok);
+        const double bv = *(k < b ? Ac + k : kZeros);  // R12, zero past b
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
           if (i < rtn) {
             const int row = i * 16 + l15;
-            const double av =
-                -ld_keep(G + min(row, r - 1) + (j + kc) * r, row < r && k < b);
+            const double av = -G[min(row, r - 1) + (j + kc) * r];
             acc[i] = mfma4(av, bv, acc[i]);
           }
         }
@@ -808,7 +868,8 @@ __device__ __forceinline__ void y2_update(Lds& L, double* __restrict__ Y,
 
 // A(:, pos) = A(:, map[pos]) for every pos (a few rows at a time through
 // LDS) and jpvt(pos) = jpvt(map[pos]).
-__device__ __forceinline__ void finish_perm(Lds& L, double* __restrict__ A, long lda, int m,
+template <int NB>
+__device__ __forceinline__ void finish_perm(Lds<NB>& L, double* __restrict__ A, long lda, int m,
                             int n, int* __restrict__ jp) {
   const int t = threadIdx.x;
   int jv[kDim / kBT];
@@ -839,8 +900,8 @@ __device__ __forceinline__ void finish_perm(Lds& L, double* __restrict__ A, long
 }
 
 template <int NB>
-__device__ __forceinline__ void factor(Lds& L, const BatchArgs& a, const BatchItem& it,
-                       double* __restrict__ work) {
+__device__ __forceinline__ void factor(Lds<NB>& L, const BatchArgs& a, const BatchItem& it,
+                       double* __restrict__ work, PhaseClock& pc) {
   const int t = threadIdx.x;
   const int m = it.m, n = it.n, nb = it.nb, pp = a.pp, r = nb + pp;
   const long lda = it.lda;
@@ -854,8 +915,10 @@ __device__ __forceinline__ void factor(Lds& L, const BatchArgs& a, const BatchIt
   double* Vt = Gw + kGW;
   double* VTc = Vt + kVW;
   for (int c = t; c < n; c += kBT) L.map[c] = c;
+  pc.mark(HQB_PH_IDLE);
   sketch_gemm<NB>(a.omega, r, m, n, A, lda, Yw);
   __syncthreads();
+  pc.mark(HQB_PH_SKETCH_GEMM);
   for (int j = 0; j < mn; j += nb) {
     const int b = min(nb, min(n - j, m - j));
     const bool last = (j + nb >= m) || (j + nb >= n);
@@ -863,19 +926,24 @@ __device__ __forceinline__ void factor(Lds& L, const BatchArgs& a, const BatchIt
     if (!last) {
       sketch_qrcp<NB>(L, Yw, r, j, n - j, b);
       __syncthreads();
+      pc.mark(HQB_PH_SKETCH_QRCP);
     }
     panel_qrcp<NB>(L, A, lda, R, j, b, a.pivoting, tau, Vt);
     __syncthreads();
+    pc.mark(HQB_PH_PANEL);
     if (nrest <= 0 && last) continue;
     build_t<NB>(L, b);
     __syncthreads();
     form_vt<NB>(L, Vt, VTc, R, b);
     __syncthreads();
+    pc.mark(HQB_PH_T);
     if (nrest > 0) {
       auto acol = [&](int c) -> double* {
         return A + (long)L.map[j + b + c] * lda + j;
       };
       apply_qt<NB>(L, Vt, VTc, R, b, nrest, 1, acol, acol);
+      __syncthreads();
+      pc.mark(HQB_PH_TRAIL);
     }
     if (!last) {
       const double* Gin = (j == 0) ? a.omega : Gw;
@@ -887,16 +955,29 @@ __device__ __forceinline__ void factor(Lds& L, const BatchArgs& a, const BatchIt
         y2_update<NB>(L, Yw, Gw, r, A, lda, j, b, nrest);
         __syncthreads();
       }
-    } else {
-      __syncthreads();
+      pc.mark(HQB_PH_DOWNDATE);
     }
   }
-  finish_perm(L, A, lda, m, n, jp);
+  finish_perm<NB>(L, A, lda, m, n, jp);
+  pc.mark(HQB_PH_FINISH);
 }
 
-__global__ __launch_bounds__(kBT, 1) void batch_hqrrp_kernel(BatchArgs a) {
-  __shared__ Lds L;
+// Workgroups per CU of each variant (registers: 128 VGPRs per thread allow
+// two 512-thread workgroups per CU).
+template <int NB>
+struct Occ {
+  static constexpr int value = NB <= 16 ? 2 : 1;
+};
+
+// (__launch_bounds__' second argument: waves per SIMD.)
+template <int NB>
+__global__ __launch_bounds__(kBT, Occ<NB>::value * kWaves / 4) void
+batch_hqrrp_kernel(
+    BatchArgs a) {
+  __shared__ Lds<NB> L;
   double* work = a.work + (long)blockIdx.x * kWork;
+  PhaseClock pc;
+  pc.start(L.clk, a.prof != nullptr);
   for (int i = threadIdx.x; i < 192; i += kBT) {
     L.onehot[i] = (i == 64) ? 1.0 : 0.0;
     L.step_gt[i] = (i > 64) ? 1.0 : 0.0;
@@ -908,29 +989,41 @@ __global__ __launch_bounds__(kBT, 1) void batch_hqrrp_kernel(BatchArgs a) {
     __syncthreads();
     if (q >= a.count) break;
     const BatchItem it = a.items[q];
-    if (it.nb > 32)
-      factor<64>(L, a, it, work);
-    else if (it.nb > 16)
-      factor<32>(L, a, it, work);
-    else
-      factor<16>(L, a, it, work);
+    factor<NB>(L, a, it, work, pc);
     __syncthreads();
   }
+  pc.mark(HQB_PH_IDLE);
+  pc.flush(a.prof);
 }
 
-__global__ void batch_reset_kernel(int* counter) {
-  if (threadIdx.x == 0) *counter = 0;
+__global__ void batch_reset_kernel(int* counter, int n) {
+  if ((int)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
 
 }  // namespace
 
 long batch_work_doubles() { return kWork; }
 
-hipError_t launch_batch_hqrrp(const BatchArgs& a, int wgs, hipStream_t s) {
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+int batch_class_nb(int nb) { return nb > 32 ? 64 : (nb > 16 ? 32 : 16); }
+
+int batch_wgs_per_cu(int class_nb) {
+  return class_nb == 64 ? Occ<64>::value
+                        : (class_nb == 32 ? Occ<32>::value : Occ<16>::value);
+}
+
+hipError_t launch_batch_reset(int* counters, int n, hipStream_t s) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_hqrrp(const BatchArgs& a, int class_nb, int wgs,
+                              hipStream_t s) {
   if (a.count <= 0 || wgs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_hqrrp_kernel, dim3(wgs), dim3(kBT), 0, s, a);
+  if (class_nb == 64)
+    hipLaunchKernelGGL(batch_hqrrp_kernel<64>, dim3(wgs), dim3(kBT), 0, s, a);
+  else if (class_nb == 32)
+    hipLaunchKernelGGL(batch_hqrrp_kernel<32>, dim3(wgs), dim3(kBT), 0, s, a);
+  else
+    hipLaunchKernelGGL(batch_hqrrp_kernel<16>, dim3(wgs), dim3(kBT), 0, s, a);
   return hipGetLastError();
 }

@@ -87,9 +87,29 @@ struct BatchArgs {
   const double* omega;  // normals: Omega of r x m is omega[0, r*m), ld r
   double* work;         // batch_work_doubles() per workgroup
   int* counter;         // queue head, reset by launch_batch_hqrrp
+  unsigned long long* prof;  // HQB_PHASES cycle sums, or null
+};
+// Phases of the batched kernel's optional cycle counts.
+enum {
+  HQB_PH_IDLE = 0,  // queue, setup, waiting for the last matrices
+  HQB_PH_SKETCH_GEMM,
+  HQB_PH_SKETCH_QRCP,
+  HQB_PH_PANEL,
+  HQB_PH_T,          // T and V T'
+  HQB_PH_TRAIL,      // trailing update of A
+  HQB_PH_DOWNDATE,   // G update and Y2
+  HQB_PH_FINISH,     // column permutation of A and jpvt
+  HQB_PHASES
 };
 long batch_work_doubles();
-hipError_t launch_batch_hqrrp(const BatchArgs& a, int wgs, hipStream_t s);
+// Kernel variant (16, 32 or 64) serving block size nb; its workgroups per CU.
+int batch_class_nb(int nb);
+int batch_wgs_per_cu(int class_nb);
+// Zeroes n queue counters; launches the variant's persistent kernel on the
+// items of one class (a.items .. a.items + a.count, all of that class).
+hipError_t launch_batch_reset(int* counters, int n, hipStream_t s);
+hipError_t launch_batch_hqrrp(const BatchArgs& a, int class_nb, int wgs,
+                              hipStream_t s);
 
 // k_trail.hip
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);

@@ -195,6 +195,14 @@ int hqrrp_batch_factor_dev(hqrrp_handle_t h, hqrrp_batch_t plan,
                            double *const *dA, int *const *djpvt,
                            double *const *dtau);
 int hqrrp_batch_destroy(hqrrp_batch_t plan);
+/* Optional cycle counts of the batched kernel's phases (no counterpart in
+ * the reference): when on, every later factorisation adds, per phase, the
+ * shader-clock cycles its workgroups spent there (summed over workgroups);
+ * profile_read waits for the device and returns the sums. */
+int hqrrp_batch_profile(hqrrp_batch_t plan, int on);
+int hqrrp_batch_profile_read(hqrrp_batch_t plan, int nphase, double *cycles);
+int hqrrp_batch_phase_count(void);
+const char *hqrrp_batch_phase_name(int phase);
 
 /* ------------------------------------- component entry points (device) */
 
