@@ -147,6 +147,13 @@ def bench_batched(args):
     mats = matgen.batch_matrices(m, n, seed=0, workers=workers)
     h = hqrrp_amd.Handle(0)
     h.use_stream(torch.cuda.current_stream())
+    tune_s, table = None, None
+    if args.batch_nb is None:
+        t0 = time.perf_counter()
+        table = h.batch_tune(pp=pp, panel_pivoting=1)
+        tune_s = time.perf_counter() - t0
+        print(f"# autotuned nb per 64x64 bucket in {tune_s:.1f} s:\n{table}",
+              file=sys.stderr, flush=True)
     plan = hqrrp_amd.BatchPlan(h, m, n, nb=args.batch_nb, pp=pp)
     nbs = plan.nb
     src = hqrrp_amd.PackedBatch(m, n)
@@ -224,6 +231,10 @@ def bench_batched(args):
                    "count": count, "pp": pp, "panel_pivoting": 1,
                    "nb_counts": {int(k): int(v) for k, v in
                                  zip(*np.unique(nbs, return_counts=True))},
+                   "nb_choice": ("autotuned per 64x64 shape bucket "
+                                 f"({tune_s:.1f} s, untimed)"
+                                 if tune_s is not None else
+                                 f"fixed {args.batch_nb}"),
                    "omega": "reference stream seed 12"},
         "gflops_realised": round(flops / (ms / 1e3) / 1e9, 1),
         "kernel_ms": round(kern_ms, 3),

@@ -76,6 +76,8 @@ SIGNATURES = {
     "hqrrp_batch_factor_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "hqrrp_batch_destroy": (C.c_int, [_vp]),
     "hqrrp_batch_profile": (C.c_int, [_vp, C.c_int]),
+    "hqrrp_batch_tune": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hqrrp_batch_tune_table": (C.c_int, [_vp, _vp]),
     "hqrrp_batch_profile_read": (C.c_int, [_vp, C.c_int, _d]),
     "hqrrp_batch_phase_count": (C.c_int, []),
     "hqrrp_batch_phase_name": (C.c_char_p, [C.c_int]),

@@ -136,6 +136,16 @@ class Handle:
         return {L.hqrrp_phase_name(i).decode(): (ms[i], la[i], fl[i], by[i])
                 for i in range(n)}
 
+    def batch_tune(self, pp: int = 8, panel_pivoting: int = 1) -> np.ndarray:
+        """Autotune the batched path's block size per 64 x 64 shape bucket;
+        returns the 8 x 8 table ([m bucket, n bucket])."""
+        check(lib().hqrrp_batch_tune(self._h, pp, panel_pivoting),
+              "hqrrp_batch_tune")
+        t = np.zeros(64, dtype=np.int32)
+        check(lib().hqrrp_batch_tune_table(self._h, _ptr(t)),
+              "hqrrp_batch_tune_table")
+        return t.reshape(8, 8)
+
     def close(self) -> None:
         if self._h:
             lib().hqrrp_destroy(self._h)

@@ -1452,6 +1452,121 @@ extern "C" int hqrrp_batch_factor_dev(hqrrp_handle_t h, hqrrp_batch_t b,
   return HQRRP_OK;
 }
 
+// Autotuner of the batched path's block size: for every 64 x 64 shape
+// bucket, a batch of its midpoint shape (hash-filled) is factored with each
+// candidate block size and the fastest kept in the handle's table.
+extern "C" int hqrrp_batch_tune(hqrrp_handle_t h, int pp, int panel_pivoting) {
+  if (!h || pp < 0 || pp > HQB_PP_MAX) return HQRRP_ERR_ARG;
+  HQ_TRY(hipSetDevice(h->device));
+  const int count = std::max(256, 4 * h->cu_count);
+  const int cand[3] = {16, 32, 64};
+  const long maxsz = (long)HQB_DIM_MAX * HQB_DIM_MAX;
+  DevBuf A, J, T, P;
+  HQ_RET(ensure(A, sizeof(double) * (size_t)count * maxsz));
+  HQ_RET(ensure(J, sizeof(int) * (size_t)count * HQB_DIM_MAX));
+  HQ_RET(ensure(T, sizeof(double) * (size_t)count * HQB_DIM_MAX));
+  HQ_RET(ensure(P, sizeof(void*) * 3 * (size_t)count));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HQ_TRY(hipEventCreate(&e0));
+  HQ_TRY(hipEventCreate(&e1));
+  hipStream_t s = h->own;
+  int rc = HQRRP_OK;
+  int table[8][8] = {};
+  std::vector<void*> ptrs(3 * (size_t)count);
+  std::vector<int> jinit((size_t)count * HQB_DIM_MAX);
+  for (int bm = 0; bm < 8 && rc == HQRRP_OK; ++bm) {
+    for (int bn = 0; bn < 8 && rc == HQRRP_OK; ++bn) {
+      const int m = 64 * bm + 32, n = 64 * bn + 32;
+      for (int i = 0; i < count; ++i) {
+        ptrs[i] = A.as<double>() + (size_t)i * m * n;
+        ptrs[count + i] = J.as<int>() + (size_t)i * n;
+        ptrs[2 * count + i] = T.as<double>() + (size_t)i * std::min(m, n);
+      }
+      if (hipMemcpy(P.p, ptrs.data(), sizeof(void*) * ptrs.size(),
+                    hipMemcpyHostToDevice) != hipSuccess) {
+        rc = HQRRP_ERR_HIP;
+        break;
+      }
+      for (int i = 0; i < count; ++i)
+        for (int c = 0; c < n; ++c) jinit[(size_t)i * n + c] = c + 1;
+      std::vector<int> vm(count, m), vn(count, n);
+      float best = 0.0f;
+      int best_nb = 0;
+      for (int ci = 0; ci < 3 && rc == HQRRP_OK; ++ci) {
+        std::vector<int> vnb(count, cand[ci]);
+        hqrrp_batch_t plan = nullptr;
+        rc = hqrrp_batch_create(h, count, vm.data(), vn.data(), nullptr,
+                                vnb.data(), pp, panel_pivoting, &plan);
+        if (rc != HQRRP_OK) break;
+        float ms_min = 0.0f;
+        for (int rep = 0; rep < 3 && rc == HQRRP_OK; ++rep) {
+          if (launch_batch_fill(A.as<double>(), (long)count * m * n,
+                                1234u + rep, s) != hipSuccess ||
+              hipMemcpyAsync(J.p, jinit.data(), sizeof(int) * (size_t)count * n,
+                             hipMemcpyHostToDevice, s) != hipSuccess ||
+              hipEventRecord(e0, s) != hipSuccess) {
+            rc = HQRRP_ERR_HIP;
+            break;
+          }
+          {
+            std::lock_guard<std::mutex> lk(h->mu);
+            h->cur = s;
+            if (launch_batch_reset(plan->counter.as<int>(), 3, s) != hipSuccess)
+              rc = HQRRP_ERR_HIP;
+            for (int c = 0; c < 3 && rc == HQRRP_OK; ++c) {
+              if (plan->cnt[c] == 0) continue;
+              BatchArgs a{};
+              a.items = plan->items.as<BatchItem>() + plan->first[c];
+              a.count = plan->cnt[c];
+              a.pp = plan->pp;
+              a.pivoting = plan->pivoting;
+              a.A = P.as<double* const>();
+              a.jpvt = reinterpret_cast<int* const*>(P.as<void*>() + count);
+              a.tau = reinterpret_cast<double* const*>(P.as<void*>() + 2 * count);
+              a.omega = plan->omega.as<double>();
+              a.work = plan->work.as<double>();
+              a.counter = plan->counter.as<int>() + c;
+              if (launch_batch_hqrrp(a, 16 << c, plan->wgs[c], s) != hipSuccess)
+                rc = HQRRP_ERR_HIP;
+            }
+            h->cur = h->stream;
+          }
+          float ms = 0.0f;
+          if (rc != HQRRP_OK || hipEventRecord(e1, s) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = rc ? rc : HQRRP_ERR_HIP;
+            break;
+          }
+          if (rep > 0 && (ms_min == 0.0f || ms < ms_min)) ms_min = ms;
+        }
+        hqrrp_batch_destroy(plan);
+        if (rc == HQRRP_OK && (best_nb == 0 || ms_min < best)) {
+          best = ms_min;
+          best_nb = cand[ci];
+        }
+      }
+      table[bm][bn] = best_nb;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (DevBuf* d : {&A, &J, &T, &P})
+    if (d->p) (void)hipFree(d->p);
+  if (rc != HQRRP_OK) return rc;
+  memcpy(h->bt_nb, table, sizeof(table));
+  h->bt_pp = pp;
+  h->bt_piv = panel_pivoting ? 1 : 0;
+  return HQRRP_OK;
+}
+
+// The tuned block sizes (64 ints, [m bucket][n bucket], 0 = not tuned).
+extern "C" int hqrrp_batch_tune_table(hqrrp_handle_t h, int* table) {
+  if (!h || !table) return HQRRP_ERR_ARG;
+  memcpy(table, h->bt_nb, sizeof(h->bt_nb));
+  return HQRRP_OK;
+}
+
 static const char* const kBatchPhaseNames[HQB_PHASES] = {
     "idle", "sketch_gemm", "sketch_qrcp", "panel", "t_factor",
     "trailing_update", "downdate", "finish_permutation"};

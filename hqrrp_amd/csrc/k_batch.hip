@@ -996,6 +996,22 @@ batch_hqrrp_kernel(
   pc.flush(a.prof);
 }
 
+// Uniform values in (-1, 1) from a hash of (seed, index): input for timing
+// runs (the autotuner), not a statistical generator.
+__global__ void batch_fill_kernel(double* p, long n, unsigned seed) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull +
+                           ((unsigned long long)seed << 32);
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    p[i] = ((double)(z >> 11) + 0.5) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
 __global__ void batch_reset_kernel(int* counter, int n) {
   if ((int)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
@@ -1009,6 +1025,15 @@ int batch_class_nb(int nb) { return nb > 32 ? 64 : (nb > 16 ? 32 : 16); }
 int batch_wgs_per_cu(int class_nb) {
   return class_nb == 64 ? Occ<64>::value
                         : (class_nb == 32 ? Occ<32>::value : Occ<16>::value);
+}
+
+hipError_t launch_batch_fill(double* p, long n, unsigned seed, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(batch_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                     p, n, seed);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_reset(int* counters, int n, hipStream_t s) {

@@ -108,6 +108,7 @@ int batch_wgs_per_cu(int class_nb);
 // Zeroes n queue counters; launches the variant's persistent kernel on the
 // items of one class (a.items .. a.items + a.count, all of that class).
 hipError_t launch_batch_reset(int* counters, int n, hipStream_t s);
+hipError_t launch_batch_fill(double* p, long n, unsigned seed, hipStream_t s);
 hipError_t launch_batch_hqrrp(const BatchArgs& a, int class_nb, int wgs,
                               hipStream_t s);
 

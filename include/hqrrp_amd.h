@@ -200,6 +200,15 @@ int hqrrp_batch_destroy(hqrrp_batch_t plan);
  * shader-clock cycles its workgroups spent there (summed over workgroups);
  * profile_read waits for the device and returns the sums. */
 int hqrrp_batch_profile(hqrrp_batch_t plan, int on);
+/* Autotuner of the batched block size (no counterpart in the reference):
+ * times each of nb = 16, 32, 64 on a batch of the midpoint shape of every
+ * 64 x 64 bucket of (m, n) and keeps the fastest per bucket in the handle;
+ * later plans created with nb == NULL (same pp and panel_pivoting) use it.
+ * Takes about a second and a few GB of scratch device memory. */
+int hqrrp_batch_tune(hqrrp_handle_t h, int pp, int panel_pivoting);
+/* The tuned table: 64 ints, table[8*bm + bn] for m in (64 bm, 64 bm + 64],
+ * n in (64 bn, 64 bn + 64]; 0 where not tuned. */
+int hqrrp_batch_tune_table(hqrrp_handle_t h, int *table);
 int hqrrp_batch_profile_read(hqrrp_batch_t plan, int nphase, double *cycles);
 int hqrrp_batch_phase_count(void);
 const char *hqrrp_batch_phase_name(int phase);

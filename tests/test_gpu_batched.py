@@ -193,3 +193,18 @@ def test_batched_graph_capture(handle):
             for u, v in zip(x, y):
                 assert np.array_equal(u, v), rep
     plan.close()
+
+
+def test_batched_autotune(handle):
+    """The autotuner fills every bucket with a candidate block size, and a
+    plan without explicit block sizes takes them from the table."""
+    import hqrrp_amd
+
+    table = handle.batch_tune(pp=8, panel_pivoting=1)
+    assert set(np.unique(table)) <= {16, 32, 64}
+    m = [64, 300, 512, 100]
+    n = [32, 250, 512, 100]
+    plan = hqrrp_amd.BatchPlan(handle, m, n, pp=8)
+    want = [table[(mi - 1) // 64, (ni - 1) // 64] for mi, ni in zip(m, n)]
+    assert plan.nb.tolist() == want
+    plan.close()
