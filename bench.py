@@ -245,6 +245,65 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args):
+    """QR at fixed pivots of a 4096 x 1024 Gaussian truncated to k = 256
+    (Q, R) and its adjoint with Gaussian cotangents: ms per forward and per
+    backward, each timed with HIP events over the steps."""
+    import torch
+
+    import hqrrp_amd
+
+    m, n, k = 4096, 1024, 256
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(m, n, dtype=torch.float64, generator=g).cuda()
+    p = (torch.randperm(n, generator=g).to(torch.int32) + 1).cuda()
+    gQ = torch.randn(m, k, dtype=torch.float64, generator=g).cuda()
+    gR = torch.randn(k, n, dtype=torch.float64, generator=g).cuda()
+    h = hqrrp_amd.Handle(0)
+    h.use_stream(torch.cuda.current_stream())
+    for _ in range(args.warmup):
+        Q, R = hqrrp_amd.qr_fixed(h, A, p, k)
+        hqrrp_amd.qr_fixed_adjoint(h, A, p, k, Q, R, gQ, gR)
+    torch.cuda.synchronize()
+    f_ms = b_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        Q, R = hqrrp_amd.qr_fixed(h, A, p, k)
+        e[1].record()
+        gA = hqrrp_amd.qr_fixed_adjoint(h, A, p, k, Q, R, gQ, gR)
+        e[2].record()
+        torch.cuda.synchronize()
+        f_ms += e[0].elapsed_time(e[1])
+        b_ms += e[1].elapsed_time(e[2])
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    f_ms /= args.steps
+    b_ms /= args.steps
+    assert torch.isfinite(gA).all()
+    h.close()
+    # forward: QR of the k pivoted columns, Q' applied to the other n-k, Q
+    # formed; backward: the products of the adjoint formula
+    fl_f = (2.0 * m * k * k - 2.0 / 3.0 * k ** 3 + 4.0 * m * k * (n - k)
+            + 4.0 * m * k * k)
+    fl_b = 2.0 * m * k * (n - k) * 2 + 2.0 * k * k * n + 2.0 * m * k * k * 3
+    line = {
+        "metric": "fixed-pivot QR + adjoint ms (4096x1024, k=256)",
+        "value": round(ms, 3), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "QR at fixed pivots 4096x1024 Gaussian, k=256, "
+                               "Gaussian cotangents",
+                   "m": m, "n": n, "k": k},
+        "forward_ms": round(f_ms, 3), "backward_ms": round(b_ms, 3),
+        "forward_gflops": round(fl_f / (f_ms / 1e3) / 1e9, 1),
+        "backward_gflops": round(fl_b / (b_ms / 1e3) / 1e9, 1),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,7 +329,7 @@ def main():
                     default="fp64",
                     help="sketch precision (panel, T and trailing update are "
                          "FP64 in every case)")
-    ap.add_argument("--workload", choices=("single", "batched"),
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint"),
                     default="single",
                     help="single: one large HQRRP (the headline); batched: "
                          "many small matrices in one launch")
@@ -305,6 +364,9 @@ def main():
 
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
 
     import numpy as np
