@@ -528,22 +528,47 @@ int apply_qt_left(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
 
 // Sketch downdate with the old G_R = [G1 G2]:  P = G_R V, Q = P T,
 // G_R -= Q V' (whose first b columns are the reference's B matrix), then
-// Y2 -= G1_new R12.
+// Y2 -= G1_new R12 (four launches, k_downdate.hip).
 int downdate_y(hqrrp_handle_t h, int r, int b, int mU, int nY2, const double* U,
                long ldu, const double* R12, long ldr, const double* T, long ldt,
                double* Y2, long ldy, double* G, long ldg) {
   HQ_RET(ensure(h->Pp, sizeof(double) * (size_t)r * b));
   HQ_RET(ensure(h->Qm, sizeof(double) * (size_t)r * b));
-  double* P = h->Pp.as<double>();
-  double* Q = h->Qm.as<double>();
-  HQ_RET(gemm(h, false, false, r, b, mU, 1.0, G, ldg, 0, U, ldu, 1, 0.0, P, r));
-  HQ_RET(gemm(h, false, false, r, b, b, 1.0, P, r, 0, T, ldt, 0, 0.0, Q, r,
-              false));
-  HQ_RET(gemm(h, false, true, r, mU, b, -1.0, Q, r, 0, U, ldu, 1, 1.0, G, ldg,
-              false));
-  if (nY2 > 0)
-    HQ_RET(gemm(h, false, false, r, nY2, b, -1.0, G, ldg, 0, R12, ldr, 0, 1.0,
-                Y2, ldy, false));
+  if (r > 80 || b > 64) {  // beyond the fused kernels: generic products
+    double* P = h->Pp.as<double>();
+    double* Q = h->Qm.as<double>();
+    HQ_RET(gemm(h, false, false, r, b, mU, 1.0, G, ldg, 0, U, ldu, 1, 0.0, P,
+                r));
+    HQ_RET(gemm(h, false, false, r, b, b, 1.0, P, r, 0, T, ldt, 0, 0.0, Q, r,
+                false));
+    HQ_RET(gemm(h, false, true, r, mU, b, -1.0, Q, r, 0, U, ldu, 1, 1.0, G,
+                ldg, false));
+    if (nY2 > 0)
+      HQ_RET(gemm(h, false, false, r, nY2, b, -1.0, G, ldg, 0, R12, ldr, 0,
+                  1.0, Y2, ldy, false));
+    return HQRRP_OK;
+  }
+  DowndateArgs d{};
+  d.S = downdate_slices(mU, &d.kc);
+  HQ_RET(ensure(h->split, sizeof(double) * (size_t)d.S * r * b));
+  d.r = r;
+  d.b = b;
+  d.mU = mU;
+  d.nY2 = std::max(nY2, 0);
+  d.U = U;
+  d.ldu = ldu;
+  d.T = T;
+  d.ldt = ldt;
+  d.R12 = R12;
+  d.ldr = ldr;
+  d.G = G;
+  d.ldg = ldg;
+  d.Y2 = Y2;
+  d.ldy = ldy;
+  d.part = h->split.as<double>();
+  d.P = h->Pp.as<double>();
+  d.Q = h->Qm.as<double>();
+  HQ_TRY(launch_downdate(d, h->cur));
   return HQRRP_OK;
 }
 

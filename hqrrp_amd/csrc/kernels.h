@@ -68,6 +68,30 @@ struct TrailArgs {
   long kchunk;   // rows of C per split
 };
 
+// k_downdate.hip: FP64 sketch downdate of one block (r <= 80, b <= 64).
+struct DowndateArgs {
+  int r, b, mU, nY2;
+  const double* U;  // mU x b reflectors (unit lower trapezoidal), ld ldu
+  long ldu;
+  const double* T;  // b x b upper triangular, ld ldt
+  long ldt;
+  const double* R12;  // b x nY2, ld ldr
+  long ldr;
+  double* G;  // r x mU, ld ldg (updated)
+  long ldg;
+  double* Y2;  // r x nY2, ld ldy (updated)
+  long ldy;
+  double* part;  // S slices of r x b
+  double* P;     // r x b
+  double* Q;     // r x b
+  int S;         // row slices of U (downdate_slices)
+  long kc;       // rows per slice
+  int ng;        // set by launch_downdate
+};
+// Slices and rows per slice for mU rows of U.
+int downdate_slices(long mU, long* kc);
+hipError_t launch_downdate(DowndateArgs a, hipStream_t s);
+
 // k_batch.hip: batched HQRRP of small matrices (m, n <= HQB_DIM_MAX;
 // nb <= 64; pp <= HQB_PP_MAX).
 #define HQB_PP_MAX 10
