@@ -22,7 +22,6 @@
 namespace {
 
 constexpr int kTile = 64;  // output tile edge (rows and columns)
-constexpr int kBK = 16;    // K rows staged per step in trail_vtc
 constexpr int kBK64 = 32;  // K rows staged per step in trail_vtc64
 
 __device__ __forceinline__ double vmask(double v, long i, long c) {
@@ -158,26 +157,41 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 
 // part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
 // computed transposed (C' V) so the accumulator stores whole columns of the
-// partial.  A workgroup owns 256 columns of C (wavefront w: columns
-// 64w..64w+63, against all 64 reflectors: 4 x 4 MFMA blocks), so each V
-// slice staged in LDS serves 256 columns and C streams through once in
-// whole 128-byte lines.  K rows are staged kBK at a time (LDS pitch kBK+1:
-// conflict-free fragment reads); the next slice is loaded into registers
-// while the matrix cores work.
-constexpr int kVN = 256;       // columns of C per workgroup
-constexpr int kVP = kBK + 1;   // LDS pitch (doubles)
+// partial.  A workgroup owns 256 columns of C; wavefront w streams its own
+// 64 columns (against all 64 reflectors: 4 x 4 MFMA blocks, accumulators in
+// AGPRs) with no LDS and no barrier: both operands go from memory straight
+// into MFMA operand registers.  K is permuted inside each 16-row chunk: in
+// MFMA substep e (0..3) lane l supplies row 4*(l/16) + e of the chunk for
+// both operands, so every lane reads 4 consecutive doubles (32 bytes) of one
+// column of C and of V per chunk, and 16 lanes cover a column's 128-byte
+// line.  A two-chunk register ring keeps the next chunk's loads in flight
+// while the matrix cores work on the current one.  VEC: the 32-byte groups are
+// 32-byte aligned (vector loads); otherwise the same values are loaded one
+// double at a time, so both forms compute identical bits.
+constexpr int kVN = 256;  // columns of C per workgroup
+constexpr int kCK = 16;   // K rows per chunk
+template <bool VEC>
 __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
-  __shared__ double Vs[kTile][kVP];  // Vs[p][k] = V(k0+k, p)
-  __shared__ double Cs[kVN][kVP];    // Cs[n][k] = C(k0+k, n0+n)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const long n0 = (long)blockIdx.x * kVN;
+  const int l15 = lane & 15, g4 = lane >> 4;
+  const long n0 = (long)blockIdx.x * kVN + wave * 64;
   const int s = blockIdx.y;
   const long kbeg = (long)s * a.kchunk;
   const long kend = min((long)a.mrows, kbeg + a.kchunk);
   const int b = a.b;
-  constexpr int kPerV = kBK * kTile / HQ_NT;  // V elements per thread
-  constexpr int kPerC = kBK * kVN / HQ_NT;    // C elements per thread
-  const int l15 = lane & 15, l4 = lane >> 4;
+
+  const double* cp[4];
+  const double* vp[4];
+  bool cok[4], vok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long gn = n0 + 16 * i + l15;
+    cok[i] = gn < a.nc;
+    cp[i] = a.C + (cok[i] ? gn : 0) * a.ldc + 4 * g4;
+    const int p = 16 * i + l15;
+    vok[i] = p < b;
+    vp[i] = a.V + (long)(vok[i] ? p : 0) * a.ldv + 4 * g4;
+  }
 
   hq_d4 acc[4][4];  // [n block][p block]
 #pragma unroll
@@ -185,60 +199,69 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = hq_d4{0.0, 0.0, 0.0, 0.0};
 
-  double rv[kPerV], rc[kPerC];
-  auto load = [&](long k0) {
-#pragma unroll
-    for (int q = 0; q < kPerV; ++q) {
-      const int idx = t + HQ_NT * q, k = idx % kBK, p = idx / kBK;
-      const long gk = k0 + k;
-      const double vv = (gk < kend && p < b) ? a.V[gk + (long)p * a.ldv] : 0.0;
-      rv[q] = a.vmask ? vmask(vv, gk, p) : vv;
-    }
-#pragma unroll
-    for (int q = 0; q < kPerC; ++q) {
-      const int idx = t + HQ_NT * q, k = idx % kBK, n = idx / kBK;
-      const long gk = k0 + k, gn = n0 + n;
-      rc[q] = (gk < kend && gn < a.nc) ? a.C[gk + gn * a.ldc] : 0.0;
-    }
-  };
   // The K slice is walked starting at a per-column-tile rotation, so that
   // concurrently running workgroups read different rows (spreading the
   // power-of-two-strided columns over HBM channels).
-  const long nsteps = (kend - kbeg + kBK - 1) / kBK;
+  const long nsteps = (kend - kbeg + kCK - 1) / kCK;
   const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
-  auto kstart = [&](long it) { return kbeg + ((it + rot) % nsteps) * kBK; };
-  if (nsteps > 0) load(kstart(0));
-  for (long it = 0; it < nsteps; ++it) {
+
+  hq_d4 cb[2][4], vb[2][4];
+  auto load = [&](hq_d4 (&cr)[4], hq_d4 (&vr)[4], long it) {
+    if (it >= nsteps) {
 #pragma unroll
-    for (int q = 0; q < kPerV; ++q) {
-      const int idx = t + HQ_NT * q;
-      Vs[idx / kBK][idx % kBK] = rv[q];
+      for (int i = 0; i < 4; ++i) {
+        cr[i] = hq_d4{0.0, 0.0, 0.0, 0.0};
+        vr[i] = hq_d4{0.0, 0.0, 0.0, 0.0};
+      }
+      return;
     }
+    const long k0 = kbeg + ((it + rot) % nsteps) * kCK;
+    const long r0 = k0 + 4 * g4;  // this lane's first row
+    if (VEC && k0 + kCK <= kend) {
 #pragma unroll
-    for (int q = 0; q < kPerC; ++q) {
-      const int idx = t + HQ_NT * q;
-      Cs[idx / kBK][idx % kBK] = rc[q];
+      for (int i = 0; i < 4; ++i) {
+        cr[i] = cok[i] ? *reinterpret_cast<const hq_d4*>(cp[i] + k0)
+                       : hq_d4{0.0, 0.0, 0.0, 0.0};
+        vr[i] = vok[i] ? *reinterpret_cast<const hq_d4*>(vp[i] + k0)
+                       : hq_d4{0.0, 0.0, 0.0, 0.0};
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool rok = r0 + e < kend;
+          cr[i][e] = (cok[i] && rok) ? cp[i][k0 + e] : 0.0;
+          vr[i][e] = (vok[i] && rok) ? vp[i][k0 + e] : 0.0;
+        }
     }
-    __syncthreads();
-    if (it + 1 < nsteps) load(kstart(it + 1));
+    if (a.vmask && k0 < b) {  // the top b x b block of V is unit lower
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
-      double af[4], bf[4];  // A = C' (n x k), B = V (k x p)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = Cs[wave * 64 + i * 16 + l15][kk + l4];
+        for (int e = 0; e < 4; ++e) vr[i][e] = vmask(vr[i][e], r0 + e, 16 * i + l15);
+    }
+  };
+  auto compute = [&](const hq_d4 (&cr)[4], const hq_d4 (&vr)[4]) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Vs[j * 16 + l15][kk + l4];
+    for (int e = 0; e < 4; ++e)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[i][e], vr[j][e],
                                                             acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
+  };
+  load(cb[0], vb[0], 0);
+  for (long it = 0; it < nsteps; it += 2) {
+    load(cb[1], vb[1], it + 1);
+    compute(cb[0], vb[0]);
+    if (it + 1 >= nsteps) break;
+    load(cb[0], vb[0], it + 2);
+    compute(cb[1], vb[1]);
   }
   // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
-  // n = n0 + 64*wave + 16i + l/16 + 4r.
+  // n = n0 + 16i + l/16 + 4r.
   double* out = a.part + (long)s * b * a.nc;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -247,7 +270,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int p = j * 16 + l15;
-        const long gn = n0 + wave * 64 + i * 16 + l4 + 4 * r;
+        const long gn = n0 + i * 16 + g4 + 4 * r;
         if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
       }
 }
@@ -381,7 +404,16 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
   if (a.wide) {
     dim3 grid(hq_cdiv(a.nc, kVN), a.splits);
-    hipLaunchKernelGGL(trail_vtc_kernel, grid, dim3(HQ_NT), 0, s, a);
+    // 32-byte groups of rows k0 + 4g (k0 a multiple of 16 from the slice
+    // start, which is a multiple of trail_kbk()) are 32-byte aligned when
+    // both bases and leading dimensions are.
+    const bool vec = (reinterpret_cast<uintptr_t>(a.C) % 32 == 0) &&
+                     (reinterpret_cast<uintptr_t>(a.V) % 32 == 0) &&
+                     a.ldc % 4 == 0 && a.ldv % 4 == 0 && a.kchunk % 4 == 0;
+    if (vec)
+      hipLaunchKernelGGL(trail_vtc_kernel<true>, grid, dim3(HQ_NT), 0, s, a);
+    else
+      hipLaunchKernelGGL(trail_vtc_kernel<false>, grid, dim3(HQ_NT), 0, s, a);
   } else {
     dim3 grid(hq_cdiv(a.nc, kTile), a.splits);
     hipLaunchKernelGGL(trail_vtc64_kernel, grid, dim3(HQ_NT), 0, s, a);
