@@ -169,7 +169,7 @@ struct hqrrp_handle_s {
   DevBuf words;  // [8] error word of the persistent kernels
   DevBuf Y, G, Gcache, moves, piv, gram, T, perm;
   Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
-  DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
+  DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm, ddpart;
   // Sketch precision of the running factorisation: Y and G hold floats
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
@@ -496,13 +496,17 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   return HQRRP_OK;
 }
 
+int trail_vtc_only(hqrrp_handle_t h, const TrailArgs& a) {
+  const double mn = (double)a.mrows * a.nc, b = a.b;
+  PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * a.mrows * b,
+                1);
+  HQ_TRY(launch_trail_vtc(a, h->cur));
+  return HQRRP_OK;
+}
+
 int trail_w(hqrrp_handle_t h, const TrailArgs& a) {
-  const double mn = (double)a.mrows * a.nc, b = a.b, S = a.splits;
-  {
-    PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * a.mrows * b,
-                  1);
-    HQ_TRY(launch_trail_vtc(a, h->cur));
-  }
+  const double b = a.b, S = a.splits;
+  HQ_RET(trail_vtc_only(h, a));
   PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * a.nc, 8.0 * (S + 1.0) * b * a.nc,
                 1);
   HQ_TRY(launch_trail_tw(a, h->cur));
@@ -510,6 +514,7 @@ int trail_w(hqrrp_handle_t h, const TrailArgs& a) {
 }
 
 int trail_apply(hqrrp_handle_t h, const TrailArgs& a) {
+  if (a.mrows <= 0 || a.nc <= 0) return HQRRP_OK;
   // C is read and written once; V and W are small.
   const double mn = (double)a.mrows * a.nc;
   PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * a.b * mn, 16.0 * mn, 1);
@@ -677,6 +682,92 @@ int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
                     static_cast<double*>(G) + (long)j * r, r);
 }
 
+// FP64 sketch downdate, split for the block loop (NoFLA_Downdate_Y,
+// NoFLA_HQRRP_WY_blk_var4.c:617-708, and the G update 743-771):
+//   block_qb (needs only the block's reflectors U and T, so it runs right
+//   after the panel): P = G U, Q = P T, B = G1 - Q U1' into G(:, j:j+b);
+//   the rest of the G update, G2 -= Q U2', is deferred to the next
+//   block_qb, which applies it to each slice of G as it reads it;
+//   block_tail: W' for the trailing update, R12 into A and Y2 -= B R12.
+struct PendingG {
+  const double* U = nullptr;  // the previous block's U2 (rows aligned with
+  long ldu = 0;               // the next block's U), or null
+  int b = 0;
+};
+
+int block_qb(hqrrp_handle_t h, int r, int j, int b, int mU, const double* U,
+             long ldu, long ldt, double* G, PendingG* pend) {
+  PhaseScope ps(h, PH_DOWNDATE, 4.0 * r * b * (double)mU,
+                8.0 * (2.0 * r * mU + (double)mU * b), 3);
+  HQ_RET(ensure(h->Pp, sizeof(double) * (size_t)r * 64));
+  HQ_RET(ensure(h->Qm, sizeof(double) * (size_t)r * 64));
+  DowndateArgs d{};
+  d.S = downdate_slices(mU, &d.kc);
+  HQ_RET(ensure(h->ddpart, sizeof(double) * (size_t)d.S * r * b));
+  d.r = r;
+  d.b = b;
+  d.mU = mU;
+  d.U = U;
+  d.ldu = ldu;
+  d.T = h->T.as<double>();
+  d.ldt = ldt;
+  d.G = G + (long)j * r;
+  d.ldg = r;
+  d.part = h->ddpart.as<double>();
+  d.P = h->Pp.as<double>();
+  d.Q = h->Qm.as<double>();
+  if (pend->U) {
+    d.Qprev = h->Qm.as<double>();
+    d.Uprev = pend->U;
+    d.ldup = pend->ldu;
+    d.bprev = pend->b;
+  }
+  HQ_TRY(launch_downdate_qb(d, h->cur));
+  return HQRRP_OK;
+}
+
+int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int r, int j, int jn,
+               bool sketch, double* dA, long lda, double* Y, double* G) {
+  const double b = ta.b, nc = ta.nc;
+  PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc * 2.0 + 2.0 * r * b * nc,
+                8.0 * nc * (b * ta.splits + 2.0 * b + (sketch ? 2.0 * r : 0.0)),
+                1);
+  TailArgs t{};
+  t.b = ta.b;
+  t.nc = ta.nc;
+  t.r = r;
+  t.S = ta.splits;
+  t.ny = sketch ? ta.nc : 0;
+  t.part = ta.part;
+  t.T = ta.T;
+  t.ldt = ta.ldt;
+  t.V = ta.V;
+  t.ldv = ta.ldv;
+  t.C = ta.C;
+  t.ldc = ta.ldc;
+  t.R = dA + j + (long)jn * lda;
+  t.ldr = lda;
+  t.W = ta.W;
+  t.ldw = ta.ldw;
+  t.B = G + (long)j * r;
+  t.ldb = r;
+  t.Y2 = Y + (long)jn * r;
+  t.ldy = r;
+  HQ_TRY(launch_trail_tail(t, h->cur));
+  return HQRRP_OK;
+}
+
+// The trailing update below row block j (R12 is already in A).
+TrailArgs below_r12(const TrailArgs& ta) {
+  TrailArgs u = ta;
+  u.mrows -= ta.b;
+  u.V += ta.b;
+  u.vmask = 0;
+  u.C += ta.b;
+  if (u.Cout) u.Cout += ta.b;
+  return u;
+}
+
 // The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.
 //
 // Look-ahead: once block j's W' = (T'V'C)' is known, the R12 rows (a copy
@@ -758,6 +849,13 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                   dA, lda, 0, 0.0, static_cast<double*>(Y), r));
   }
 
+  // FP64 sketch: the downdate runs split (block_qb after the panel,
+  // block_tail after V'C) with the G update deferred; the FP32 sketch keeps
+  // the four products of downdate_y32 after the trailing update.
+  const bool fused = !h->sk32;
+  PendingG pend;
+  double* Gd = static_cast<double*>(G);
+  double* Yd = static_cast<double*>(Y);
   bool have_panel = false;  // block j's panel already factored by look-ahead
   for (int j = 0; j < kmax; j += nb) {
     const int b = std::min(nb, std::min(n - j, m - j));
@@ -772,6 +870,10 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
                        h->perm.as<int>(), T, nb));
       HQ_RET(block_permute(h, j, b, dA, lda, djpvt));
+      if (fused && !last && nc > 0) {
+        HQ_RET(block_qb(h, r, j, b, mrows, Ajj, lda, nb, Gd, &pend));
+        pend = PendingG{Ajj + b, lda, b};
+      }
     }
     have_panel = false;
     if (nc <= 0) continue;
@@ -794,12 +896,21 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     double* C = X[xc] + j + (long)jn * ldx[xc];
     TrailArgs ta;
     HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta));
-    HQ_RET(trail_w(h, ta));
+    if (fused) {
+      HQ_RET(trail_vtc_only(h, ta));
+      HQ_RET(block_tail(h, ta, r, j, jn, !last, dA, lda, Yd, Gd));
+    } else {
+      HQ_RET(trail_w(h, ta));
+    }
     if (!la) {
-      HQ_RET(trail_apply(h, ta));
-      if (!last)
-        HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, C, lda, Y, G,
-                              nb));
+      if (fused) {
+        HQ_RET(trail_apply(h, below_r12(ta)));
+      } else {
+        HQ_RET(trail_apply(h, ta));
+        if (!last)
+          HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, C, lda, Y, G,
+                                nb));
+      }
       continue;
     }
 
@@ -810,29 +921,34 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     const int mnext = m - jn;
     const int b2 = std::min(nb, std::min(n - jn, m - jn));
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
-    // R12 rows and the sketch downdate on the whole chip (split-K GEMMs).
-    {
-      PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)b * nc, 1);
-      HQ_TRY(launch_copy_block(R12, b, C, ldx[xc], b, nc, h->cur));
+    if (!fused) {
+      // R12 rows and the sketch downdate on the whole chip.
+      {
+        PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)b * nc, 1);
+        HQ_TRY(launch_copy_block(R12, b, C, ldx[xc], b, nc, h->cur));
+      }
+      {
+        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
+                      16.0 * (double)b * nc, 1);
+        TrailArgs tr = ta;
+        tr.mrows = b;
+        tr.C = R12;
+        tr.ldc = b;
+        HQ_TRY(launch_trail_update(tr, h->cur));
+      }
+      HQ_RET(
+          block_downdate(h, r, j, b, mrows, nc, Ajj, lda, R12, b, Y, G, nb));
     }
-    {
-      PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
-                    16.0 * (double)b * nc, 1);
-      TrailArgs tr = ta;
-      tr.mrows = b;
-      tr.C = R12;
-      tr.ldc = b;
-      HQ_TRY(launch_trail_update(tr, h->cur));
-    }
-    HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, R12, b, Y, G, nb));
     ta.Cout = X[xn] + j + (long)jn * ldx[xn];
     ta.ldco = ldx[xn];
+    // (FP64 sketch: R12 is already in A, the update covers the rows below.)
+    const TrailArgs tu = fused ? below_r12(ta) : ta;
     // Fork: block j's update || block jn's pivot chain.
     HQ_TRY(hipEventRecord(h->ev_fork, main_s));
     HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_fork, 0));
     HQ_TRY(hipStreamWaitEvent(h->s_chain, h->ev_fork, 0));
     h->cur = h->s_upd;
-    int rc = trail_apply(h, ta);
+    int rc = trail_apply(h, tu);
     if (rc == HQRRP_OK && hipEventRecord(h->ev_upd, h->s_upd) != hipSuccess)
       rc = HQRRP_ERR_HIP;
     h->cur = h->s_chain;
@@ -861,6 +977,11 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       }
       HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
                        h->perm.as<int>(), T, nb));
+      if (fused && !last_next) {
+        HQ_RET(block_qb(h, r, jn, b2, mnext, Pn, mnext, nb, Gd, &pend));
+        // Block jn's reflectors below its row block, in A after the join.
+        pend = PendingG{dA + (jn + b2) + (long)jn * lda, lda, b2};
+      }
       HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
       return HQRRP_OK;
     };
@@ -872,7 +993,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     // Join: the R12 rows into A (when the update wrote them elsewhere), block
     // jn's columns into place (rows above jn in A, the rest in the trailing
     // buffer), its factored panel over them, and the row permutation.
-    if (xn != 0) {
+    if (xn != 0 && !fused) {
       PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)b * nc, 1);
       HQ_TRY(launch_copy_block(dA + j + (long)jn * lda, lda, R12, b, b, nc,
                                h->cur));
@@ -1129,6 +1250,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
+                    &h->ddpart,
                     &h->hA,    &h->hJ,    &h->hT,     &h->hW,     &h->R12,
                     &h->Pn,    &h->Wsel,  &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};

@@ -401,6 +401,150 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
 
 }  // namespace
 
+// The block tail of the HQRRP driver, per 64 columns of the trailing
+// matrix (one workgroup each), after the split-K partials of M = V'C:
+//   W   = T' sum_s part[s]          (written as W' for the trailing update)
+//   R12 = C(0:b, :) - V1 W          (V1: the top b x b block of V, unit
+//                                    lower; R12 is row block j of R)
+//   Y2 -= B R12                     (the sketch downdate, B = G(:, j:j+b)
+//                                    after the block's G update)
+// Each product is 64 x 64 x 64 (r x 64 x 64 for Y2) on the matrix cores
+// with both operands in LDS; the three depend on each other, so they share
+// one launch.
+namespace {
+constexpr int kTP = 65;  // LDS pitch of the 64-wide tiles
+constexpr int kTB = 81;  // LDS pitch of B' (r <= 80 rows)
+__global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
+  __shared__ double R1[64 * kTB];      // T (pitch kTP), then B' (pitch kTB)
+  __shared__ double Ms[64][kTP];       // M(p, c), then R12(i, c)
+  __shared__ double V1s[64][kTP];      // V1(i, p)
+  __shared__ double Ws[64][kTP];       // W(q, c)
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l15 = lane & 15, l4 = lane >> 4;
+  const int b = a.b, r = a.r;
+  const long c0 = (long)blockIdx.x * 64;
+  const long slice = (long)b * a.nc;
+  // Loads: partial slices (fixed order), T, V1, and B into registers.
+  for (int idx = t; idx < 64 * 64; idx += HQ_NT) {
+    const int p = idx & 63, c = idx >> 6;
+    const long gc = c0 + c;
+    double m = 0.0;
+    if (p < b && gc < a.nc) {
+      const double* src = a.part + p + gc * b;
+      for (int q = 0; q < a.S; ++q) m += src[q * slice];
+    }
+    Ms[p][c] = m;
+    R1[p * kTP + c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
+    // idx -> (i = p, col = c) of V1
+    V1s[p][c] = (p < b && c < b) ? vmask(a.V[p + (long)c * a.ldv], p, c) : 0.0;
+  }
+  constexpr int kBPer = 64 * 80 / HQ_NT;
+  double breg[kBPer];
+#pragma unroll
+  for (int q = 0; q < kBPer; ++q) {
+    const int idx = t + HQ_NT * q, i = idx % 80, p = idx / 80;
+    breg[q] = (a.ny > 0 && i < r && p < b) ? -a.B[i + (long)p * a.ldb] : 0.0;
+  }
+  __syncthreads();
+  const int cw = wave * 16 + l15;  // this lane's column of the tile
+  const long gc = c0 + cw;
+  const bool cok = gc < a.nc;
+  // W = T' M: A = T' (q x p) from T(p, q), B = M.
+  {
+    hq_d4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const double bf = Ms[kk + l4][cw];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            R1[(kk + l4) * kTP + q * 16 + l15], bf, acc[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int qq = q * 16 + l4 + 4 * e;
+        Ws[qq][cw] = acc[q][e];
+        if (cok && qq < b) a.W[gc + (long)qq * a.ldw] = acc[q][e];
+      }
+  }
+  __syncthreads();  // T and M consumed; W complete
+#pragma unroll
+  for (int q = 0; q < kBPer; ++q) {
+    const int idx = t + HQ_NT * q, i = idx % 80, p = idx / 80;
+    R1[p * kTB + i] = breg[q];
+  }
+  // R12 = C_top - V1 W: A = -V1 (i x p), B = W; C_top straight into the
+  // accumulators.
+  {
+    hq_d4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        acc[q][e] = (cok && i < b) ? a.C[i + gc * a.ldc] : 0.0;
+      }
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const double bf = Ws[kk + l4][cw];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-V1s[q * 16 + l15][kk + l4],
+                                                      bf, acc[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        Ms[i][cw] = acc[q][e];
+        if (cok && i < b) a.R[i + gc * a.ldr] = acc[q][e];
+      }
+  }
+  if (a.ny <= 0) return;
+  __syncthreads();  // R12 and B' complete
+  // Y2 -= B R12: A = -B (i x p) from B', B = R12; Y2 into the accumulators.
+  {
+    const bool yok = gc < a.ny;
+    hq_d4 acc[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        acc[q][e] = (yok && i < r) ? a.Y2[i + gc * a.ldy] : 0.0;
+      }
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const double bf = Ms[kk + l4][cw];
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            R1[(kk + l4) * kTB + q * 16 + l15], bf, acc[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        if (yok && i < r) a.Y2[i + gc * a.ldy] = acc[q][e];
+      }
+  }
+}
+}  // namespace
+
+hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s) {
+  if (a.b < 1 || a.b > 64 || a.r > 80 || a.nc < 1 || a.S < 1)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.nc, 64)), dim3(HQ_NT),
+                     0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
   if (a.wide) {
     dim3 grid(hq_cdiv(a.nc, kVN), a.splits);

@@ -87,10 +87,18 @@ struct DowndateArgs {
   int S;         // row slices of U (downdate_slices)
   long kc;       // rows per slice
   int ng;        // set by launch_downdate
+  // Deferred G update of the previous block (launch_downdate_qb only):
+  // G -= Qprev Uprev' before P is formed; Qprev r x bprev (ld r), Uprev
+  // mU x bprev (ld ldup, rows aligned with U's).  Null: none.
+  const double* Qprev;
+  const double* Uprev;
+  long ldup;
+  int bprev;
 };
 // Slices and rows per slice for mU rows of U.
 int downdate_slices(long mU, long* kc);
 hipError_t launch_downdate(DowndateArgs a, hipStream_t s);
+hipError_t launch_downdate_qb(DowndateArgs a, hipStream_t s);
 
 // k_batch.hip: batched HQRRP of small matrices (m, n <= HQB_DIM_MAX;
 // nb <= 64; pp <= HQB_PP_MAX).
@@ -137,6 +145,27 @@ hipError_t launch_batch_hqrrp(const BatchArgs& a, int class_nb, int wgs,
                               hipStream_t s);
 
 // k_trail.hip
+// Block tail of the HQRRP driver (see trail_tail_kernel).
+struct TailArgs {
+  int b, nc, r, S;
+  int ny;              // columns of Y2 to downdate (0: none)
+  const double* part;  // S slices of b x nc (trail_vtc's partials)
+  const double* T;     // b x b upper triangular, ld ldt
+  long ldt;
+  const double* V;  // the block's reflectors (top b x b read unit lower)
+  long ldv;
+  const double* C;  // rows 0..b-1 of the trailing matrix, ld ldc
+  long ldc;
+  double* R;  // R12 out (b x nc), ld ldr (may equal C)
+  long ldr;
+  double* W;  // W' out (nc x b), ld ldw
+  long ldw;
+  const double* B;  // r x b, ld ldb
+  long ldb;
+  double* Y2;  // r x ny, ld ldy
+  long ldy;
+};
+hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s);
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
