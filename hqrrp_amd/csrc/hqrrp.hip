@@ -169,7 +169,7 @@ struct hqrrp_handle_s {
   DevBuf words;  // [8] error word of the persistent kernels
   DevBuf Y, G, Gcache, moves, piv, gram, T, perm;
   Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
-  DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm, ddpart;
+  DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
   // Sketch precision of the running factorisation: Y and G hold floats
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
@@ -458,11 +458,20 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
 // stream of its own.
 int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
                   long ldv, const double* T, long ldt, double* C, long ldc,
-                  TrailArgs* out) {
+                  TrailArgs* out, double* C2 = nullptr, long ldc2 = 0,
+                  int nc2 = 0) {
   TrailArgs a{};
   a.mrows = mrows;
   a.nc = nc;
   a.b = b;
+  // Optional second column range (updated in place), from virtual column
+  // nca on: tiles of every kernel (64, 128, 256 columns) never straddle it.
+  a.C2 = (C2 && nc2 > 0) ? C2 : nullptr;
+  a.ldc2 = ldc2;
+  a.nc2 = a.C2 ? nc2 : 0;
+  a.nca = a.C2 ? (int)(hq_cdiv(nc, 256) * 256L) : nc;
+  a.ncv = a.C2 ? a.nca + a.nc2 : nc;
+  nc = a.ncv;  // tiles, partials and W' below count virtual columns
   a.vmask = 1;
   a.V = V;
   a.ldv = ldv;
@@ -607,8 +616,8 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
       HQ_TRY(launch_cast_f32(o.omega, h->G.as<float>(), (long)r * m, h->cur));
       return HQRRP_OK;
     }
-    HQ_TRY(hipMemcpyAsync(h->G.p, o.omega, sizeof(double) * (size_t)r * m,
-                          hipMemcpyDeviceToDevice, h->cur));
+    // The FP64 driver keeps Omega' (m x r, ld m).
+    HQ_TRY(launch_transpose(o.omega, r, r, m, h->G.as<double>(), m, h->cur));
     return HQRRP_OK;
   }
   if (h->om_r != r || h->om_m != m || h->om_seed != o.seed) {
@@ -626,8 +635,8 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
                            (long)r * m, h->cur));
     return HQRRP_OK;
   }
-  HQ_TRY(hipMemcpyAsync(h->G.p, h->Gcache.p, sizeof(double) * (size_t)r * m,
-                        hipMemcpyDeviceToDevice, h->cur));
+  HQ_TRY(launch_transpose(h->Gcache.as<double>(), r, r, m, h->G.as<double>(),
+                          m, h->cur));
   return HQRRP_OK;
 }
 
@@ -666,78 +675,41 @@ int block_permute(hqrrp_handle_t h, int j, int b, double* dA, long lda,
   return HQRRP_OK;
 }
 
+// The FP32 sketch's downdate (G: r x m floats), after the trailing update.
 int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
                    const double* V, long ldv, const double* R12, long ldr,
                    void* Y, void* G, int nb) {
   PhaseScope ps(h, PH_DOWNDATE,
                 4.0 * r * b * (double)mrows + 2.0 * r * b * nc,
-                (h->sk32 ? 4.0 : 8.0) * (2.0 * r * mrows + 2.0 * r * nc), 4);
-  if (h->sk32)
-    return downdate_y32(h, r, b, mrows, nc, V, ldv, R12, ldr,
-                        h->T.as<double>(), nb,
-                        static_cast<float*>(sk_at(h, Y, (long)(j + b) * r)), r,
-                        static_cast<float*>(sk_at(h, G, (long)j * r)), r);
-  return downdate_y(h, r, b, mrows, nc, V, ldv, R12, ldr, h->T.as<double>(), nb,
-                    static_cast<double*>(Y) + (long)(j + b) * r, r,
-                    static_cast<double*>(G) + (long)j * r, r);
+                4.0 * (2.0 * r * mrows + 2.0 * r * nc), 4);
+  return downdate_y32(h, r, b, mrows, nc, V, ldv, R12, ldr, h->T.as<double>(),
+                      nb, static_cast<float*>(sk_at(h, Y, (long)(j + b) * r)),
+                      r, static_cast<float*>(sk_at(h, G, (long)j * r)), r);
 }
 
-// FP64 sketch downdate, split for the block loop (NoFLA_Downdate_Y,
-// NoFLA_HQRRP_WY_blk_var4.c:617-708, and the G update 743-771):
-//   block_qb (needs only the block's reflectors U and T, so it runs right
-//   after the panel): P = G U, Q = P T, B = G1 - Q U1' into G(:, j:j+b);
-//   the rest of the G update, G2 -= Q U2', is deferred to the next
-//   block_qb, which applies it to each slice of G as it reads it;
-//   block_tail: W' for the trailing update, R12 into A and Y2 -= B R12.
-struct PendingG {
-  const double* U = nullptr;  // the previous block's U2 (rows aligned with
-  long ldu = 0;               // the next block's U), or null
-  int b = 0;
-};
-
-int block_qb(hqrrp_handle_t h, int r, int j, int b, int mU, const double* U,
-             long ldu, long ldt, double* G, PendingG* pend) {
-  PhaseScope ps(h, PH_DOWNDATE, 4.0 * r * b * (double)mU,
-                8.0 * (2.0 * r * mU + (double)mU * b), 3);
-  HQ_RET(ensure(h->Pp, sizeof(double) * (size_t)r * 64));
-  HQ_RET(ensure(h->Qm, sizeof(double) * (size_t)r * 64));
-  DowndateArgs d{};
-  d.S = downdate_slices(mU, &d.kc);
-  HQ_RET(ensure(h->ddpart, sizeof(double) * (size_t)d.S * r * b));
-  d.r = r;
-  d.b = b;
-  d.mU = mU;
-  d.U = U;
-  d.ldu = ldu;
-  d.T = h->T.as<double>();
-  d.ldt = ldt;
-  d.G = G + (long)j * r;
-  d.ldg = r;
-  d.part = h->ddpart.as<double>();
-  d.P = h->Pp.as<double>();
-  d.Q = h->Qm.as<double>();
-  if (pend->U) {
-    d.Qprev = h->Qm.as<double>();
-    d.Uprev = pend->U;
-    d.ldup = pend->ldu;
-    d.bprev = pend->b;
-  }
-  HQ_TRY(launch_downdate_qb(d, h->cur));
-  return HQRRP_OK;
-}
-
-int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int r, int j, int jn,
-               bool sketch, double* dA, long lda, double* Y, double* G) {
-  const double b = ta.b, nc = ta.nc;
-  PhaseScope ps(h, PH_TRAIL_TW, 2.0 * b * b * nc * 2.0 + 2.0 * r * b * nc,
-                8.0 * nc * (b * ta.splits + 2.0 * b + (sketch ? 2.0 * r : 0.0)),
-                1);
+// FP64 sketch downdate (NoFLA_Downdate_Y, NoFLA_HQRRP_WY_blk_var4.c:617-708,
+// and the G update 743-771) folded into the trailing update: G' = G_R'
+// (rows j..m-1 of the transposed sketch operator) rides along as r extra
+// columns of the trailing matrix, so V'[C G'] gives P' = (G_R V)' next to
+// M, the tail forms Q' = T'P' next to W and B' = G'(0:b, :) - V1 Q' (the
+// reference's B, stored over G'(0:b, :)) next to R12, and the trailing
+// update applies G' -= V Q' (G_R := G_R (I - V T V')) next to C -= V W.
+// block_tail launches the tail and then Y2 -= B R12.
+int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int j, int jn,
+               double* dA, long lda, double* Y, int r) {
+  const double b = ta.b, nc = ta.ncv;
+  PhaseScope ps(h, PH_TRAIL_TW,
+                4.0 * b * b * nc + (ta.C2 ? 2.0 * r * b * ta.nc : 0.0),
+                8.0 * nc * (b * ta.splits + 2.0 * b) +
+                    (ta.C2 ? 16.0 * r * ta.nc : 0.0),
+                ta.C2 ? 2 : 1);
   TailArgs t{};
   t.b = ta.b;
   t.nc = ta.nc;
-  t.r = r;
   t.S = ta.splits;
-  t.ny = sketch ? ta.nc : 0;
+  t.nc2 = ta.nc2;
+  t.nca = ta.nca;
+  t.ncv = ta.ncv;
   t.part = ta.part;
   t.T = ta.T;
   t.ldt = ta.ldt;
@@ -747,13 +719,14 @@ int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int r, int j, int jn,
   t.ldc = ta.ldc;
   t.R = dA + j + (long)jn * lda;
   t.ldr = lda;
+  t.C2 = ta.C2;
+  t.ldc2 = ta.ldc2;
   t.W = ta.W;
   t.ldw = ta.ldw;
-  t.B = G + (long)j * r;
-  t.ldb = r;
   t.Y2 = Y + (long)jn * r;
   t.ldy = r;
   HQ_TRY(launch_trail_tail(t, h->cur));
+  if (ta.C2) HQ_TRY(launch_trail_y2(t, h->cur));
   return HQRRP_OK;
 }
 
@@ -765,6 +738,7 @@ TrailArgs below_r12(const TrailArgs& ta) {
   u.vmask = 0;
   u.C += ta.b;
   if (u.Cout) u.Cout += ta.b;
+  if (u.C2) u.C2 += ta.b;
   return u;
 }
 
@@ -844,17 +818,17 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     if (h->sk32)
       HQ_RET(gemm32(h, false, false, r, n, m, 1.0f, G, r, 0, 0, dA, lda, 1, 0,
                     0.0f, static_cast<float*>(Y), r, h->skbf));
-    else
-      HQ_RET(gemm(h, false, false, r, n, m, 1.0, static_cast<double*>(G), r, 0,
+    else  // G holds Omega' (m x r)
+      HQ_RET(gemm(h, true, false, r, n, m, 1.0, static_cast<double*>(G), m, 0,
                   dA, lda, 0, 0.0, static_cast<double*>(Y), r));
   }
 
-  // FP64 sketch: the downdate runs split (block_qb after the panel,
-  // block_tail after V'C) with the G update deferred; the FP32 sketch keeps
-  // the four products of downdate_y32 after the trailing update.
+  // FP64 sketch: G' (m x r, ld m) is updated as extra columns of the
+  // trailing matrix and the downdate of Y follows the block tail
+  // (block_tail); the FP32 sketch keeps G (r x m) and the four products of
+  // downdate_y32 after the trailing update.
   const bool fused = !h->sk32;
-  PendingG pend;
-  double* Gd = static_cast<double*>(G);
+  double* Gt = static_cast<double*>(G);
   double* Yd = static_cast<double*>(Y);
   bool have_panel = false;  // block j's panel already factored by look-ahead
   for (int j = 0; j < kmax; j += nb) {
@@ -870,10 +844,6 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
                        h->perm.as<int>(), T, nb));
       HQ_RET(block_permute(h, j, b, dA, lda, djpvt));
-      if (fused && !last && nc > 0) {
-        HQ_RET(block_qb(h, r, j, b, mrows, Ajj, lda, nb, Gd, &pend));
-        pend = PendingG{Ajj + b, lda, b};
-      }
     }
     have_panel = false;
     if (nc <= 0) continue;
@@ -895,10 +865,13 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     }
     double* C = X[xc] + j + (long)jn * ldx[xc];
     TrailArgs ta;
-    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta));
+    // With the FP64 sketch and a next block to pivot, G' rows j.. ride along.
+    const bool gcols = fused && !last;
+    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta,
+                         gcols ? Gt + j : nullptr, m, gcols ? r : 0));
     if (fused) {
       HQ_RET(trail_vtc_only(h, ta));
-      HQ_RET(block_tail(h, ta, r, j, jn, !last, dA, lda, Yd, Gd));
+      HQ_RET(block_tail(h, ta, j, jn, dA, lda, Yd, r));
     } else {
       HQ_RET(trail_w(h, ta));
     }
@@ -966,6 +939,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
         TrailArgs tp = ta;
         tp.mrows = mnext;
         tp.nc = b2;
+        tp.C2 = nullptr;
         tp.vmask = 0;
         tp.V = Ajj + b;
         tp.C = Pn;
@@ -977,11 +951,6 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       }
       HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
                        h->perm.as<int>(), T, nb));
-      if (fused && !last_next) {
-        HQ_RET(block_qb(h, r, jn, b2, mnext, Pn, mnext, nb, Gd, &pend));
-        // Block jn's reflectors below its row block, in A after the join.
-        pend = PendingG{dA + (jn + b2) + (long)jn * lda, lda, b2};
-      }
       HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
       return HQRRP_OK;
     };
@@ -1250,7 +1219,6 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
-                    &h->ddpart,
                     &h->hA,    &h->hJ,    &h->hT,     &h->hW,     &h->R12,
                     &h->Pn,    &h->Wsel,  &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};

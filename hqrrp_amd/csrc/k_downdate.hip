@@ -34,77 +34,36 @@ __device__ __forceinline__ double unit_lower_dd(double v, long i, long c) {
 
 // part[s] (r x b, ld r) = G(:, Ks) U(Ks, :) over the rows
 // Ks = [s*kc, min(mU, (s+1)*kc)).  Wavefront w owns columns 16w..16w+15 of
-// the slice and every 16-row block of r.  With a.Qprev set, the previous
-// block's deferred G update G(:, Ks) -= Qprev Uprev(Ks, :)' is applied to
-// the slice first (and stored), so a block's G update costs no pass of its
-// own: it rides on the next block's read of the same columns of G.
+// the slice and every 16-row block of r.
 __global__ __launch_bounds__(HQ_NT) void dd_partial_kernel(DowndateArgs a) {
-  __shared__ double Gs[kK][kRP];   // Gs[k][i] = G(i, k0+k)
-  __shared__ double Us[kK][kCP];   // Us[k][p] = U(k0+k, p)
-  __shared__ double Qs[64][kRP];   // Qs[p][i] = -Qprev(i, p)
-  __shared__ double Ups[kK][kCP];  // Ups[k][p] = Uprev(k0+k, p)
+  __shared__ double Gs[kK][kRP];  // Gs[k][i] = G(i, k0+k)
+  __shared__ double Us[kK][kCP];  // Us[k][p] = U(k0+k, p)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int r = a.r, b = a.b;
-  const bool pend = a.Qprev != nullptr;
   const long kb = (long)blockIdx.x * a.kc;
   const long ke = min((long)a.mU, kb + a.kc);
-  if (pend) {
-    for (int idx = t; idx < 64 * 16 * kRB; idx += HQ_NT) {
-      const int i = idx % (16 * kRB), p = idx / (16 * kRB);
-      Qs[p][i] = (i < r && p < a.bprev) ? -a.Qprev[i + (long)p * r] : 0.0;
-    }
-  }
   hq_d4 acc[kRB];
 #pragma unroll
   for (int q = 0; q < kRB; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
   for (long k0 = kb; k0 < ke; k0 += kK) {
-    for (int idx = t; idx < kK * 16 * kRB; idx += HQ_NT) {
+#pragma unroll
+    for (int u = 0; u < kK * 16 * kRB / HQ_NT; ++u) {
+      const int idx = t + u * HQ_NT;
       const int k = idx / (16 * kRB), i = idx % (16 * kRB);
       const long gk = k0 + k;
       Gs[k][i] = (i < r && gk < ke) ? a.G[i + gk * a.ldg] : 0.0;
     }
-    for (int idx = t; idx < kK * 64; idx += HQ_NT) {
+#pragma unroll
+    for (int u = 0; u < kK * 64 / HQ_NT; ++u) {
+      const int idx = t + u * HQ_NT;
       const int k = idx & (kK - 1), p = idx / kK;
       const long gk = k0 + k;
       double v = 0.0;
       if (p < b && gk < ke) v = unit_lower_dd(a.U[gk + (long)p * a.ldu], gk, p);
       Us[k][p] = v;
-      if (pend)
-        Ups[k][p] = (p < a.bprev && gk < ke) ? a.Uprev[gk + (long)p * a.ldup]
-                                             : 0.0;
     }
     __syncthreads();
-    if (pend) {
-      // G(i, k0+k) for k in wavefront w's 16 columns, every row block.
-      const int kc = wave * 16 + (lane & 15);
-      const long gk = k0 + kc;
-      hq_d4 g[kRB];
 #pragma unroll
-      for (int q = 0; q < kRB; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          g[q][e] = Gs[kc][q * 16 + (lane >> 4) + 4 * e];
-#pragma unroll 4
-      for (int kk = 0; kk < 64; kk += 4) {
-        const double bf = Ups[wave * 16 + (lane & 15)][kk + (lane >> 4)];
-#pragma unroll
-        for (int q = 0; q < kRB; ++q) {
-          const double af = Qs[kk + (lane >> 4)][q * 16 + (lane & 15)];
-          g[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, g[q], 0, 0, 0);
-        }
-      }
-      __syncthreads();  // every wavefront has read Gs
-#pragma unroll
-      for (int q = 0; q < kRB; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int i = q * 16 + (lane >> 4) + 4 * e;
-          Gs[kc][i] = g[q][e];
-          if (i < r && gk < ke) a.G[i + gk * a.ldg] = g[q][e];
-        }
-      __syncthreads();
-    }
-#pragma unroll 4
     for (int kk = 0; kk < kK; kk += 4) {
       const double bf = Us[kk + (lane >> 4)][wave * 16 + (lane & 15)];
 #pragma unroll
@@ -162,18 +121,24 @@ __global__ __launch_bounds__(HQ_NT) void dd_qb_kernel(DowndateArgs a) {
   __shared__ double Us[64][kCP];  // Us[c][p] = U1(c, p)
   const int t = threadIdx.x, r = a.r, b = a.b;
   const int i0 = blockIdx.x * 16;
-  for (int idx = t; idx < 16 * 64; idx += HQ_NT) {
+  #pragma unroll
+  for (int u_ = 0; u_ < (16 * 64) / HQ_NT; ++u_) {
+    const int idx = t + u_ * HQ_NT;
     const int i = idx & 15, p = idx >> 4;
     Ps[i][p] = (i0 + i < r && p < b) ? a.P[i0 + i + (long)p * r] : 0.0;
   }
-  for (int idx = t; idx < 64 * 64; idx += HQ_NT) {
+  #pragma unroll
+  for (int u_ = 0; u_ < (64 * 64) / HQ_NT; ++u_) {
+    const int idx = t + u_ * HQ_NT;
     const int p = idx & 63, q = idx >> 6;
     Ts[p][q] = (p <= q && q < b) ? a.T[p + (long)q * a.ldt] : 0.0;
     Us[p][q] = (q < b && p < b) ? unit_lower_dd(a.U[p + (long)q * a.ldu], p, q)
                                 : 0.0;
   }
   __syncthreads();
-  for (int idx = t; idx < 16 * 64; idx += HQ_NT) {
+  #pragma unroll
+  for (int u_ = 0; u_ < (16 * 64) / HQ_NT; ++u_) {
+    const int idx = t + u_ * HQ_NT;
     const int i = idx & 15, q = idx >> 4;
     double s = 0.0;
     for (int p = 0; p <= q; ++p) s += Ps[i][p] * Ts[p][q];
@@ -181,7 +146,9 @@ __global__ __launch_bounds__(HQ_NT) void dd_qb_kernel(DowndateArgs a) {
     if (i0 + i < r && q < b) a.Q[i0 + i + (long)q * r] = s;
   }
   __syncthreads();
-  for (int idx = t; idx < 16 * 64; idx += HQ_NT) {
+  #pragma unroll
+  for (int u_ = 0; u_ < (16 * 64) / HQ_NT; ++u_) {
+    const int idx = t + u_ * HQ_NT;
     const int i = idx & 15, c = idx >> 4;
     if (i0 + i >= r || c >= b) continue;
     double s = 0.0;
@@ -217,18 +184,24 @@ __global__ __launch_bounds__(HQ_NT) void dd_apply_kernel(DowndateArgs a) {
     out = a.Y2;
     ldo = a.ldy;
   }
-  for (int idx = t; idx < 64 * 16 * kRB; idx += HQ_NT) {
+  #pragma unroll
+  for (int u_ = 0; u_ < (64 * 16 * kRB) / HQ_NT; ++u_) {
+    const int idx = t + u_ * HQ_NT;
     const int i = idx % (16 * kRB), p = idx / (16 * kRB);
     As[p][i] = (i < r && p < b) ? -Asrc[i + (long)p * ldas] : 0.0;
   }
   if (isg) {
-    for (int idx = t; idx < 64 * 64; idx += HQ_NT) {
+    #pragma unroll
+    for (int u_ = 0; u_ < (64 * 64) / HQ_NT; ++u_) {
+      const int idx = t + u_ * HQ_NT;
       const int c = idx & 63, p = idx >> 6;
       const long gc = c0 + c;
       Bs[p][c] = (gc < ncol && p < b) ? a.U[gc + (long)p * a.ldu] : 0.0;
     }
   } else {
-    for (int idx = t; idx < 64 * 64; idx += HQ_NT) {
+    #pragma unroll
+    for (int u_ = 0; u_ < (64 * 64) / HQ_NT; ++u_) {
+      const int idx = t + u_ * HQ_NT;
       const int p = idx & 63, c = idx >> 6;
       const long gc = c0 + c;
       Bs[p][c] = (gc < ncol && p < b) ? a.R12[p + gc * a.ldr] : 0.0;
@@ -245,7 +218,7 @@ __global__ __launch_bounds__(HQ_NT) void dd_apply_kernel(DowndateArgs a) {
       acc[q][e] = (cok && i < r) ? out[i + gc * ldo] : 0.0;
     }
   __syncthreads();
-#pragma unroll 4
+#pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
     const double bf = Bs[kk + (lane >> 4)][wave * 16 + (lane & 15)];
 #pragma unroll
@@ -274,23 +247,7 @@ int downdate_slices(long mU, long* kc) {
   return (int)std::max(1L, (mU + c - 1) / c);
 }
 
-// P, Q and B only (dd_partial with the optional deferred update, the sum,
-// Q and B); the rest of the G update is left to the next block's call.
-hipError_t launch_downdate_qb(DowndateArgs a, hipStream_t s) {
-  if (a.r < 1 || a.r > 16 * kRB || a.b < 1 || a.b > 64 || a.mU < a.b ||
-      a.S < 1 || (a.Qprev && (a.bprev < 1 || a.bprev > 64)))
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(dd_partial_kernel, dim3(a.S), dim3(HQ_NT), 0, s, a);
-  const long total = (long)a.r * a.b;
-  hipLaunchKernelGGL(dd_reduce_kernel, dim3(hq_cdiv(total, 64)), dim3(HQ_NT),
-                     0, s, a);
-  hipLaunchKernelGGL(dd_qb_kernel, dim3(hq_cdiv(a.r, 16)), dim3(HQ_NT), 0, s,
-                     a);
-  return hipGetLastError();
-}
-
 hipError_t launch_downdate(DowndateArgs a, hipStream_t s) {
-  a.Qprev = nullptr;
   if (a.r < 1 || a.r > 16 * kRB || a.b < 1 || a.b > 64 || a.mU < a.b ||
       a.S < 1)
     return hipErrorInvalidValue;

@@ -110,6 +110,26 @@ __global__ __launch_bounds__(HQ_NT) void fill_sentinel_kernel(double* p,
     p[i] = v;
 }
 
+// dst(c, i) = src(i, c) for a rows x cols column-major src, through a
+// 32 x 32 LDS tile (both sides coalesced).
+__global__ __launch_bounds__(HQ_NT) void transpose_kernel(
+    const double* src, long lds, int rows, int cols, double* dst, long ldd) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const long i0 = (long)blockIdx.x * 32, c0 = (long)blockIdx.y * 32;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long i = i0 + tx, c = c0 + ty + k;
+    tile[ty + k][tx] = (i < rows && c < cols) ? src[i + c * lds] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + tx, i = i0 + ty + k;
+    if (i < rows && c < cols) dst[c + i * ldd] = tile[tx][ty + k];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s) {
@@ -157,5 +177,13 @@ hipError_t launch_gather_cols(const int* moves, int max_moves, double* A,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(gather_in_kernel, grid, dim3(HQ_NT), 0, s, moves, A, lda,
                      m, Yw, ldy * yw, r * yw, jpvt, tmpA, tY, tmpJ);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
+                            double* dst, long ldd, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transpose_kernel, dim3(hq_cdiv(rows, 32), hq_cdiv(cols, 32)),
+                     dim3(HQ_NT), 0, s, src, lds, rows, cols, dst, ldd);
   return hipGetLastError();
 }

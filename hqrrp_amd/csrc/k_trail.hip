@@ -39,6 +39,15 @@ __device__ __forceinline__ double vmask(double v, long i, long c) {
 // while the matrix cores work on the current one.  Row tiles vary fastest
 // across workgroups so concurrently running tiles cover different rows
 // (different HBM channels) of the power-of-two-strided columns.
+// Base of virtual column v (C's columns, then C2's from a.nca), or null
+// for the gap and beyond.
+__device__ __forceinline__ const double* vcol(const TrailArgs& a, long v) {
+  if (v < a.nc) return a.C + v * a.ldc;
+  if (a.C2 && v >= a.nca && v < a.nca + a.nc2)
+    return a.C2 + (v - a.nca) * a.ldc2;
+  return nullptr;
+}
+
 constexpr int kUT = 128;          // update tile edge
 constexpr int kUK = 16;           // K rows per pipeline stage
 constexpr int kUP = kUT + 8;      // LDS pitch (doubles)
@@ -51,23 +60,41 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
   const int l15 = lane & 15, l4 = lane >> 4;
   const int b = a.b;
   const int nchunks = (b + kUK - 1) / kUK;
-  const long m0 = (long)blockIdx.x * kUT, n0 = (long)blockIdx.y * kUT;
+  const long m0 = (long)blockIdx.x * kUT;
   const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
+  // The tile's column range: C's, C2's (updated in place) or the gap.
+  const long nv = (long)blockIdx.y * kUT;  // first virtual column (W' row)
+  long n0, nc, ldc, ldo;
+  double *Cb, *Ob;
+  if (nv < a.nc) {
+    n0 = nv;
+    nc = a.nc;
+    Cb = a.C;
+    ldc = a.ldc;
+    Ob = a.Cout ? a.Cout : a.C;
+    ldo = a.Cout ? a.ldco : a.ldc;
+  } else if (a.C2 && nv >= a.nca && nv < a.nca + a.nc2) {
+    n0 = nv - a.nca;
+    nc = a.nc2;
+    Cb = Ob = a.C2;
+    ldc = ldo = a.ldc2;
+  } else {
+    return;
+  }
 
   // acc[i][j] element r: C'(n, m), n = n0+64wn+16i+l/16+4r, m = m0+64wm+16j+l%16,
   // i.e. cbase[16j + (16i+4r)*ldc] with the lane's base below.
-  const bool interior = m0 + kUT <= a.mrows && n0 + kUT <= a.nc;
+  const bool interior = m0 + kUT <= a.mrows && n0 + kUT <= nc;
   const long lm = m0 + wm * 64 + l15, ln = n0 + wn * 64 + l4;
-  double* cbase = a.C + lm + ln * a.ldc;
-  const long ldo = a.Cout ? a.ldco : a.ldc;
-  double* obase = (a.Cout ? a.Cout : a.C) + lm + ln * ldo;
+  double* cbase = Cb + lm + ln * ldc;
+  double* obase = Ob + lm + ln * ldo;
   hq_d4 acc[4][4];
   if (interior) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+        const double* cc = cbase + (long)(16 * i + 4 * r) * ldc;
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j][r] = cc[16 * j];
       }
@@ -76,8 +103,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const bool nok = ln + 16 * i + 4 * r < a.nc;
-        const double* cc = cbase + (long)(16 * i + 4 * r) * a.ldc;
+        const bool nok = ln + 16 * i + 4 * r < nc;
+        const double* cc = cbase + (long)(16 * i + 4 * r) * ldc;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           acc[i][j][r] = (nok && lm + 16 * j < a.mrows) ? cc[16 * j] : 0.0;
@@ -98,7 +125,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
         if (masked) v = vmask(v, gm, k);
       }
       rv[q] = -v;
-      rw[q] = (k < b && gn < a.nc) ? a.W[gn + (long)k * a.ldw] : 0.0;
+      rw[q] = (k < b && gn < nc) ? a.W[nv + mn + (long)k * a.ldw] : 0.0;
     }
   };
   auto stash = [&](int buf) {
@@ -146,7 +173,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const bool nok = ln + 16 * i + 4 * r < a.nc;
+        const bool nok = ln + 16 * i + 4 * r < nc;
         double* cc = obase + (long)(16 * i + 4 * r) * ldo;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -185,9 +212,9 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
   bool cok[4], vok[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const long gn = n0 + 16 * i + l15;
-    cok[i] = gn < a.nc;
-    cp[i] = a.C + (cok[i] ? gn : 0) * a.ldc + 4 * g4;
+    const double* col = vcol(a, n0 + 16 * i + l15);
+    cok[i] = col != nullptr;
+    cp[i] = (cok[i] ? col : a.C) + 4 * g4;
     const int p = 16 * i + l15;
     vok[i] = p < b;
     vp[i] = a.V + (long)(vok[i] ? p : 0) * a.ldv + 4 * g4;
@@ -262,7 +289,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
   }
   // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
   // n = n0 + 16i + l/16 + 4r.
-  double* out = a.part + (long)s * b * a.nc;
+  double* out = a.part + (long)s * b * a.ncv;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -271,7 +298,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int p = j * 16 + l15;
         const long gn = n0 + i * 16 + g4 + 4 * r;
-        if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
+        if (p < b && gn < a.ncv) out[p + gn * b] = acc[i][j][r];
       }
 }
 
@@ -310,8 +337,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc64_kernel(TrailArgs a) {
       long gk = k0 + k;
       double vv = (gk < kend && p < b) ? a.V[gk + (long)p * a.ldv] : 0.0;
       rv[q] = a.vmask ? vmask(vv, gk, p) : vv;
-      long gn = n0 + p;
-      rc[q] = (gk < kend && gn < a.nc) ? a.C[gk + gn * a.ldc] : 0.0;
+      const double* col = vcol(a, n0 + p);
+      rc[q] = (gk < kend && col) ? col[gk] : 0.0;
     }
   };
   // The K slice is walked starting at a per-column-tile rotation, so that
@@ -349,7 +376,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc64_kernel(TrailArgs a) {
   }
   // acc element r of lane l: partial(p, n) with p = 32wp+16j+l%16 and
   // n = n0+32wn+16i+l/16+4r.
-  double* out = a.part + (long)s * b * a.nc;
+  double* out = a.part + (long)s * b * a.ncv;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -358,7 +385,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_vtc64_kernel(TrailArgs a) {
       for (int r = 0; r < 4; ++r) {
         int p = wp * 32 + j * 16 + l15;
         long gn = n0 + wn * 32 + i * 16 + l4 + 4 * r;
-        if (p < b && gn < a.nc) out[p + gn * b] = acc[i][j][r];
+        if (p < b && gn < a.ncv) out[p + gn * b] = acc[i][j][r];
       }
 }
 
@@ -371,7 +398,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
   __shared__ double Ts[kTile][kTile + 1];  // Ts[p][q] = T(p, q)
   const int t = threadIdx.x, b = a.b;
   const long n0 = (long)blockIdx.x * kTile;
-  const long slice = (long)b * a.nc;
+  const long slice = (long)b * a.ncv;
   for (int idx = t; idx < b * kTile; idx += HQ_NT) {
     int p = idx % b, n = idx / b;
     long gn = n0 + n;
@@ -401,66 +428,87 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
 
 }  // namespace
 
-// The block tail of the HQRRP driver, per 64 columns of the trailing
-// matrix (one workgroup each), after the split-K partials of M = V'C:
-//   W   = T' sum_s part[s]          (written as W' for the trailing update)
-//   R12 = C(0:b, :) - V1 W          (V1: the top b x b block of V, unit
-//                                    lower; R12 is row block j of R)
-//   Y2 -= B R12                     (the sketch downdate, B = G(:, j:j+b)
-//                                    after the block's G update)
-// Each product is 64 x 64 x 64 (r x 64 x 64 for Y2) on the matrix cores
-// with both operands in LDS; the three depend on each other, so they share
-// one launch.
+// The block tail of the HQRRP driver, per 64 virtual columns (one
+// workgroup each), after the split-K partials of M = V'[C C2]:
+//   W   = T' sum_s part[s]       (written as W' for the trailing update)
+//   R12 = C(0:b, :) - V1 W        (V1: the top b x b block of V, unit lower;
+//                                  R12 is row block j of R, written to R)
+// and for C2 = G' (the sketch operator, transposed) the same two lines give
+// Q' = T' P' and B' = G'(0:b, :) - V1 Q', written over G'(0:b, :): the
+// reference's B = G1 - (G1 U11 + G2 U21) T U11' (NoFLA_HQRRP_WY_blk_var4.c:
+// 646-686).  A second launch (trail_y2_kernel) then applies
+// Y2 -= B R12 (691-693).  Each product is 64 x 64 x 64 on the matrix cores
+// with both operands in LDS.
 namespace {
 constexpr int kTP = 65;  // LDS pitch of the 64-wide tiles
 constexpr int kTB = 81;  // LDS pitch of B' (r <= 80 rows)
 __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
-  __shared__ double R1[64 * kTB];      // T (pitch kTP), then B' (pitch kTB)
-  __shared__ double Ms[64][kTP];       // M(p, c), then R12(i, c)
-  __shared__ double V1s[64][kTP];      // V1(i, p)
-  __shared__ double Ws[64][kTP];       // W(q, c)
+  __shared__ double Ts[64][kTP];   // T(p, q)
+  __shared__ double Ms[64][kTP];   // M(p, c)
+  __shared__ double V1s[64][kTP];  // V1(i, p)
+  __shared__ double Ws[64][kTP];   // W(q, c)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l15 = lane & 15, l4 = lane >> 4;
-  const int b = a.b, r = a.r;
-  const long c0 = (long)blockIdx.x * 64;
-  const long slice = (long)b * a.nc;
-  // Loads: partial slices (fixed order), T, V1, and B into registers.
-  for (int idx = t; idx < 64 * 64; idx += HQ_NT) {
-    const int p = idx & 63, c = idx >> 6;
-    const long gc = c0 + c;
-    double m = 0.0;
-    if (p < b && gc < a.nc) {
-      const double* src = a.part + p + gc * b;
-      for (int q = 0; q < a.S; ++q) m += src[q * slice];
-    }
-    Ms[p][c] = m;
-    R1[p * kTP + c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
-    // idx -> (i = p, col = c) of V1
-    V1s[p][c] = (p < b && c < b) ? vmask(a.V[p + (long)c * a.ldv], p, c) : 0.0;
+  const int b = a.b;
+  const long v0 = (long)blockIdx.x * 64;  // first virtual column
+  long c0, nc, ldc, ldr;
+  const double* Cb;
+  double* Rb;
+  if (v0 < a.nc) {
+    c0 = v0;
+    nc = a.nc;
+    Cb = a.C;
+    ldc = a.ldc;
+    Rb = a.R;
+    ldr = a.ldr;
+  } else if (a.C2 && v0 >= a.nca && v0 < a.nca + a.nc2) {
+    c0 = v0 - a.nca;
+    nc = a.nc2;
+    Cb = Rb = a.C2;
+    ldc = ldr = a.ldc2;
+  } else {
+    return;
   }
-  constexpr int kBPer = 64 * 80 / HQ_NT;
-  double breg[kBPer];
+  const long slice = (long)b * a.ncv;
+  // Loads: partial slices (summed in slice order), T and V1, every load of
+  // a kind in flight together.
+  constexpr int kPer = 64 * 64 / HQ_NT;
+  double msum[kPer];
 #pragma unroll
-  for (int q = 0; q < kBPer; ++q) {
-    const int idx = t + HQ_NT * q, i = idx % 80, p = idx / 80;
-    breg[q] = (a.ny > 0 && i < r && p < b) ? -a.B[i + (long)p * a.ldb] : 0.0;
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+    Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
+    V1s[p][c] = (p < b && c < b) ? vmask(a.V[p + (long)c * a.ldv], p, c) : 0.0;
+    msum[u] = 0.0;
+  }
+  for (int q = 0; q < a.S; ++q) {
+    const double* src = a.part + q * slice;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      if (p < b && c0 + c < nc) msum[u] += src[p + (v0 + c) * b];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = t + u * HQ_NT;
+    Ms[idx & 63][idx >> 6] = msum[u];
   }
   __syncthreads();
   const int cw = wave * 16 + l15;  // this lane's column of the tile
-  const long gc = c0 + cw;
-  const bool cok = gc < a.nc;
+  const bool cok = c0 + cw < nc;
   // W = T' M: A = T' (q x p) from T(p, q), B = M.
   {
     hq_d4 acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+#pragma unroll
     for (int kk = 0; kk < 64; kk += 4) {
       const double bf = Ms[kk + l4][cw];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            R1[(kk + l4) * kTP + q * 16 + l15], bf, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[kk + l4][q * 16 + l15],
+                                                      bf, acc[q], 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -468,27 +516,23 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
       for (int e = 0; e < 4; ++e) {
         const int qq = q * 16 + l4 + 4 * e;
         Ws[qq][cw] = acc[q][e];
-        if (cok && qq < b) a.W[gc + (long)qq * a.ldw] = acc[q][e];
+        if (cok && qq < b) a.W[v0 + cw + (long)qq * a.ldw] = acc[q][e];
       }
   }
-  __syncthreads();  // T and M consumed; W complete
-#pragma unroll
-  for (int q = 0; q < kBPer; ++q) {
-    const int idx = t + HQ_NT * q, i = idx % 80, p = idx / 80;
-    R1[p * kTB + i] = breg[q];
-  }
-  // R12 = C_top - V1 W: A = -V1 (i x p), B = W; C_top straight into the
-  // accumulators.
+  __syncthreads();  // W complete
+  // R12 (or B') = top rows - V1 W: A = -V1 (i x p), B = W; the top rows
+  // straight into the accumulators.
   {
+    const long gc = c0 + cw;
     hq_d4 acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = q * 16 + l4 + 4 * e;
-        acc[q][e] = (cok && i < b) ? a.C[i + gc * a.ldc] : 0.0;
+        acc[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
       }
-#pragma unroll 4
+#pragma unroll
     for (int kk = 0; kk < 64; kk += 4) {
       const double bf = Ws[kk + l4][cw];
 #pragma unroll
@@ -501,65 +545,95 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = q * 16 + l4 + 4 * e;
-        Ms[i][cw] = acc[q][e];
-        if (cok && i < b) a.R[i + gc * a.ldr] = acc[q][e];
+        if (cok && i < b) Rb[i + gc * ldr] = acc[q][e];
       }
   }
-  if (a.ny <= 0) return;
-  __syncthreads();  // R12 and B' complete
-  // Y2 -= B R12: A = -B (i x p) from B', B = R12; Y2 into the accumulators.
-  {
-    const bool yok = gc < a.ny;
-    hq_d4 acc[5];
+}
+
+// Y2(:, c) -= B R12(:, c) on 64-column tiles of C, B = (C2(0:b, :))' (r x b,
+// r = nc2 <= 80) and R12 = R(0:b, :).  Wavefront w owns columns
+// 16w..16w+15 and every 16-row block of r; Y2 goes straight into the
+// accumulators.
+__global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
+  __shared__ double As[64][kTB];  // As[p][i] = -B(i, p) = -C2(p, i)
+  __shared__ double Bs[64][kTP];  // Bs[p][c] = R12(p, c0 + c)
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l15 = lane & 15, l4 = lane >> 4;
+  const int b = a.b, r = a.nc2;
+  const long c0 = (long)blockIdx.x * 64;
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
+  for (int u = 0; u < 64 * 80 / HQ_NT; ++u) {
+    const int idx = t + u * HQ_NT, p = idx & 63, i = idx >> 6;
+    As[p][i] = (p < b && i < r) ? -a.C2[p + (long)i * a.ldc2] : 0.0;
+  }
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = q * 16 + l4 + 4 * e;
-        acc[q][e] = (yok && i < r) ? a.Y2[i + gc * a.ldy] : 0.0;
-      }
-#pragma unroll 4
-    for (int kk = 0; kk < 64; kk += 4) {
-      const double bf = Ms[kk + l4][cw];
+  for (int u = 0; u < 64 * 64 / HQ_NT; ++u) {
+    const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+    Bs[p][c] = (p < b && c0 + c < a.nc) ? a.R[p + (c0 + c) * a.ldr] : 0.0;
+  }
+  const int cw = wave * 16 + l15;
+  const long gc = c0 + cw;
+  const bool cok = gc < a.nc;
+  hq_d4 acc[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            R1[(kk + l4) * kTB + q * 16 + l15], bf, acc[q], 0, 0, 0);
+  for (int q = 0; q < 5; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = q * 16 + l4 + 4 * e;
+      acc[q][e] = (cok && i < r) ? a.Y2[i + gc * a.ldy] : 0.0;
     }
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < 64; kk += 4) {
+    const double bf = Bs[kk + l4][cw];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = q * 16 + l4 + 4 * e;
-        if (yok && i < r) a.Y2[i + gc * a.ldy] = acc[q][e];
-      }
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[kk + l4][q * 16 + l15],
+                                                    bf, acc[q], 0, 0, 0);
   }
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = q * 16 + l4 + 4 * e;
+      if (cok && i < r) a.Y2[i + gc * a.ldy] = acc[q][e];
+    }
 }
 }  // namespace
 
 hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s) {
-  if (a.b < 1 || a.b > 64 || a.r > 80 || a.nc < 1 || a.S < 1)
+  if (a.b < 1 || a.b > 64 || a.nc < 1 || a.S < 1 || (a.C2 && a.nc2 > 80))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.nc, 64)), dim3(HQ_NT),
+  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.ncv, 64)), dim3(HQ_NT),
                      0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_trail_y2(const TailArgs& a, hipStream_t s) {
+  if (a.b < 1 || a.b > 64 || a.nc < 1 || !a.C2 || a.nc2 < 1 || a.nc2 > 80)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(trail_y2_kernel, dim3(hq_cdiv(a.nc, 64)), dim3(HQ_NT), 0,
+                     s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
   if (a.wide) {
-    dim3 grid(hq_cdiv(a.nc, kVN), a.splits);
+    dim3 grid(hq_cdiv(a.ncv, kVN), a.splits);
     // 32-byte groups of rows k0 + 4g (k0 a multiple of 16 from the slice
     // start, which is a multiple of trail_kbk()) are 32-byte aligned when
     // both bases and leading dimensions are.
     const bool vec = (reinterpret_cast<uintptr_t>(a.C) % 32 == 0) &&
                      (reinterpret_cast<uintptr_t>(a.V) % 32 == 0) &&
-                     a.ldc % 4 == 0 && a.ldv % 4 == 0 && a.kchunk % 4 == 0;
+                     a.ldc % 4 == 0 && a.ldv % 4 == 0 && a.kchunk % 4 == 0 &&
+                     (!a.C2 || (reinterpret_cast<uintptr_t>(a.C2) % 32 == 0 &&
+                                a.ldc2 % 4 == 0));
     if (vec)
       hipLaunchKernelGGL(trail_vtc_kernel<true>, grid, dim3(HQ_NT), 0, s, a);
     else
       hipLaunchKernelGGL(trail_vtc_kernel<false>, grid, dim3(HQ_NT), 0, s, a);
   } else {
-    dim3 grid(hq_cdiv(a.nc, kTile), a.splits);
+    dim3 grid(hq_cdiv(a.ncv, kTile), a.splits);
     hipLaunchKernelGGL(trail_vtc64_kernel, grid, dim3(HQ_NT), 0, s, a);
   }
   return hipGetLastError();
@@ -572,7 +646,7 @@ hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s) {
-  dim3 grid(hq_cdiv(a.mrows, kUT), hq_cdiv(a.nc, kUT));
+  dim3 grid(hq_cdiv(a.mrows, kUT), hq_cdiv(a.C2 ? a.ncv : a.nc, kUT));
   hipLaunchKernelGGL(trail_update_kernel, grid, dim3(HQ_NT), 0, s, a);
   return hipGetLastError();
 }

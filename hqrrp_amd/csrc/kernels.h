@@ -64,8 +64,16 @@ struct TrailArgs {
   long ldw;
   const double* T;  // b x b upper triangular, ld ldt
   long ldt;
-  double* part;  // splits slices of b x nc
+  double* part;  // splits slices of b x ncv
   long kchunk;   // rows of C per split
+  // Optional second column range, updated with C (in place): virtual
+  // columns [0, nc) are C's, [nca, nca + nc2) are C2's (mrows x nc2, ld
+  // ldc2, rows aligned with C's; nca a multiple of 256 >= nc, the gap
+  // empty).  ncv = nca + nc2 with C2, else nc: the column count of the
+  // partials and of W'.
+  double* C2;
+  long ldc2;
+  int nc2, nca, ncv;
 };
 
 // k_downdate.hip: FP64 sketch downdate of one block (r <= 80, b <= 64).
@@ -87,18 +95,10 @@ struct DowndateArgs {
   int S;         // row slices of U (downdate_slices)
   long kc;       // rows per slice
   int ng;        // set by launch_downdate
-  // Deferred G update of the previous block (launch_downdate_qb only):
-  // G -= Qprev Uprev' before P is formed; Qprev r x bprev (ld r), Uprev
-  // mU x bprev (ld ldup, rows aligned with U's).  Null: none.
-  const double* Qprev;
-  const double* Uprev;
-  long ldup;
-  int bprev;
 };
 // Slices and rows per slice for mU rows of U.
 int downdate_slices(long mU, long* kc);
 hipError_t launch_downdate(DowndateArgs a, hipStream_t s);
-hipError_t launch_downdate_qb(DowndateArgs a, hipStream_t s);
 
 // k_batch.hip: batched HQRRP of small matrices (m, n <= HQB_DIM_MAX;
 // nb <= 64; pp <= HQB_PP_MAX).
@@ -147,9 +147,9 @@ hipError_t launch_batch_hqrrp(const BatchArgs& a, int class_nb, int wgs,
 // k_trail.hip
 // Block tail of the HQRRP driver (see trail_tail_kernel).
 struct TailArgs {
-  int b, nc, r, S;
-  int ny;              // columns of Y2 to downdate (0: none)
-  const double* part;  // S slices of b x nc (trail_vtc's partials)
+  int b, nc, S;
+  int nc2, nca, ncv;   // C2's columns, as in TrailArgs
+  const double* part;  // S slices of b x ncv (trail_vtc's partials)
   const double* T;     // b x b upper triangular, ld ldt
   long ldt;
   const double* V;  // the block's reflectors (top b x b read unit lower)
@@ -158,14 +158,16 @@ struct TailArgs {
   long ldc;
   double* R;  // R12 out (b x nc), ld ldr (may equal C)
   long ldr;
-  double* W;  // W' out (nc x b), ld ldw
+  double* C2;  // rows 0..b-1 of the second range (updated in place), ld ldc2
+  long ldc2;
+  double* W;  // W' out (ncv x b), ld ldw
   long ldw;
-  const double* B;  // r x b, ld ldb
-  long ldb;
-  double* Y2;  // r x ny, ld ldy
+  double* Y2;  // r x nc, ld ldy: Y2 -= C2(0:b, :)' R12 (r = nc2)
   long ldy;
 };
+// W', R12 (and C2's top rows); then, with Y2, the sketch downdate.
 hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s);
+hipError_t launch_trail_y2(const TailArgs& a, hipStream_t s);
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
@@ -196,6 +198,9 @@ hipError_t launch_reduce_splitk(int M, int N, int S, const double* part,
 
 // k_misc.hip
 hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s);
+// dst (cols x rows, ld ldd) = src' (src: rows x cols, ld lds).
+hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
+                            double* dst, long ldd, hipStream_t s);
 hipError_t launch_copy_block(double* dst, long ldd, const double* src,
                              long lds, int rows, int cols, hipStream_t s);
 hipError_t launch_select_cols(const int* moves, int b2, const double* C,
