@@ -6,6 +6,7 @@
 #define HQ_NT 256  // threads per workgroup for every kernel in this library
 
 typedef double hq_d4 __attribute__((ext_vector_type(4)));
+typedef double hq_d2 __attribute__((ext_vector_type(2)));
 
 // Error flags written by kernels (slot 0 of the handle's error word).
 enum {

@@ -191,8 +191,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 // MFMA substep e (0..3) lane l supplies row 4*(l/16) + e of the chunk for
 // both operands, so every lane reads 4 consecutive doubles (32 bytes) of one
 // column of C and of V per chunk, and 16 lanes cover a column's 128-byte
-// line.  A two-chunk register ring keeps the next chunk's loads in flight
-// while the matrix cores work on the current one.  VEC: the 32-byte groups are
+// line.  A three-chunk register ring keeps the next two chunks' loads in
+// flight while the matrix cores work on the current one.  VEC: the 32-byte groups are
 // 32-byte aligned (vector loads); otherwise the same values are loaded one
 // double at a time, so both forms compute identical bits.
 constexpr int kVN = 256;  // columns of C per workgroup
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
   const long nsteps = (kend - kbeg + kCK - 1) / kCK;
   const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
 
-  hq_d4 cb[2][4], vb[2][4];
+  hq_d4 cb[3][4], vb[3][4];
   auto load = [&](hq_d4 (&cr)[4], hq_d4 (&vr)[4], long it) {
     if (it >= nsteps) {
 #pragma unroll
@@ -279,13 +279,17 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[i][e], vr[j][e],
                                                             acc[i][j], 0, 0, 0);
   };
+  // Three-chunk ring, the step count padded to a multiple of three
+  // (padding chunks are zeros): constant ring indices, no early exit.
   load(cb[0], vb[0], 0);
-  for (long it = 0; it < nsteps; it += 2) {
-    load(cb[1], vb[1], it + 1);
+  load(cb[1], vb[1], 1);
+  for (long it = 0; it < nsteps; it += 3) {
+    load(cb[2], vb[2], it + 2);
     compute(cb[0], vb[0]);
-    if (it + 1 >= nsteps) break;
-    load(cb[0], vb[0], it + 2);
+    load(cb[0], vb[0], it + 3);
     compute(cb[1], vb[1]);
+    load(cb[1], vb[1], it + 4);
+    compute(cb[2], vb[2]);
   }
   // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
   // n = n0 + 16i + l/16 + 4r.
