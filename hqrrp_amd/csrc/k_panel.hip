@@ -328,43 +328,51 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
   __shared__ double sG[kMaxB][kMaxB + 1];
   __shared__ double sX[kMaxB][kMaxB + 1];
   const int t = threadIdx.x;
-  for (int idx = t; idx < kMaxB * (kMaxB + 1); idx += HQ_NT) {
-    (&sT[0][0])[idx] = 0.0;
-    (&sX[0][0])[idx] = 0.0;
+  // sT: tau on the diagonal, zero elsewhere; sG: the strictly upper gram.
+#pragma unroll
+  for (int u = 0; u < kMaxB * kMaxB / HQ_NT; ++u) {
+    const int idx = t + u * HQ_NT, i = idx & (kMaxB - 1), j = idx >> 6;
+    const bool in = i < b && j < b;
+    sG[i][j] = (in && i < j) ? gram[i + j * b] : 0.0;
+    sT[i][j] = (in && i == j) ? tau[i] : 0.0;
   }
   __syncthreads();
-  for (int idx = t; idx < b * b; idx += HQ_NT) {
-    int i = idx % b, j = idx / b;
-    sG[i][j] = (i < j) ? gram[idx] : 0.0;
-    if (i == j) sT[i][i] = tau[i];
-  }
-  __syncthreads();
-  for (int s = 1; s < b; s *= 2) {
-    // X(B1,B2) = G(B1,B2) T(B2,B2) for every pair (B1,B2) at this level.
-    for (int e = t; e < b * s; e += HQ_NT) {
-      const int col = e / s, r = e % s;           // col: global column in B2
-      const int c0 = (col / (2 * s)) * 2 * s;     // start of B1
-      const int c1 = c0 + s;                      // start of B2
-      if (col < c1 || col >= min(b, c0 + 2 * s)) continue;
+  // Level s: for the pairs of column blocks B1 = [c0, c0+s), B2 = [c0+s,
+  // c0+2s), element (r, cc) of the s x s block (row c0+r, column c0+s+cc).
+  // s is a compile-time power of two after unrolling, so the index split is
+  // shifts and masks.
+#pragma unroll
+  for (int lv = 0; lv < 6; ++lv) {
+    const int s2 = 1 << lv;
+    if (s2 >= b) break;
+    const int per = s2 * s2, npairs = (b + 2 * s2 - 1) / (2 * s2);
+    // X(B1, B2) = G(B1, B2) T(B2, B2).
+    for (int e = t; e < npairs * per; e += HQ_NT) {
+      const int pr = e >> (2 * lv), rem = e & (per - 1);
+      const int r = rem >> lv, cc = rem & (s2 - 1);
+      const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
+      if (col >= b) continue;
       double acc = 0.0;
       for (int u = c1; u <= col; ++u) acc += sG[c0 + r][u] * sT[u][col];
       sX[c0 + r][col] = acc;
     }
     __syncthreads();
-    // T(B1,B2) = -T(B1,B1) X(B1,B2).
-    for (int e = t; e < b * s; e += HQ_NT) {
-      const int col = e / s, r = e % s;
-      const int c0 = (col / (2 * s)) * 2 * s, c1 = c0 + s;
-      if (col < c1 || col >= min(b, c0 + 2 * s)) continue;
+    // T(B1, B2) = -T(B1, B1) X(B1, B2).
+    for (int e = t; e < npairs * per; e += HQ_NT) {
+      const int pr = e >> (2 * lv), rem = e & (per - 1);
+      const int r = rem >> lv, cc = rem & (s2 - 1);
+      const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
+      if (col >= b) continue;
       double acc = 0.0;
       for (int u = c0 + r; u < c1; ++u) acc += sT[c0 + r][u] * sX[u][col];
       sT[c0 + r][col] = -acc;
     }
     __syncthreads();
   }
-  for (int idx = t; idx < b * b; idx += HQ_NT) {
-    int i = idx % b, j = idx / b;
-    T[i + (long)j * ldt] = (i <= j) ? sT[i][j] : 0.0;
+#pragma unroll
+  for (int u = 0; u < kMaxB * kMaxB / HQ_NT; ++u) {
+    const int idx = t + u * HQ_NT, i = idx & (kMaxB - 1), j = idx >> 6;
+    if (i < b && j < b) T[i + (long)j * ldt] = (i <= j) ? sT[i][j] : 0.0;
   }
 }
 
