@@ -474,24 +474,51 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
     return;
   }
   const long slice = (long)b * a.ncv;
-  // Loads: partial slices (summed in slice order), T and V1, every load of
-  // a kind in flight together.
+  const int cw = wave * 16 + l15;  // this lane's column of the tile
+  const bool cok = c0 + cw < nc;
+  const long gc = c0 + cw;
+  // Every load up front (T, V1, the top rows for R12, then the partial
+  // slices summed in slice order, four slices' loads in flight at a time),
+  // then the LDS stores.
   constexpr int kPer = 64 * 64 / HQ_NT;
-  double msum[kPer];
+  hq_d4 top[4];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
     Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
     V1s[p][c] = (p < b && c < b) ? vmask(a.V[p + (long)c * a.ldv], p, c) : 0.0;
-    msum[u] = 0.0;
   }
-  for (int q = 0; q < a.S; ++q) {
-    const double* src = a.part + q * slice;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
-      if (p < b && c0 + c < nc) msum[u] += src[p + (v0 + c) * b];
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = q * 16 + l4 + 4 * e;
+      top[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
     }
+  // The slices four at a time: 4 x kPer independent loads in flight, then
+  // added in slice order.
+  double msum[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) msum[u] = 0.0;
+  const long off0 = (long)(t & 63) + (v0 + (t >> 6)) * b;  // u = 0
+  const bool pok = (t & 63) < b;
+  for (int q0 = 0; q0 < a.S; q0 += 4) {
+    double x[4][kPer];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool sok = q0 + k < a.S;
+      const double* src = a.part + (q0 + k) * slice + off0;
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int c = (t >> 6) + u * (HQ_NT / 64);
+        x[k][u] = (sok && pok && c0 + c < nc) ? src[(long)u * (HQ_NT / 64) * b]
+                                              : 0.0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) msum[u] += x[k][u];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
@@ -499,8 +526,6 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
     Ms[idx & 63][idx >> 6] = msum[u];
   }
   __syncthreads();
-  const int cw = wave * 16 + l15;  // this lane's column of the tile
-  const bool cok = c0 + cw < nc;
   // W = T' M: A = T' (q x p) from T(p, q), B = M.
   {
     hq_d4 acc[4];
@@ -527,15 +552,9 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   // R12 (or B') = top rows - V1 W: A = -V1 (i x p), B = W; the top rows
   // straight into the accumulators.
   {
-    const long gc = c0 + cw;
     hq_d4 acc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = q * 16 + l4 + 4 * e;
-        acc[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
-      }
+    for (int q = 0; q < 4; ++q) acc[q] = top[q];
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 4) {
       const double bf = Ws[kk + l4][cw];
