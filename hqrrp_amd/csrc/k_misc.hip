@@ -83,12 +83,10 @@ __global__ __launch_bounds__(HQ_NT) void select_cols_kernel(
   const int t = threadIdx.x;
   if (t < 64) src[t] = t;
   __syncthreads();
-  if (moves && t == 0) {
-    const int cnt = moves[0];
-    for (int e = 0; e < cnt; ++e) {
-      const int dst = moves[1 + 2 * e];
-      if (dst < b2) src[dst] = moves[2 + 2 * e];
-    }
+  // The composed moves have distinct destinations: one thread per move.
+  if (moves && t < moves[0]) {  // at most 2 * 80 moves < HQ_NT
+    const int dst = moves[1 + 2 * t];
+    if (dst < b2) src[dst] = moves[2 + 2 * t];
   }
   __syncthreads();
   const int q = blockIdx.y;

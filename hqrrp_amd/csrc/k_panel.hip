@@ -381,23 +381,35 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
 __global__ __launch_bounds__(HQ_NT) void panel_permute_kernel(
     double* A, long lda, int nrows, int b, const int* perm, int* jpvt) {
   __shared__ double buf[kMaxB][65];
-  __shared__ int sj[kMaxB];
+  __shared__ int sj[kMaxB], sp[kMaxB];
   const int t = threadIdx.x;
   const long r0 = (long)blockIdx.x * 64;
-  if (blockIdx.x == 0 && jpvt) {
-    if (t < b) sj[t] = jpvt[perm[t]];
-    __syncthreads();
-    if (t < b) jpvt[t] = sj[t];
+  const bool jp = blockIdx.x == 0 && jpvt;
+  if (t < b) {
+    const int pt = perm[t];
+    sp[t] = pt;
+    if (jp) sj[t] = jpvt[pt];
   }
-  if (r0 >= nrows) return;
-  for (int idx = t; idx < 64 * b; idx += HQ_NT) {
-    int i = idx & 63, c = idx >> 6;
-    if (r0 + i < nrows) buf[c][i] = A[(r0 + i) + (long)c * lda];
+  // Every load of the block's 64 rows in flight together (compile-time
+  // trip count), then the stores in the new column order.
+  constexpr int kPer = 64 * kMaxB / HQ_NT;
+  double v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = t + u * HQ_NT, i = idx & 63, c = idx >> 6;
+    v[u] = (c < b && r0 + i < nrows) ? A[(r0 + i) + (long)c * lda] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = t + u * HQ_NT;
+    buf[idx >> 6][idx & 63] = v[u];
   }
   __syncthreads();
-  for (int idx = t; idx < 64 * b; idx += HQ_NT) {
-    int i = idx & 63, q = idx >> 6;
-    if (r0 + i < nrows) A[(r0 + i) + (long)q * lda] = buf[perm[q]][i];
+  if (jp && t < b) jpvt[t] = sj[t];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int idx = t + u * HQ_NT, i = idx & 63, q = idx >> 6;
+    if (q < b && r0 + i < nrows) A[(r0 + i) + (long)q * lda] = buf[sp[q]][i];
   }
 }
 
