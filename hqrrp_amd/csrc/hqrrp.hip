@@ -173,7 +173,7 @@ struct hqrrp_handle_s {
   // Sketch precision of the running factorisation: Y and G hold floats
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
-  DevBuf R12, Pn, Wsel;  // look-ahead: updated R12 rows, next panel, its W'
+  DevBuf R12, Pn;  // look-ahead: updated R12 rows (FP32 sketch), next panel
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
@@ -799,7 +799,6 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   if (use_la) {
     HQ_RET(ensure(h->R12, sizeof(double) * (size_t)nb * n));
     HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
-    HQ_RET(ensure(h->Wsel, sizeof(double) * (size_t)nb * nb));
     HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
   }
   void* Y = h->Y.p;  // FP64 or FP32 (h->sk32)
@@ -928,26 +927,13 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     auto chain = [&]() -> int {
       if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
       {
-        PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)mnext * b2, 1);
-        HQ_TRY(launch_select_cols(last_next ? nullptr : h->moves.as<int>(), b2,
-                                  C, ldx[xc], b, mnext, ta.W, ta.ldw, b, Pn,
-                                  mnext, h->Wsel.as<double>(), h->cur));
-      }
-      {
+        // The next panel's columns, selected and updated in one pass (the
+        // same products, in the same order, as the full update's).
         PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
                       16.0 * (double)mnext * b2, 1);
-        TrailArgs tp = ta;
-        tp.mrows = mnext;
-        tp.nc = b2;
-        tp.C2 = nullptr;
-        tp.vmask = 0;
-        tp.V = Ajj + b;
-        tp.C = Pn;
-        tp.ldc = mnext;
-        tp.Cout = nullptr;
-        tp.W = h->Wsel.as<double>();
-        tp.ldw = b2;
-        HQ_TRY(launch_trail_update(tp, h->cur));
+        HQ_TRY(launch_select_update(last_next ? nullptr : h->moves.as<int>(),
+                                    b2, C, ldx[xc], b, mnext, Ajj + b, lda,
+                                    ta.W, ta.ldw, b, Pn, mnext, h->cur));
       }
       HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
                        h->perm.as<int>(), T, nb));
@@ -1220,7 +1206,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
                     &h->hA,    &h->hJ,    &h->hT,     &h->hW,     &h->R12,
-                    &h->Pn,    &h->Wsel,  &h->Xb,     &h->adjAP,
+                    &h->Pn,    &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);

@@ -353,7 +353,9 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
       const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
       if (col >= b) continue;
       double acc = 0.0;
-      for (int u = c1; u <= col; ++u) acc += sG[c0 + r][u] * sT[u][col];
+#pragma unroll
+      for (int u = 0; u < s2; ++u)  // u <= cc: T(B2, B2) is upper
+        acc += (u <= cc) ? sG[c0 + r][c1 + u] * sT[c1 + u][col] : 0.0;
       sX[c0 + r][col] = acc;
     }
     __syncthreads();
@@ -364,7 +366,9 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
       const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
       if (col >= b) continue;
       double acc = 0.0;
-      for (int u = c0 + r; u < c1; ++u) acc += sT[c0 + r][u] * sX[u][col];
+#pragma unroll
+      for (int u = 0; u < s2; ++u)  // u >= r: T(B1, B1) is upper
+        acc += (u >= r) ? sT[c0 + r][c0 + u] * sX[c0 + u][col] : 0.0;
       sT[c0 + r][col] = -acc;
     }
     __syncthreads();

@@ -584,15 +584,19 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
   const int l15 = lane & 15, l4 = lane >> 4;
   const int b = a.b, r = a.nc2;
   const long c0 = (long)blockIdx.x * 64;
+  // Every load in flight together (B', R12, the Y2 tile), then the LDS
+  // stores.
+  constexpr int kPa = 64 * 80 / HQ_NT, kPb = 64 * 64 / HQ_NT;
+  double va[kPa], vb[kPb];
 #pragma unroll
-  for (int u = 0; u < 64 * 80 / HQ_NT; ++u) {
+  for (int u = 0; u < kPa; ++u) {
     const int idx = t + u * HQ_NT, p = idx & 63, i = idx >> 6;
-    As[p][i] = (p < b && i < r) ? -a.C2[p + (long)i * a.ldc2] : 0.0;
+    va[u] = (p < b && i < r) ? -a.C2[p + (long)i * a.ldc2] : 0.0;
   }
 #pragma unroll
-  for (int u = 0; u < 64 * 64 / HQ_NT; ++u) {
+  for (int u = 0; u < kPb; ++u) {
     const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
-    Bs[p][c] = (p < b && c0 + c < a.nc) ? a.R[p + (c0 + c) * a.ldr] : 0.0;
+    vb[u] = (p < b && c0 + c < a.nc) ? a.R[p + (c0 + c) * a.ldr] : 0.0;
   }
   const int cw = wave * 16 + l15;
   const long gc = c0 + cw;
@@ -605,6 +609,16 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
       const int i = q * 16 + l4 + 4 * e;
       acc[q][e] = (cok && i < r) ? a.Y2[i + gc * a.ldy] : 0.0;
     }
+#pragma unroll
+  for (int u = 0; u < kPa; ++u) {
+    const int idx = t + u * HQ_NT;
+    As[idx & 63][idx >> 6] = va[u];
+  }
+#pragma unroll
+  for (int u = 0; u < kPb; ++u) {
+    const int idx = t + u * HQ_NT;
+    Bs[idx & 63][idx >> 6] = vb[u];
+  }
   __syncthreads();
 #pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {

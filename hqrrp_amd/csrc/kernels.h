@@ -203,10 +203,13 @@ hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
                             double* dst, long ldd, hipStream_t s);
 hipError_t launch_copy_block(double* dst, long ldd, const double* src,
                              long lds, int rows, int cols, hipStream_t s);
-hipError_t launch_select_cols(const int* moves, int b2, const double* C,
-                              long ldc, int r0, int rows, const double* Wt,
-                              long ldw, int b, double* P, long ldp,
-                              double* Wsel, hipStream_t s);
+// Look-ahead: P(:, q) = C(r0.., src(q)) - V W'(src(q), :)' for the b2
+// columns the next panel will hold (src from the composed moves; null:
+// identity).
+hipError_t launch_select_update(const int* moves, int b2, const double* C,
+                                long ldc, int r0, int rows, const double* V,
+                                long ldv, const double* Wt, long ldw, int b,
+                                double* P, long ldp, hipStream_t s);
 // Moves m rows of A's columns (A may point at any row), r rows of Y's
 // columns (r = 0: none; Y's elements are ysz = 4 or 8 bytes, ldy in
 // elements) and jpvt (null: none).
