@@ -45,7 +45,8 @@ PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 # profiles/trail_update_16384_hbm_counters.md), each counter scaled by what it
 # reads for a plain copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE
 # x1.0).  Keyed by (phase, m, n, nb); other shapes report null.
-MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64): round(4.658e11 / 765)}
+MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64):
+                    round((523284.3 * 2.0 + 713515.4) * 1024)}
 
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {
@@ -58,6 +59,7 @@ PHASE_BOUND = {
     "trail_tw": "mfma",
     "trail_update": "hbm",
     "downdate": "hbm",
+    "select_update": "hbm",
 }
 
 

@@ -139,15 +139,16 @@ enum {
   PH_GATHER,           // A/Y/jpvt column gathers, look-ahead column copies
   PH_PANEL,            // panel QRCP kernel
   PH_PANEL_POST,       // T factor + permutation of the rows above the panel
-  PH_TRAIL_VTC,        // trailing M = V'C (+ split-K reduction)
-  PH_TRAIL_TW,         // W = T'M
+  PH_TRAIL_VTC,        // trailing M = V'C (split-K partials)
+  PH_TRAIL_TW,         // W = T'M; in the FP64 driver also R12, B' and Y2
   PH_TRAIL_UPD,        // trailing C -= V W
-  PH_DOWNDATE,         // sketch downdate
+  PH_DOWNDATE,         // sketch downdate (FP32 sketch)
+  PH_SELECT,           // look-ahead: next panel's columns selected + updated
   HQ_PHASES
 };
 static const char* const kPhaseNames[HQ_PHASES] = {
     "sketch_gemm", "sketch_qrcp", "gather", "panel", "panel_post",
-    "trail_vtc", "trail_tw", "trail_update", "downdate"};
+    "trail_vtc", "trail_tw", "trail_update", "downdate", "select_update"};
 
 struct hqrrp_handle_s {
   int device = 0;
@@ -506,7 +507,7 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
 }
 
 int trail_vtc_only(hqrrp_handle_t h, const TrailArgs& a) {
-  const double mn = (double)a.mrows * a.nc, b = a.b;
+  const double mn = (double)a.mrows * (a.nc + (a.C2 ? a.nc2 : 0)), b = a.b;
   PhaseScope ps(h, PH_TRAIL_VTC, 2.0 * b * mn, 8.0 * mn + 8.0 * a.mrows * b,
                 1);
   HQ_TRY(launch_trail_vtc(a, h->cur));
@@ -524,8 +525,8 @@ int trail_w(hqrrp_handle_t h, const TrailArgs& a) {
 
 int trail_apply(hqrrp_handle_t h, const TrailArgs& a) {
   if (a.mrows <= 0 || a.nc <= 0) return HQRRP_OK;
-  // C is read and written once; V and W are small.
-  const double mn = (double)a.mrows * a.nc;
+  // C (and C2) are read and written once; V and W are small.
+  const double mn = (double)a.mrows * (a.nc + (a.C2 ? a.nc2 : 0));
   PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * a.b * mn, 16.0 * mn, 1);
   HQ_TRY(launch_trail_update(a, h->cur));
   return HQRRP_OK;
@@ -929,7 +930,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       {
         // The next panel's columns, selected and updated in one pass (the
         // same products, in the same order, as the full update's).
-        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)mnext * b2,
+        PhaseScope ps(h, PH_SELECT, 2.0 * b * (double)mnext * b2,
                       16.0 * (double)mnext * b2, 1);
         HQ_TRY(launch_select_update(last_next ? nullptr : h->moves.as<int>(),
                                     b2, C, ldx[xc], b, mnext, Ajj + b, lda,
