@@ -140,9 +140,9 @@ enum {
   PH_PANEL,            // panel QRCP kernel
   PH_PANEL_POST,       // T factor + permutation of the rows above the panel
   PH_TRAIL_VTC,        // trailing M = V'C (split-K partials)
-  PH_TRAIL_TW,         // W = T'M; in the FP64 driver also R12, B' and Y2
+  PH_TRAIL_TW,         // W = T'M; in the driver also R12, B' and Y2
   PH_TRAIL_UPD,        // trailing C -= V W
-  PH_DOWNDATE,         // sketch downdate (FP32 sketch)
+  PH_DOWNDATE,         // sketch downdate (component entry only)
   PH_SELECT,           // look-ahead: next panel's columns selected + updated
   HQ_PHASES
 };
@@ -174,7 +174,7 @@ struct hqrrp_handle_s {
   // Sketch precision of the running factorisation: Y and G hold floats
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
-  DevBuf R12, Pn;  // look-ahead: updated R12 rows (FP32 sketch), next panel
+  DevBuf Pn;  // look-ahead: the next panel
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
@@ -587,37 +587,13 @@ int downdate_y(hqrrp_handle_t h, int r, int b, int mU, int nY2, const double* U,
   return HQRRP_OK;
 }
 
-// downdate_y with G and Y2 in FP32 (the mixed-precision sketch): the
-// same four products in FP32 arithmetic on the FP64 reflectors, T and R12.
-int downdate_y32(hqrrp_handle_t h, int r, int b, int mU, int nY2,
-                 const double* U, long ldu, const double* R12, long ldr,
-                 const double* T, long ldt, float* Y2, long ldy, float* G,
-                 long ldg) {
-  HQ_RET(ensure(h->Pp, sizeof(float) * (size_t)r * b));
-  HQ_RET(ensure(h->Qm, sizeof(float) * (size_t)r * b));
-  float* P = h->Pp.as<float>();
-  float* Q = h->Qm.as<float>();
-  HQ_RET(gemm32(h, false, false, r, b, mU, 1.0f, G, ldg, 0, 0, U, ldu, 1, 1,
-                0.0f, P, r, 0));
-  HQ_RET(gemm32(h, false, false, r, b, b, 1.0f, P, r, 0, 0, T, ldt, 1, 0,
-                0.0f, Q, r, 0, false));
-  HQ_RET(gemm32(h, false, true, r, mU, b, -1.0f, Q, r, 0, 0, U, ldu, 1, 1,
-                1.0f, G, ldg, 0, false));
-  if (nY2 > 0)
-    HQ_RET(gemm32(h, false, false, r, nY2, b, -1.0f, G, ldg, 0, 0, R12, ldr, 1,
-                  0, 1.0f, Y2, ldy, 0, false));
-  return HQRRP_OK;
-}
 
 int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
+  // The driver keeps Omega' (m x r, ld m, FP64) for both sketch precisions:
+  // it is updated as extra columns of the FP64 trailing matrix.
   HQ_RET(ensure(h->G, sizeof(double) * (size_t)r * m));
   if (o.omega_kind == HQRRP_OMEGA_USER) {
     if (!o.omega) return HQRRP_ERR_ARG;
-    if (h->sk32) {
-      HQ_TRY(launch_cast_f32(o.omega, h->G.as<float>(), (long)r * m, h->cur));
-      return HQRRP_OK;
-    }
-    // The FP64 driver keeps Omega' (m x r, ld m).
     HQ_TRY(launch_transpose(o.omega, r, r, m, h->G.as<double>(), m, h->cur));
     return HQRRP_OK;
   }
@@ -630,11 +606,6 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
     h->om_r = r;
     h->om_m = m;
     h->om_seed = o.seed;
-  }
-  if (h->sk32) {
-    HQ_TRY(launch_cast_f32(h->Gcache.as<double>(), h->G.as<float>(),
-                           (long)r * m, h->cur));
-    return HQRRP_OK;
   }
   HQ_TRY(launch_transpose(h->Gcache.as<double>(), r, r, m, h->G.as<double>(),
                           m, h->cur));
@@ -676,17 +647,6 @@ int block_permute(hqrrp_handle_t h, int j, int b, double* dA, long lda,
   return HQRRP_OK;
 }
 
-// The FP32 sketch's downdate (G: r x m floats), after the trailing update.
-int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
-                   const double* V, long ldv, const double* R12, long ldr,
-                   void* Y, void* G, int nb) {
-  PhaseScope ps(h, PH_DOWNDATE,
-                4.0 * r * b * (double)mrows + 2.0 * r * b * nc,
-                4.0 * (2.0 * r * mrows + 2.0 * r * nc), 4);
-  return downdate_y32(h, r, b, mrows, nc, V, ldv, R12, ldr, h->T.as<double>(),
-                      nb, static_cast<float*>(sk_at(h, Y, (long)(j + b) * r)),
-                      r, static_cast<float*>(sk_at(h, G, (long)j * r)), r);
-}
 
 // FP64 sketch downdate (NoFLA_Downdate_Y, NoFLA_HQRRP_WY_blk_var4.c:617-708,
 // and the G update 743-771) folded into the trailing update: G' = G_R'
@@ -697,7 +657,7 @@ int block_downdate(hqrrp_handle_t h, int r, int j, int b, int mrows, int nc,
 // update applies G' -= V Q' (G_R := G_R (I - V T V')) next to C -= V W.
 // block_tail launches the tail and then Y2 -= B R12.
 int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int j, int jn,
-               double* dA, long lda, double* Y, int r) {
+               double* dA, long lda, void* Y, int r) {
   const double b = ta.b, nc = ta.ncv;
   PhaseScope ps(h, PH_TRAIL_TW,
                 4.0 * b * b * nc + (ta.C2 ? 2.0 * r * b * ta.nc : 0.0),
@@ -724,7 +684,11 @@ int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int j, int jn,
   t.ldc2 = ta.ldc2;
   t.W = ta.W;
   t.ldw = ta.ldw;
-  t.Y2 = Y + (long)jn * r;
+  // Y2 in the sketch's precision (FP32: computed in FP64, stored rounded).
+  if (h->sk32)
+    t.Y2f = static_cast<float*>(Y) + (long)jn * r;
+  else
+    t.Y2 = static_cast<double*>(Y) + (long)jn * r;
   t.ldy = r;
   HQ_TRY(launch_trail_tail(t, h->cur));
   if (ta.C2) HQ_TRY(launch_trail_y2(t, h->cur));
@@ -745,15 +709,18 @@ TrailArgs below_r12(const TrailArgs& ta) {
 
 // The block loop of NoFLA_HQRRP_WY_blk_var4.c:415-562.
 //
-// Look-ahead: once block j's W' = (T'V'C)' is known, the R12 rows (a copy
-// of C's top b rows, updated by the same update kernel: bitwise the rows the
-// full update writes) and the sketch downdate are computed on the whole
-// chip; then the pivot chain of block j+1 runs on stream s_chain (a reserved
-// quarter of the CUs, so its persistent kernels are co-resident) while block
-// j's trailing update runs on s_upd (the other CUs):
+// Per block: V'[C G'] (split-K partials), then the block tail (W', R12
+// into A, B' over G's top rows, Y2 -= B R12), then the update of the rows
+// below R12 (C -= V W, G' -= V Q').
+//
+// Look-ahead: once block j's tail is done, the pivot chain of block j+1 runs
+// on stream s_chain (a reserved quarter of the CUs, so its persistent
+// kernels are co-resident) while block j's trailing update runs on s_upd
+// (the other CUs):
 //   sketch pivots of block j+1;
-//   the b2 selected columns (rows below R12) copied out of the not yet
-//             updated C and updated by the same kernel into a panel buffer;
+//   the b2 selected columns (rows below R12) read from the not yet updated
+//             C and updated -- the same products in the same order as the
+//             full update's -- into a panel buffer;
 //   panel QRCP and T of block j+1 on that buffer.
 // To let the chain read C while the update rewrites it, the update runs out
 // of place: the trailing matrix alternates between A and an internal buffer
@@ -798,7 +765,6 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd &&
                       kmax > nb && n > nb;
   if (use_la) {
-    HQ_RET(ensure(h->R12, sizeof(double) * (size_t)nb * n));
     HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
     HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
   }
@@ -815,21 +781,20 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   {
     PhaseScope ps(h, PH_SKETCH_GEMM, 2.0 * r * (double)m * n,
                   8.0 * ((double)m * n + (double)r * (m + n)), 1);
+    // G holds Omega' (m x r, FP64); the mixed sketch forms Y in FP32
+    // (optionally bf16-rounded operands) with FP32 accumulation.
     if (h->sk32)
-      HQ_RET(gemm32(h, false, false, r, n, m, 1.0f, G, r, 0, 0, dA, lda, 1, 0,
+      HQ_RET(gemm32(h, true, false, r, n, m, 1.0f, G, m, 1, 0, dA, lda, 1, 0,
                     0.0f, static_cast<float*>(Y), r, h->skbf));
-    else  // G holds Omega' (m x r)
+    else
       HQ_RET(gemm(h, true, false, r, n, m, 1.0, static_cast<double*>(G), m, 0,
                   dA, lda, 0, 0.0, static_cast<double*>(Y), r));
   }
 
-  // FP64 sketch: G' (m x r, ld m) is updated as extra columns of the
-  // trailing matrix and the downdate of Y follows the block tail
-  // (block_tail); the FP32 sketch keeps G (r x m) and the four products of
-  // downdate_y32 after the trailing update.
-  const bool fused = !h->sk32;
+  // G' (m x r, ld m, FP64) is updated as extra columns of the trailing
+  // matrix and the downdate of Y follows the block tail (block_tail); Y is
+  // FP64 or, for the mixed sketch, FP32.
   double* Gt = static_cast<double*>(G);
-  double* Yd = static_cast<double*>(Y);
   bool have_panel = false;  // block j's panel already factored by look-ahead
   for (int j = 0; j < kmax; j += nb) {
     const int b = std::min(nb, std::min(n - j, m - j));
@@ -865,57 +830,27 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     }
     double* C = X[xc] + j + (long)jn * ldx[xc];
     TrailArgs ta;
-    // With the FP64 sketch and a next block to pivot, G' rows j.. ride along.
-    const bool gcols = fused && !last;
+    // With a next block to pivot, G' rows j.. ride along.
+    const bool gcols = !last;
     HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta,
                          gcols ? Gt + j : nullptr, m, gcols ? r : 0));
-    if (fused) {
-      HQ_RET(trail_vtc_only(h, ta));
-      HQ_RET(block_tail(h, ta, j, jn, dA, lda, Yd, r));
-    } else {
-      HQ_RET(trail_w(h, ta));
-    }
+    HQ_RET(trail_vtc_only(h, ta));
+    HQ_RET(block_tail(h, ta, j, jn, dA, lda, Y, r));
     if (!la) {
-      if (fused) {
-        HQ_RET(trail_apply(h, below_r12(ta)));
-      } else {
-        HQ_RET(trail_apply(h, ta));
-        if (!last)
-          HQ_RET(block_downdate(h, r, j, b, mrows, nc, Ajj, lda, C, lda, Y, G,
-                                nb));
-      }
+      HQ_RET(trail_apply(h, below_r12(ta)));
       continue;
     }
 
     // jn < kmax implies !last and b == nb.
     const int xn = xc ^ 1;
-    double* R12 = h->R12.as<double>();
     double* Pn = h->Pn.as<double>();
     const int mnext = m - jn;
     const int b2 = std::min(nb, std::min(n - jn, m - jn));
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
-    if (!fused) {
-      // R12 rows and the sketch downdate on the whole chip.
-      {
-        PhaseScope ps(h, PH_GATHER, 0.0, 16.0 * (double)b * nc, 1);
-        HQ_TRY(launch_copy_block(R12, b, C, ldx[xc], b, nc, h->cur));
-      }
-      {
-        PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * b * (double)b * nc,
-                      16.0 * (double)b * nc, 1);
-        TrailArgs tr = ta;
-        tr.mrows = b;
-        tr.C = R12;
-        tr.ldc = b;
-        HQ_TRY(launch_trail_update(tr, h->cur));
-      }
-      HQ_RET(
-          block_downdate(h, r, j, b, mrows, nc, Ajj, lda, R12, b, Y, G, nb));
-    }
     ta.Cout = X[xn] + j + (long)jn * ldx[xn];
     ta.ldco = ldx[xn];
-    // (FP64 sketch: R12 is already in A, the update covers the rows below.)
-    const TrailArgs tu = fused ? below_r12(ta) : ta;
+    // R12 is already in A: the update covers the rows below it.
+    const TrailArgs tu = below_r12(ta);
     // Fork: block j's update || block jn's pivot chain.
     HQ_TRY(hipEventRecord(h->ev_fork, main_s));
     HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_fork, 0));
@@ -949,11 +884,6 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     // Join: the R12 rows into A (when the update wrote them elsewhere), block
     // jn's columns into place (rows above jn in A, the rest in the trailing
     // buffer), its factored panel over them, and the row permutation.
-    if (xn != 0 && !fused) {
-      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)b * nc, 1);
-      HQ_TRY(launch_copy_block(dA + j + (long)jn * lda, lda, R12, b, b, nc,
-                               h->cur));
-    }
     if (!last_next) {
       if (xn == 0) {
         HQ_RET(block_gather(h, r, jn, m, b2, dA, lda, Y, djpvt));
@@ -1206,7 +1136,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
-                    &h->hA,    &h->hJ,    &h->hT,     &h->hW,     &h->R12,
+                    &h->hA,    &h->hJ,    &h->hT,     &h->hW,     
                     &h->Pn,    &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)

@@ -574,7 +574,8 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
 }
 
 // Y2(:, c) -= B R12(:, c) on 64-column tiles of C, B = (C2(0:b, :))' (r x b,
-// r = nc2 <= 80) and R12 = R(0:b, :).  Wavefront w owns columns
+// r = nc2 <= 80) and R12 = R(0:b, :); an FP32 Y2 (the mixed sketch) is read
+// into and computed in FP64 and stored rounded.  Wavefront w owns columns
 // 16w..16w+15 and every 16-row block of r; Y2 goes straight into the
 // accumulators.
 __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
@@ -607,7 +608,8 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int i = q * 16 + l4 + 4 * e;
-      acc[q][e] = (cok && i < r) ? a.Y2[i + gc * a.ldy] : 0.0;
+      const long at = i + gc * a.ldy;
+      acc[q][e] = (cok && i < r) ? (a.Y2f ? (double)a.Y2f[at] : a.Y2[at]) : 0.0;
     }
 #pragma unroll
   for (int u = 0; u < kPa; ++u) {
@@ -633,7 +635,13 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int i = q * 16 + l4 + 4 * e;
-      if (cok && i < r) a.Y2[i + gc * a.ldy] = acc[q][e];
+      const long at = i + gc * a.ldy;
+      if (cok && i < r) {
+        if (a.Y2f)
+          a.Y2f[at] = (float)acc[q][e];
+        else
+          a.Y2[at] = acc[q][e];
+      }
     }
 }
 }  // namespace

@@ -163,6 +163,8 @@ struct TailArgs {
   double* W;  // W' out (ncv x b), ld ldw
   long ldw;
   double* Y2;  // r x nc, ld ldy: Y2 -= C2(0:b, :)' R12 (r = nc2)
+  float* Y2f;  // the same for an FP32 sketch (computed in FP64, stored
+               // rounded); exactly one of Y2, Y2f is set
   long ldy;
 };
 // W', R12 (and C2's top rows); then, with Y2, the sketch downdate.
