@@ -64,11 +64,17 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   const int cc = t & 63, g = t >> 6;
   const bool col_ok = cc < b;
   const long glo = row0 + kRT * g;  // first global row of my group
-  // Position of global row kk inside my group, clamped to [-1, kRT]:
-  // -1: every row of mine is below kk, kRT: every row of mine is above it.
+  // The wavefront holding row k keeps it in x[0]: after each step it stashes
+  // its finished row in `stage` and shifts its registers up by one, so x[i]
+  // holds row glo + sh + i and no per-row test against k is needed.  Only
+  // rows k < b <= 64 are ever finished, so only wavefronts 0 and 1 of
+  // workgroup 0 shift, and `stage` (idle during the steps) holds them.
+  int sh = 0;
+  // Row kk relative to my first unfinished row: -1 when all my rows are
+  // below kk, 0 when kk is my first row, kRT when all my rows are done.
   auto rel = [&](long kk) -> int {
-    long v = kk - glo;
-    return (int)(v < -1 ? -1 : (v > kRT ? kRT : v));
+    if (sh >= kRT) return kRT;
+    return kk < glo + sh ? -1 : 0;
   };
   if (t == 0) s_abort = 0;
 
@@ -103,6 +109,22 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   // [q0, q1) into s_acc[g]; group 0 also fetches the row of `rowner` (if
   // >= 0) into s_rowin.  Returns false, in every thread, on failure.
   auto exchange = [&](int e, int rowner) -> bool {
+    if (nw == 1) {
+      // One workgroup: the same sums through LDS only (bitwise what the
+      // published/polled path gives: group 0 adds 0 + the sum, the other
+      // groups have no workgroups to add).  s_red was completed before the
+      // barrier preceding this call; s_acc is not read again before the
+      // barrier below.
+      if (col_ok) {
+        double a1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < kPG; ++q) a1 += s_red[q][cc];
+        s_acc[g][cc] = (g == 0) ? 0.0 + a1 : 0.0;
+        if (g == 0 && rowner == 0) s_rowin[cc] = s_rowk[cc];
+      }
+      __syncthreads();
+      return true;
+    }
     if (t < b) {
       double a1 = 0.0;
 #pragma unroll
@@ -191,7 +213,8 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
       if (r < kRT) {
         if (cc == pc) {
 #pragma unroll
-          for (int i = 0; i < kRT; ++i) s_pv[g][i] = x[i];
+          for (int i = 0; i < kRT; ++i)
+            s_pv[g][i] = (i == 0 && r == 0) ? 0.0 : x[i];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -200,14 +223,10 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
         if (r < 0) {
 #pragma unroll
           for (int i = 0; i < kRT; ++i) s4[i & 3] += pv[i] * x[i];
-        } else {
-          double rk = 0.0;
+        } else {  // row k is x[0]
 #pragma unroll
-          for (int i = 0; i < kRT; ++i) {
-            s4[i & 3] += (i > r) ? pv[i] * x[i] : 0.0;
-            rk = (i == r) ? x[i] : rk;
-          }
-          s_rowk[cc] = rk;
+          for (int i = 1; i < kRT; ++i) s4[i & 3] += pv[i] * x[i];
+          s_rowk[cc] = x[0];
         }
         s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
       }
@@ -238,22 +257,25 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
         else if (cc == pc)
           x[i] = v;
       }
-    } else if (r < kRT) {
+    } else if (r < kRT) {  // row k is x[0]
 #pragma unroll
-      for (int i = 0; i < kRT; ++i) {
+      for (int i = 1; i < kRT; ++i) {
         const double v = pv[i] * h.scale;
-        if (i > r) {
-          if (upd)
-            x[i] += v * f;
-          else if (cc == pc)
-            x[i] = v;
-        } else if (i == r) {
-          if (upd)
-            x[i] += f;
-          else if (cc == pc)
-            x[i] = h.beta;
-        }
+        if (upd)
+          x[i] += v * f;
+        else if (cc == pc)
+          x[i] = v;
       }
+      if (upd)
+        x[0] += f;
+      else if (cc == pc)
+        x[0] = h.beta;
+      // Row k is final: stash it (stage[column][row]) and shift up.
+      if (col_ok) stage[cc][k - row0] = x[0];
+#pragma unroll
+      for (int i = 0; i < kRT - 1; ++i) x[i] = x[i + 1];
+      x[kRT - 1] = 0.0;
+      ++sh;
     }
     if (cc == pc) done = true;
     if (k + 1 == b) break;
@@ -272,13 +294,11 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
         dn = dn * sqrt(tt);
     }
     if (__ballot(need)) {  // identical in every wavefront of every workgroup
+      // Every row still held is below k (the row-k wavefront has shifted).
       double s = 0.0;
-      if (r < 0) {
+      if (r < kRT) {
 #pragma unroll
         for (int i = 0; i < kRT; ++i) s += x[i] * x[i];
-      } else if (r < kRT) {
-#pragma unroll
-        for (int i = 0; i < kRT; ++i) s += (i > r) ? x[i] * x[i] : 0.0;
       }
       s_red[g][cc] = s;
       __syncthreads();
@@ -292,7 +312,20 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     pc = choose(k + 1);
   }
   // ---- write back in pivoted order (position q holds column ph(q)).
+  // A shifted wavefront first adds its unfinished rows to the stash (rows
+  // glo+sh.., from x[0..]), then reloads all its rows from it.
+  const int lg = (int)(glo - row0);
+  if (sh > 0 && col_ok) {
+#pragma unroll
+    for (int i = 0; i < kRT; ++i)
+      if (i < kRT - sh) stage[cc][lg + sh + i] = x[i];
+  }
   if (g == 0 && col_ok) s_pos[ph] = cc;
+  __syncthreads();
+  if (sh > 0 && col_ok) {
+#pragma unroll
+    for (int i = 0; i < kRT; ++i) x[i] = stage[cc][lg + i];
+  }
   __syncthreads();
   const int myq = col_ok ? s_pos[cc] : 0;
 #pragma unroll
