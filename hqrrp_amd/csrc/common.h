@@ -125,6 +125,83 @@ __device__ __forceinline__ void hq_wave_argmax(double& key, int& pos,
   key = m;
 }
 
+// Value of v in lane `lane` (wave-uniform) for every lane: two scalar reads,
+// no LDS round trip.
+__device__ __forceinline__ double hq_readlane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int hq_readlane(int v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+// Value of v in lane l ^ 32 (the other half of the wavefront) by gfx950's
+// v_permlane32_swap (a VALU swap of the lower half of one register with the
+// upper half of another; no LDS).  With both operands holding v, each lane
+// ends up with its own value in one result and its partner's in the other.
+__device__ __forceinline__ unsigned hq_xor32_u32(unsigned v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (threadIdx.x & 32) ? r[0] : r[1];
+}
+__device__ __forceinline__ float hq_xor32(float v) {
+  return __uint_as_float(hq_xor32_u32(__float_as_uint(v)));
+}
+__device__ __forceinline__ double hq_xor32(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = hq_xor32_u32((unsigned)b);
+  const unsigned hi = hq_xor32_u32((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One argmax step: (key, pos) becomes the better of itself and (ko, po) --
+// the larger key, the smaller position among equal keys.
+__device__ __forceinline__ void hq_argmax_take(double& key, int& pos, double ko,
+                                               int po) {
+  const bool take = (ko > key) | ((ko == key) & (po < pos));  // branch-free
+  key = take ? ko : key;
+  pos = take ? po : pos;
+}
+// Combines each lane with the lane that DPP control CTRL selects inside its
+// row of 16 (quad permutations and mirrors: a VALU move, no LDS).
+template <int CTRL>
+__device__ __forceinline__ void hq_argmax_dpp(double& key, int& pos) {
+  const long long b = __double_as_longlong(key);
+  const unsigned lo =
+      (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(
+      0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int po = __builtin_amdgcn_update_dpp(0, pos, CTRL, 0xF, 0xF, false);
+  hq_argmax_take(
+      key, pos,
+      __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo)),
+      po);
+}
+// Argmax over the 64 lanes of a wavefront with the rule of hq_argmax_take
+// (idamax's: largest key, first position among equals); every lane receives
+// the result.  Within each row of 16 lanes: xor 1, xor 2 (quad
+// permutations), then the half-row and row mirrors, each pairing every lane
+// with one holding the other half of its group; the four row results are
+// then combined from scalar reads of lanes 0, 16, 32, 48.  Positions must be
+// unique among lanes with equal keys unless those lanes hold the same pair.
+// All 64 lanes must be active.
+__device__ __forceinline__ void hq_wave_argmax64(double& key, int& pos) {
+  hq_argmax_dpp<0xB1>(key, pos);   // quad_perm [1,0,3,2]
+  hq_argmax_dpp<0x4E>(key, pos);   // quad_perm [2,3,0,1]
+  hq_argmax_dpp<0x141>(key, pos);  // row_half_mirror
+  hq_argmax_dpp<0x140>(key, pos);  // row_mirror
+  double k0 = hq_readlane(key, 0), k1 = hq_readlane(key, 16);
+  double k2 = hq_readlane(key, 32), k3 = hq_readlane(key, 48);
+  int p0 = hq_readlane(pos, 0), p1 = hq_readlane(pos, 16);
+  int p2 = hq_readlane(pos, 32), p3 = hq_readlane(pos, 48);
+  hq_argmax_take(k0, p0, k1, p1);
+  hq_argmax_take(k2, p2, k3, p3);
+  hq_argmax_take(k0, p0, k2, p2);
+  key = k0;
+  pos = p0;
+}
+
 // Largest finite value of T (LAPACK's overflow threshold in dlapy2/slapy2).
 template <class T>
 __device__ __forceinline__ T hq_huge();

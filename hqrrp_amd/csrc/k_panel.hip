@@ -55,7 +55,6 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   __shared__ double s_rowk[kMaxB];      // row k, staged by its owner group
   __shared__ double s_rowin[kMaxB];     // row k, as received
   __shared__ double s_pv[kPG][kRT];     // each group's rows of the pivot column
-  __shared__ int s_pos[kMaxB];
   __shared__ int s_abort;
 
   const int t = threadIdx.x, w = blockIdx.x;
@@ -165,8 +164,9 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   };
 
   // Per-column state, one copy per wavefront (lane c: column c): partial
-  // norm dn, its reference value en (LAWN 176's vn2), done flag; and the
-  // pivoting map ph = column at position c.
+  // norm dn, its reference value en (LAWN 176's vn2), done flag, and the
+  // column's current position pos (the pivoting swaps positions, columns
+  // stay in their lanes).
   {
     double s = 0.0;
 #pragma unroll
@@ -177,26 +177,29 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   if (!exchange(0, -1)) return;
   double dn = col_ok ? sqrt(full(cc)) : 0.0, en = dn;
   bool done = !col_ok;
-  int ph = cc;
+  int pos = cc;
 
   // Pivot of step k (idamax over positions k..b-1: largest partial norm,
-  // first position on ties); swaps positions k and p in ph and returns the
-  // column now at position k.
+  // first position on ties): the argmax runs over the columns at positions
+  // >= k keyed by (norm, position), so no gather by position is needed.
+  // Swaps positions k and p and returns the column now at position k.
   auto choose = [&](int k) -> int {
     int p = k;
     if (a.pivoting) {
-      const double nv = __shfl(dn, ph, 64);
-      double key = (cc >= k && cc < b) ? nv : -1.0;
-      int q = (cc >= k && cc < b) ? cc : 1 << 30;
-      hq_wave_argmax(key, q, 64);
+      const bool cand = col_ok && pos >= k;
+      double key = cand ? dn : -1.0;
+      int q = cand ? pos : 1 << 30;
+      hq_wave_argmax64(key, q);
       p = q;
     }
-    const int at_k = __shfl(ph, k, 64), at_p = __shfl(ph, p, 64);
-    if (cc == k)
-      ph = at_p;
-    else if (cc == p)
-      ph = at_k;
-    return at_p;
+    const int col = __builtin_ctzll(__ballot(col_ok && pos == p));
+    if (col_ok) {
+      if (pos == p)
+        pos = k;
+      else if (pos == k)
+        pos = p;
+    }
+    return col;
   };
 
   int pc = choose(0);
@@ -235,15 +238,15 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     __syncthreads();
     if (!exchange(1 + 2 * k, (int)(k / kPR))) return;
 
-    // 2. Reflector, dots and gram entries (every thread, identical bits).
-    const hq_reflector h = hq_dlarfg(s_rowin[pc], full(pc));
+    // 2. Reflector, dots and gram entries (every thread, identical bits):
+    //    the pivot's row-k entry and G_p are read from the pivot's lane.
+    const double gsum = full(cc), rin = s_rowin[cc];
+    const hq_reflector h = hq_dlarfg(hq_readlane(rin, pc), hq_readlane(gsum, pc));
     if (w == 0 && t == 0) a.tau[k] = h.tau;
-    const double xk = col_ok ? s_rowin[cc] : 0.0;
-    const double dot = xk + h.scale * (col_ok ? full(cc) : 0.0);
-    {
-      const double dpos = __shfl(dot, ph, 64);  // dot of the column at my position
-      if (w == 0 && t < k) a.gram[t + (long)k * b] = dpos;
-    }
+    const double xk = col_ok ? rin : 0.0;
+    const double dot = xk + h.scale * (col_ok ? gsum : 0.0);
+    // Gram entry (position of my column, k) for the factored columns.
+    if (w == 0 && g == 0 && col_ok && pos < k) a.gram[pos + (long)k * b] = dot;
     // 3. dlarf on my rows >= k: pending columns x += v * (-tau * dot), the
     //    pivot column becomes (beta; v).
     const double f = -h.tau * dot;
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     // 5. Next pivot.
     pc = choose(k + 1);
   }
-  // ---- write back in pivoted order (position q holds column ph(q)).
+  // ---- write back in pivoted order (my column goes to position pos).
   // A shifted wavefront first adds its unfinished rows to the stash (rows
   // glo+sh.., from x[0..]), then reloads all its rows from it.
   const int lg = (int)(glo - row0);
@@ -320,14 +323,13 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     for (int i = 0; i < kRT; ++i)
       if (i < kRT - sh) stage[cc][lg + sh + i] = x[i];
   }
-  if (g == 0 && col_ok) s_pos[ph] = cc;
   __syncthreads();
   if (sh > 0 && col_ok) {
 #pragma unroll
     for (int i = 0; i < kRT; ++i) x[i] = stage[cc][lg + i];
   }
   __syncthreads();
-  const int myq = col_ok ? s_pos[cc] : 0;
+  const int myq = col_ok ? pos : 0;
 #pragma unroll
   for (int rd = 0; rd < kPR / 128; ++rd) {
     if (col_ok && g / (128 / kRT) == rd) {
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
     __syncthreads();
   }
-  if (w == 0 && g == 0 && col_ok) a.perm[cc] = ph;
+  if (w == 0 && g == 0 && col_ok) a.perm[pos] = cc;
   hq_xreset(a.xprev, a.xprev_len, w, nw, kPT);
 }
 

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   T ss = 0;
 #pragma unroll
   for (int q = 0; q < H; ++q) ss += x[q] * x[q];
-  ss += __shfl_xor(ss, 32, 64);
+  ss += hq_xor32(ss);
   T d = sqrt(ss), e = d;
   // Squared norm of the rows below the first active one (the ||x||^2 of
   // this column's reflector, should it win).
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 #pragma unroll
     for (int q = 1; q < H; ++q) s1 += x[q] * x[q];
     s1 += half ? x[0] * x[0] : 0.0;
-    sqn = s1 + __shfl_xor(s1, 32, 64);
+    sqn = s1 + hq_xor32(s1);
   }
   int pos = own ? c : -1;  // current position; -1 once chosen as a pivot
 
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     {
       double key = (pos >= 0) ? (double)d : -1.0;
       int kp = (pos >= 0) ? pos : INT_MAX;
-      hq_wave_argmax(key, kp, 32);  // lanes l, l+32 hold the same column
+      hq_wave_argmax64(key, kp);  // lanes l, l+32 hold the same column
       if (lane == 0) {
         s_key[wave] = key;
         s_pos[wave] = kp;
@@ -119,12 +119,13 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     {
-      // Every wavefront combines the 8 wavefront winners over lane groups
-      // of 8 (identical result in every lane); bq: the winner's wavefront.
+      // Every wavefront combines the 8 wavefront winners (lane l holds
+      // winner l % 8; identical result in every lane); bq: the winner's
+      // wavefront.
       const int wq = s_pos[lane & 7];
       double bk = s_key[lane & 7];
       int bp = wq;
-      hq_wave_argmax(bk, bp, 8);
+      hq_wave_argmax64(bk, bp);
       const int bq = __builtin_ctzll(__ballot(wq == bp) & 0xFFull);
       if (bp == INT_MAX) {  // no candidate here
         if (t == 0) {
@@ -179,11 +180,11 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       // workgroup, alpha and norm.
       const double lk = bk;
       const int lp = bp;
-      hq_wave_argmax(bk, bp, 64);
+      hq_wave_argmax64(bk, bp);
       const int bl = __builtin_ctzll(__ballot(lk == bk && lp == bp));
-      bw = __shfl(bw, bl, 64);
-      ba = __shfl(ba, bl, 64);
-      bs = __shfl(bs, bl, 64);
+      bw = hq_readlane(bw, bl);
+      ba = hq_readlane(ba, bl);
+      bs = hq_readlane(bs, bl);
       if (lane == 0) {
         const hq_reflector_t<T> h = hq_larfg<T>((T)ba, (T)bs);
         s_tau = h.tau;
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
         T wd = 0;
 #pragma unroll
         for (int q = 0; q < H; ++q) wd += s_v[ibase + q] * x[q];
-        wd += __shfl_xor(wd, 32, 64);
+        wd += hq_xor32(wd);
         const T f = -tau * wd;
 #pragma unroll
         for (int q = 0; q < H; ++q) x[q] += f * s_v[ibase + q];
@@ -243,11 +244,11 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 #pragma unroll
       for (int q = 2; q < H; ++q) s3 += x[q] * x[q];
       const T x0 = x[0];
-      const T up = __shfl_xor(x0, 32, 64);  // partner's first row
+      const T up = hq_xor32(x0);  // partner's first row
       b2 += half ? x0 * x0 : T(0);
       s3 += half ? x[1] * x[1] + x0 * x0 : T(0);
-      b2 += __shfl_xor(b2, 32, 64);
-      sqn = s3 + __shfl_xor(s3, 32, 64);
+      b2 += hq_xor32(b2);
+      sqn = s3 + hq_xor32(s3);
       const T wt = half ? up : x0;  // row k
       hq_downdate_norm<T>(d, e, wt, b2, r - k - 1);
 #pragma unroll
