@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
   __shared__ int s_win_pos, s_win_wg;
-  __shared__ T s_tau, s_scale;
+  __shared__ double s_alpha, s_sqn;  // the winner's row k and ||x||^2
   __shared__ int s_piv[kMaxStages];
   __shared__ int s_abort;
 
@@ -86,16 +86,15 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   for (int q = 0; q < H; ++q) ss += x[q] * x[q];
   ss += hq_xor32(ss);
   T d = sqrt(ss), e = d;
-  // Squared norm of the rows below the first active one (the ||x||^2 of
-  // this column's reflector, should it win).
-  T sqn;
-  {
+  // Squared norm of this column's rows below the first active one (the
+  // ||x||^2 of its reflector, or LAWN 176's recompute): lane pair sum.
+  auto below2 = [&]() -> T {
     T s1 = 0;
 #pragma unroll
     for (int q = 1; q < H; ++q) s1 += x[q] * x[q];
-    s1 += half ? x[0] * x[0] : 0.0;
-    sqn = s1 + hq_xor32(s1);
-  }
+    s1 += half ? x[0] * x[0] : T(0);
+    return s1 + hq_xor32(s1);
+  };
   int pos = own ? c : -1;  // current position; -1 once chosen as a pivot
 
   if (t == 0) s_abort = 0;
@@ -137,7 +136,9 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       } else if (wave == bq) {
         // The winner's two lanes stage the slot in LDS and their wavefront
         // publishes it with two coalesced stores (the LDS traffic stays
-        // inside this wavefront, ordered by wavefront-scope fences).
+        // inside this wavefront, ordered by wavefront-scope fences).  Only
+        // this wavefront computes the ||x||^2 that the slot carries.
+        const T sqn = below2();
         if (pos == bp) {
 #pragma unroll
           for (int q = 0; q < H; ++q) s_slot[kSlotHead + ibase + q] = x[q];
@@ -186,9 +187,8 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       ba = hq_readlane(ba, bl);
       bs = hq_readlane(bs, bl);
       if (lane == 0) {
-        const hq_reflector_t<T> h = hq_larfg<T>((T)ba, (T)bs);
-        s_tau = h.tau;
-        s_scale = h.scale;
+        s_alpha = ba;
+        s_sqn = bs;
         s_win_pos = bp;
         s_win_wg = bw;
         s_piv[k] = bp;
@@ -200,18 +200,20 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     if (s_abort) return;
     const int p = s_win_pos;
     // 3. the winner's rows below k, scaled, as the reflector (relative rows,
-    //    1 at row k, zero beyond span).
-    if (t < RMAX) {
-      T v = 0;
-      if (t > 0 && t < span) {
-        double cv[1];
-        const double* src = cbuf + (long)s_win_wg * slot + kSlotHead + t;
-        ok = hq_await(cv, [&](int) -> const double* { return src; }, a.err);
-        v = (T)cv[0] * s_scale;
-      }
-      s_v[t] = (t == 0) ? T(1) : v;
+    //    1 at row k, zero beyond span).  Every thread builds the reflector's
+    //    scalars (dlarfg needs only alpha and ||x||^2) while the rows are
+    //    in flight.
+    const bool fetch = t > 0 && t < RMAX && t < span;
+    const double* src = cbuf + (long)s_win_wg * slot + kSlotHead + t;
+    double cv = fetch ? hq_fetch(src) : 0.0;
+    const hq_reflector_t<T> h = hq_larfg<T>((T)s_alpha, (T)s_sqn);
+    if (fetch && hq_pending(cv)) {
+      double c1[1];
+      ok = hq_await(c1, [&](int) -> const double* { return src; }, a.err);
+      cv = c1[0];
       if (!ok) s_abort = 1;
     }
+    if (t < RMAX) s_v[t] = (t == 0) ? T(1) : (T)cv * h.scale;
     __syncthreads();
     if (s_abort) return;
     // 4. bookkeeping of positions (swap of positions k and p).
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    active rows up by one (row k leaves).  Both lanes of a column take
     //    the same branches (pos is per column).
     if (pos >= 0) {
-      const T tau = s_tau;
+      const T tau = h.tau;
       if (tau != T(0)) {
         T wd = 0;
 #pragma unroll
@@ -236,21 +238,33 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 #pragma unroll
         for (int q = 0; q < H; ++q) x[q] += f * s_v[ibase + q];
       }
-      // b2: rows below k (Drmac's recompute); s3: rows below k+1 (the next
-      // step's ||x||^2).
-      T b2 = 0, s3 = 0;
-#pragma unroll
-      for (int q = 1; q < H; ++q) b2 += x[q] * x[q];
-#pragma unroll
-      for (int q = 2; q < H; ++q) s3 += x[q] * x[q];
+      // LAWN-176 downdate with row k; the rows below k are summed only
+      // when some column of the wavefront needs Drmac's recompute.
       const T x0 = x[0];
       const T up = hq_xor32(x0);  // partner's first row
-      b2 += half ? x0 * x0 : T(0);
-      s3 += half ? x[1] * x[1] + x0 * x0 : T(0);
-      b2 += hq_xor32(b2);
-      sqn = s3 + hq_xor32(s3);
       const T wt = half ? up : x0;  // row k
-      hq_downdate_norm<T>(d, e, wt, b2, r - k - 1);
+      bool need = false;
+      T tt = 0;
+      if (d != T(0)) {
+        const T t1 = fabs(wt) / d;
+        const T u = (T(1) + t1) * (T(1) - t1);
+        tt = (T(0) > u) ? T(0) : u;
+        const T ratio = d / e;
+        need = tt * ratio * ratio <= hq_tol3z<T>();
+        if (!need) d = d * sqrt(tt);
+      }
+      if (__ballot(need)) {
+        const T b2 = below2();
+        if (need) {
+          if (r - k - 1 > 0) {
+            d = sqrt(b2);
+            e = d;
+          } else {
+            d = T(0);
+            e = T(0);
+          }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < H - 1; ++q) x[q] = x[q + 1];
       x[H - 1] = half ? T(0) : up;
