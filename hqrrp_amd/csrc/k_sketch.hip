@@ -69,8 +69,8 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   __shared__ double s_slot[kSlotHead + RMAX];  // this workgroup's slot
   __shared__ double s_key[8];
   __shared__ int s_pos[8];
-  __shared__ int s_win_pos, s_win_wg;
-  __shared__ double s_alpha, s_sqn;  // the winner's row k and ||x||^2
+  __shared__ int s_win_pos;
+  __shared__ T s_tau;
   __shared__ int s_piv[kMaxStages];
   __shared__ int s_abort;
 
@@ -157,8 +157,12 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
         if (lane + 64 < slot) hq_publish(mine + lane + 64, s_slot[lane + 64]);
       }
     }
-    // 2. wavefront 0: every header, the global winner and the reflector's
-    //    scalars (dlarfg needs only alpha and ||x||^2).
+    // 2. wavefront 0: every header, the global winner, the reflector's
+    //    scalars (dlarfg needs only alpha and ||x||^2) and the winner's rows
+    //    below k, scaled, as the reflector (relative rows, 1 at row k, zero
+    //    beyond span) -- the rows are fetched as soon as the winner is
+    //    known and dlarfg runs while they are in flight; one workgroup
+    //    barrier then hands the reflector to the other wavefronts.
     bool ok = true;
     if (wave == 0) {
       double bk = -2.0, ba = 0.0, bs = 0.0;
@@ -186,11 +190,23 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       bw = hq_readlane(bw, bl);
       ba = hq_readlane(ba, bl);
       bs = hq_readlane(bs, bl);
+      // Lane l: relative rows l and l + 64 of the winner.
+      const double* src = cbuf + (long)bw * slot + kSlotHead;
+      const bool f0 = lane > 0 && lane < span;
+      const bool f1 = lane + 64 < span;
+      double cv[2];
+      cv[0] = f0 ? hq_fetch(src + lane) : 0.0;
+      cv[1] = f1 ? hq_fetch(src + lane + 64) : 0.0;
+      const hq_reflector_t<T> h = hq_larfg<T>((T)ba, (T)bs);
+      if (ok && (hq_pending(cv[0]) || hq_pending(cv[1])))
+        ok = hq_await(cv, [&](int i) -> const double* {
+          return (i == 0 ? f0 : f1) ? src + lane + 64 * i : nullptr;
+        }, a.err);
+      if (lane < RMAX) s_v[lane] = (lane == 0) ? T(1) : (T)cv[0] * h.scale;
+      if (lane + 64 < RMAX) s_v[lane + 64] = (T)cv[1] * h.scale;
       if (lane == 0) {
-        s_alpha = ba;
-        s_sqn = bs;
+        s_tau = h.tau;
         s_win_pos = bp;
-        s_win_wg = bw;
         s_piv[k] = bp;
         if (w == 0) a.piv[k] = bp;
       }
@@ -199,23 +215,6 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     __syncthreads();
     if (s_abort) return;
     const int p = s_win_pos;
-    // 3. the winner's rows below k, scaled, as the reflector (relative rows,
-    //    1 at row k, zero beyond span).  Every thread builds the reflector's
-    //    scalars (dlarfg needs only alpha and ||x||^2) while the rows are
-    //    in flight.
-    const bool fetch = t > 0 && t < RMAX && t < span;
-    const double* src = cbuf + (long)s_win_wg * slot + kSlotHead + t;
-    double cv = fetch ? hq_fetch(src) : 0.0;
-    const hq_reflector_t<T> h = hq_larfg<T>((T)s_alpha, (T)s_sqn);
-    if (fetch && hq_pending(cv)) {
-      double c1[1];
-      ok = hq_await(c1, [&](int) -> const double* { return src; }, a.err);
-      cv = c1[0];
-      if (!ok) s_abort = 1;
-    }
-    if (t < RMAX) s_v[t] = (t == 0) ? T(1) : (T)cv * h.scale;
-    __syncthreads();
-    if (s_abort) return;
     // 4. bookkeeping of positions (swap of positions k and p).
     if (pos >= 0) {
       if (pos == p) {
@@ -228,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    active rows up by one (row k leaves).  Both lanes of a column take
     //    the same branches (pos is per column).
     if (pos >= 0) {
-      const T tau = h.tau;
+      const T tau = s_tau;
       if (tau != T(0)) {
         T wd = 0;
 #pragma unroll
