@@ -248,27 +248,21 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     // Gram entry (position of my column, k) for the factored columns.
     if (w == 0 && g == 0 && col_ok && pos < k) a.gram[pos + (long)k * b] = dot;
     // 3. dlarf on my rows >= k: pending columns x += v * (-tau * dot), the
-    //    pivot column becomes (beta; v).
+    //    pivot column becomes (beta; v).  Branch-free per row: x becomes
+    //    fma(v, fq, x * keep) with (keep, fq) = (1, f) for a pending
+    //    column, (0, 1) for the pivot column and (1, 0) for the others.
     const double f = -h.tau * dot;
     const bool upd = col_ok && !done && cc != pc && h.tau != 0.0;
+    const double keep = (cc == pc) ? 0.0 : 1.0;
+    const double fq = upd ? f : ((cc == pc) ? 1.0 : 0.0);
     if (r < 0) {
 #pragma unroll
-      for (int i = 0; i < kRT; ++i) {
-        const double v = pv[i] * h.scale;
-        if (upd)
-          x[i] += v * f;
-        else if (cc == pc)
-          x[i] = v;
-      }
+      for (int i = 0; i < kRT; ++i)
+        x[i] = __builtin_fma(pv[i] * h.scale, fq, x[i] * keep);
     } else if (r < kRT) {  // row k is x[0]
 #pragma unroll
-      for (int i = 1; i < kRT; ++i) {
-        const double v = pv[i] * h.scale;
-        if (upd)
-          x[i] += v * f;
-        else if (cc == pc)
-          x[i] = v;
-      }
+      for (int i = 1; i < kRT; ++i)
+        x[i] = __builtin_fma(pv[i] * h.scale, fq, x[i] * keep);
       if (upd)
         x[0] += f;
       else if (cc == pc)
