@@ -34,11 +34,16 @@ __device__ __forceinline__ double vmask(double v, long i, long c) {
 // layout maps to 16 consecutive rows of one column of C per lane group:
 // every C load and store moves whole 128-byte lines.  The C tile is loaded
 // straight into the accumulators, so C makes exactly one round trip through
-// HBM.  K = b <= 64 is staged in chunks of 16 through double-buffered LDS
-// (70 KB: two workgroups per CU), the next chunk prefetched into registers
-// while the matrix cores work on the current one.  Row tiles vary fastest
-// across workgroups so concurrently running tiles cover different rows
-// (different HBM channels) of the power-of-two-strided columns.
+// HBM.  Each wavefront owns a 64x64 quadrant of the tile and runs on its own:
+// no LDS and no barrier.  Both MFMA operands go from memory (W' and V stay
+// in L2: every tile of a row or column range re-reads the same panel) into
+// operand registers -- lane l supplies rows/columns l % 16 of reflector
+// k0 + l / 16, 16 lanes covering a 128-byte line -- through a four-step
+// register ring that keeps three K steps of loads in flight while the
+// matrix cores work.  K runs over all 16 steps of 4 (operands beyond b are
+// zero, adding exact zeros), so the ring has no exit branch.  Row tiles vary
+// fastest across workgroups so concurrently running tiles cover different
+// rows (different HBM channels) of the power-of-two-strided columns.
 // Base of virtual column v (C's columns, then C2's from a.nca), or null
 // for the gap and beyond.
 __device__ __forceinline__ const double* vcol(const TrailArgs& a, long v) {
@@ -49,17 +54,12 @@ __device__ __forceinline__ const double* vcol(const TrailArgs& a, long v) {
 }
 
 constexpr int kUT = 128;          // update tile edge
-constexpr int kUK = 16;           // K rows per pipeline stage
-constexpr int kUP = kUT + 8;      // LDS pitch (doubles)
 
 __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
-  __shared__ double Vs[2][kUK][kUP];  // Vs[buf][k][m] = -V(m0+m, k0+k)
-  __shared__ double Ws[2][kUK][kUP];  // Ws[buf][k][n] = W(k0+k, n0+n) (from W')
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 64-row / 64-column quadrant
   const int l15 = lane & 15, l4 = lane >> 4;
   const int b = a.b;
-  const int nchunks = (b + kUK - 1) / kUK;
   const long m0 = (long)blockIdx.x * kUT;
   const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
   // The tile's column range: C's, C2's (updated in place) or the gap.
@@ -111,53 +111,54 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
       }
   }
 
-  constexpr int kPer = kUT * kUK / HQ_NT;  // 8 elements of V and of W'
-  double rv[kPer], rw[kPer];
-  auto load = [&](int kc) {
+  // Operands of K step s (reflectors 4s..4s+3): af[i] = W'(n_i, k), n_i =
+  // n0+64wn+16i+l%16; bf[j] = -V(m_j, k) (unit lower on the top tile),
+  // m_j = m0+64wm+16j+l%16; k = 4s + l/16.  Every load is issued
+  // unconditionally (an out-of-range operand reads the tile's first row or
+  // column, which exists, and is then replaced by zero), so the loads stay
+  // in flight across the ring instead of behind branches.
+  const long wcol = n0 + wn * 64 + l15, vrow = m0 + wm * 64 + l15;
+  auto load = [&](double (&af)[4], double (&bf)[4], int st) {
+    const int k = 4 * st + l4;
+    const bool kin = k < b;
+    const long kw = (kin ? k : 0) * a.ldw, kv = (kin ? k : 0) * a.ldv;
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int idx = t + HQ_NT * q, mn = idx & (kUT - 1), kl = idx >> 7;
-      const int k = kc * kUK + kl;
-      const long gm = m0 + mn, gn = n0 + mn;
-      double v = 0.0;
-      if (k < b && gm < a.mrows) {
-        v = a.V[gm + (long)k * a.ldv];
-        if (masked) v = vmask(v, gm, k);
-      }
-      rv[q] = -v;
-      rw[q] = (k < b && gn < nc) ? a.W[nv + mn + (long)k * a.ldw] : 0.0;
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = kin && wcol + 16 * i < nc;
+      const double w = a.W[nv + (ok ? wn * 64 + 16 * i + l15 : 0) + kw];
+      af[i] = ok ? w : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool ok = kin && vrow + 16 * j < a.mrows;
+      double v = a.V[m0 + (ok ? wm * 64 + 16 * j + l15 : 0) + kv];
+      if (masked) v = vmask(v, vrow + 16 * j, k);
+      bf[j] = ok ? -v : 0.0;
     }
   };
-  auto stash = [&](int buf) {
+  auto mma = [&](const double (&af)[4], const double (&bf)[4]) {
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int idx = t + HQ_NT * q, mn = idx & (kUT - 1), kl = idx >> 7;
-      Vs[buf][kl][mn] = rv[q];
-      Ws[buf][kl][mn] = rw[q];
-    }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+                                                          acc[i][j], 0, 0, 0);
   };
-  load(0);
-  stash(0);
-  __syncthreads();
-  for (int kc = 0; kc < nchunks; ++kc) {
-    const int buf = kc & 1;
-    if (kc + 1 < nchunks) load(kc + 1);
+  constexpr int kSteps = 16;  // 64 / 4
+  double ra[4][4], rb[4][4];
+  load(ra[0], rb[0], 0);
+  load(ra[1], rb[1], 1);
+  load(ra[2], rb[2], 2);
 #pragma unroll
-    for (int kk = 0; kk < kUK; kk += 4) {
-      double af[4], bf[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = Ws[buf][kk + l4][wn * 64 + i * 16 + l15];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Vs[buf][kk + l4][wm * 64 + j * 16 + l15];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
-                                                            acc[i][j], 0, 0, 0);
-    }
-    if (kc + 1 < nchunks) stash(buf ^ 1);
-    __syncthreads();
+  for (int st = 0; st < kSteps; st += 4) {
+    load(ra[3], rb[3], st + 3);
+    mma(ra[0], rb[0]);
+    if (st + 4 < kSteps) load(ra[0], rb[0], st + 4);
+    mma(ra[1], rb[1]);
+    if (st + 5 < kSteps) load(ra[1], rb[1], st + 5);
+    mma(ra[2], rb[2]);
+    if (st + 6 < kSteps) load(ra[2], rb[2], st + 6);
+    mma(ra[3], rb[3]);
   }
   if (interior) {
 #pragma unroll
