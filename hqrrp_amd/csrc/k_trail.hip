@@ -55,38 +55,23 @@ __device__ __forceinline__ const double* vcol(const TrailArgs& a, long v) {
 
 constexpr int kUT = 128;          // update tile edge
 
-__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+// One wavefront's 64x64 quadrant of a tile.  FAST: an interior tile with
+// all 64 reflectors and no unit-lower mask, so no load or store needs a
+// predicate and nothing separates the C and operand loads from their uses.
+template <bool FAST>
+__device__ __forceinline__ void update_quadrant(const TrailArgs& a, long m0,
+                                                long nv, long n0, long nc,
+                                                const double* Cb, long ldc,
+                                                double* Ob, long ldo,
+                                                bool masked, int b) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 64-row / 64-column quadrant
   const int l15 = lane & 15, l4 = lane >> 4;
-  const int b = a.b;
-  const long m0 = (long)blockIdx.x * kUT;
-  const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
-  // The tile's column range: C's, C2's (updated in place) or the gap.
-  const long nv = (long)blockIdx.y * kUT;  // first virtual column (W' row)
-  long n0, nc, ldc, ldo;
-  double *Cb, *Ob;
-  if (nv < a.nc) {
-    n0 = nv;
-    nc = a.nc;
-    Cb = a.C;
-    ldc = a.ldc;
-    Ob = a.Cout ? a.Cout : a.C;
-    ldo = a.Cout ? a.ldco : a.ldc;
-  } else if (a.C2 && nv >= a.nca && nv < a.nca + a.nc2) {
-    n0 = nv - a.nca;
-    nc = a.nc2;
-    Cb = Ob = a.C2;
-    ldc = ldo = a.ldc2;
-  } else {
-    return;
-  }
-
   // acc[i][j] element r: C'(n, m), n = n0+64wn+16i+l/16+4r, m = m0+64wm+16j+l%16,
   // i.e. cbase[16j + (16i+4r)*ldc] with the lane's base below.
-  const bool interior = m0 + kUT <= a.mrows && n0 + kUT <= nc;
+  const bool interior = FAST || (m0 + kUT <= a.mrows && n0 + kUT <= nc);
   const long lm = m0 + wm * 64 + l15, ln = n0 + wn * 64 + l4;
-  double* cbase = Cb + lm + ln * ldc;
+  const double* cbase = Cb + lm + ln * ldc;
   double* obase = Ob + lm + ln * ldo;
   hq_d4 acc[4][4];
   if (interior) {
@@ -120,6 +105,15 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
   const long wcol = n0 + wn * 64 + l15, vrow = m0 + wm * 64 + l15;
   auto load = [&](double (&af)[4], double (&bf)[4], int st) {
     const int k = 4 * st + l4;
+    if (FAST) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        af[i] = a.W[nv + wn * 64 + 16 * i + l15 + (long)k * a.ldw];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bf[j] = -a.V[m0 + wm * 64 + 16 * j + l15 + (long)k * a.ldv];
+      return;
+    }
     const bool kin = k < b;
     const long kw = (kin ? k : 0) * a.ldw, kv = (kin ? k : 0) * a.ldv;
 #pragma unroll
@@ -181,6 +175,37 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
           if (nok && lm + 16 * j < a.mrows) cc[16 * j] = acc[i][j][r];
       }
   }
+}
+
+__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
+  const int b = a.b;
+  const long m0 = (long)blockIdx.x * kUT;
+  const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
+  // The tile's column range: C's, C2's (updated in place) or the gap.
+  const long nv = (long)blockIdx.y * kUT;  // first virtual column (W' row)
+  long n0, nc, ldc, ldo;
+  double *Cb, *Ob;
+  if (nv < a.nc) {
+    n0 = nv;
+    nc = a.nc;
+    Cb = a.C;
+    ldc = a.ldc;
+    Ob = a.Cout ? a.Cout : a.C;
+    ldo = a.Cout ? a.ldco : a.ldc;
+  } else if (a.C2 && nv >= a.nca && nv < a.nca + a.nc2) {
+    n0 = nv - a.nca;
+    nc = a.nc2;
+    Cb = Ob = a.C2;
+    ldc = ldo = a.ldc2;
+  } else {
+    return;
+  }
+
+  const bool fast = m0 + kUT <= a.mrows && n0 + kUT <= nc && b == 64 && !masked;
+  if (fast)
+    update_quadrant<true>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, false, 64);
+  else
+    update_quadrant<false>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, masked, b);
 }
 
 // part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
