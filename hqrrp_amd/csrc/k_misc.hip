@@ -98,6 +98,7 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
   const long mw = (long)blockIdx.x * 128 + wave * 32;  // this wave's rows
   // acc[i][j] element e: P'(n, m), n = 16i + l/16 + 4e, m = mw + 16j + l%16.
   hq_d4 acc[4][2];
+  const bool full = b2 == 64 && mw + 32 <= rows;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -107,12 +108,49 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const long m = mw + 16 * j + l15;
-        acc[i][j][e] = (n < b2 && m < rows) ? cc[m] : 0.0;
+        if (full)
+          acc[i][j][e] = cc[m];
+        else
+          acc[i][j][e] = (n < b2 && m < rows) ? cc[m] : 0.0;
       }
     }
   int wsrc[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) wsrc[i] = (16 * i + l15 < b2) ? src[16 * i + l15] : -1;
+  if (b == 64 && b2 == 64 && mw + 32 <= rows) {
+    // Every operand exists: predicate-free loads through a four-step
+    // register ring (the same products in the same order as below).
+    auto load = [&](double (&af)[4], double (&bf)[2], int st) {
+      const int k = 4 * st + l4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = Wt[wsrc[i] + (long)k * ldw];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = -V[mw + 16 * j + l15 + (long)k * ldv];
+    };
+    auto mma = [&](const double (&af)[4], const double (&bf)[2]) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
+                                                            acc[i][j], 0, 0, 0);
+    };
+    double ra[4][4], rb[4][2];
+    load(ra[0], rb[0], 0);
+    load(ra[1], rb[1], 1);
+    load(ra[2], rb[2], 2);
+#pragma unroll
+    for (int st = 0; st < 16; st += 4) {
+      load(ra[3], rb[3], st + 3);
+      mma(ra[0], rb[0]);
+      if (st + 4 < 16) load(ra[0], rb[0], st + 4);
+      mma(ra[1], rb[1]);
+      if (st + 5 < 16) load(ra[1], rb[1], st + 5);
+      mma(ra[2], rb[2]);
+      if (st + 6 < 16) load(ra[2], rb[2], st + 6);
+      mma(ra[3], rb[3]);
+    }
+  } else {
 #pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
     const int k = kk + l4;
@@ -131,6 +169,7 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
       for (int j = 0; j < 2; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
                                                           acc[i][j], 0, 0, 0);
+  }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
