@@ -160,6 +160,7 @@ struct hqrrp_handle_s {
   hipStream_t s_chain = nullptr, s_upd = nullptr;
   int chain_cus = 0;
   hipEvent_t ev_fork = nullptr, ev_upd = nullptr, ev_chain = nullptr;
+  hipEvent_t ev_sketch = nullptr, ev_t = nullptr;  // look-ahead: moves, T
   // Joins the caller's stream to the handle's own stream when the caller's
   // is the null stream (whose implicit synchronisation with every blocking
   // stream would slow the loop's many small launches).
@@ -423,6 +424,7 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
   return HQRRP_OK;
 }
 
+// dT == nullptr: T is left to the caller (larft from h->gram).
 int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
               int pivoting, double* dtau, int* dperm, double* dT, long ldt) {
   if (b < 1 || b > 64 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
@@ -447,6 +449,7 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
     HQ_TRY(launch_panel_qrcp(a, h->cur));
   }
   xchg_end(h->xp, need);
+  if (!dT) return HQRRP_OK;
   PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
   HQ_TRY(launch_larft(b, a.gram, dtau, dT, ldt, h->cur));
   return HQRRP_OK;
@@ -796,6 +799,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   // FP64 or, for the mixed sketch, FP32.
   double* Gt = static_cast<double*>(G);
   bool have_panel = false;  // block j's panel already factored by look-ahead
+  bool t_pending = false;   // its T still being formed on s_chain (ev_t)
   for (int j = 0; j < kmax; j += nb) {
     const int b = std::min(nb, std::min(n - j, m - j));
     const bool last = (j + nb >= m) || (j + nb >= n);
@@ -835,6 +839,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta,
                          gcols ? Gt + j : nullptr, m, gcols ? r : 0));
     HQ_RET(trail_vtc_only(h, ta));
+    if (t_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_t, 0));
+    t_pending = false;
     HQ_RET(block_tail(h, ta, j, jn, dA, lda, Y, r));
     if (!la) {
       HQ_RET(trail_apply(h, below_r12(ta)));
@@ -851,40 +857,25 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     ta.ldco = ldx[xn];
     // R12 is already in A: the update covers the rows below it.
     const TrailArgs tu = below_r12(ta);
-    // Fork: block j's update || block jn's pivot chain.
+    // Fork: block j's update || block jn's pivot chain.  s_upd: the update,
+    // then (once the chain's sketch has named them) block jn's column
+    // gathers, which touch the updated trailing matrix, Y and jpvt.
+    // s_chain: sketch, column selection and update, panel; its T follows
+    // on s_chain after the join point, needed only by block jn's tail.
     HQ_TRY(hipEventRecord(h->ev_fork, main_s));
     HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_fork, 0));
     HQ_TRY(hipStreamWaitEvent(h->s_chain, h->ev_fork, 0));
     h->cur = h->s_upd;
     int rc = trail_apply(h, tu);
-    if (rc == HQRRP_OK && hipEventRecord(h->ev_upd, h->s_upd) != hipSuccess)
-      rc = HQRRP_ERR_HIP;
     h->cur = h->s_chain;
-    auto chain = [&]() -> int {
-      if (!last_next) HQ_RET(block_sketch(h, r, jn, n, b2, Y));
-      {
-        // The next panel's columns, selected and updated in one pass (the
-        // same products, in the same order, as the full update's).
-        PhaseScope ps(h, PH_SELECT, 2.0 * b * (double)mnext * b2,
-                      16.0 * (double)mnext * b2, 1);
-        HQ_TRY(launch_select_update(last_next ? nullptr : h->moves.as<int>(),
-                                    b2, C, ldx[xc], b, mnext, Ajj + b, lda,
-                                    ta.W, ta.ldw, b, Pn, mnext, h->cur));
-      }
-      HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
-                       h->perm.as<int>(), T, nb));
-      HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
-      return HQRRP_OK;
-    };
-    if (rc == HQRRP_OK) rc = chain();
-    h->cur = main_s;
-    HQ_RET(rc);
-    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
-    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
-    // Join: the R12 rows into A (when the update wrote them elsewhere), block
-    // jn's columns into place (rows above jn in A, the rest in the trailing
-    // buffer), its factored panel over them, and the row permutation.
-    if (!last_next) {
+    if (rc == HQRRP_OK && !last_next) {
+      rc = block_sketch(h, r, jn, n, b2, Y);
+      if (rc == HQRRP_OK && hipEventRecord(h->ev_sketch, h->s_chain) != hipSuccess)
+        rc = HQRRP_ERR_HIP;
+    }
+    auto gathers = [&]() -> int {
+      if (last_next) return HQRRP_OK;
+      HQ_TRY(hipStreamWaitEvent(h->s_upd, h->ev_sketch, 0));
       if (xn == 0) {
         HQ_RET(block_gather(h, r, jn, m, b2, dA, lda, Y, djpvt));
       } else {
@@ -896,7 +887,44 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                                   h->tmpY.as<double>(), h->tmpJ.as<int>(),
                                   h->cur));
       }
+      return HQRRP_OK;
+    };
+    if (rc == HQRRP_OK) {
+      h->cur = h->s_upd;
+      rc = gathers();
+      if (rc == HQRRP_OK && hipEventRecord(h->ev_upd, h->s_upd) != hipSuccess)
+        rc = HQRRP_ERR_HIP;
+      h->cur = h->s_chain;
     }
+    auto chain = [&]() -> int {
+      {
+        // The next panel's columns, selected and updated in one pass (the
+        // same products, in the same order, as the full update's).
+        PhaseScope ps(h, PH_SELECT, 2.0 * b * (double)mnext * b2,
+                      16.0 * (double)mnext * b2, 1);
+        HQ_TRY(launch_select_update(last_next ? nullptr : h->moves.as<int>(),
+                                    b2, C, ldx[xc], b, mnext, Ajj + b, lda,
+                                    ta.W, ta.ldw, b, Pn, mnext, h->cur));
+      }
+      HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
+                       h->perm.as<int>(), nullptr, nb));
+      HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
+      {
+        PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
+        HQ_TRY(launch_larft(b2, h->gram.as<double>(), dtau + jn, T, nb,
+                            h->cur));
+      }
+      HQ_TRY(hipEventRecord(h->ev_t, h->s_chain));
+      return HQRRP_OK;
+    };
+    if (rc == HQRRP_OK) rc = chain();
+    h->cur = main_s;
+    HQ_RET(rc);
+    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
+    HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
+    t_pending = true;
+    // Join: block jn's factored panel over its gathered columns, and the
+    // row permutation of the rows above it.
     {
       PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mnext * b2, 1);
       HQ_TRY(launch_copy_block(dA + jn + (long)jn * lda, lda, Pn, mnext, mnext,
@@ -906,6 +934,9 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     xc = xn;
     have_panel = true;
   }
+  // A last T formed on s_chain but never consumed (no trailing columns)
+  // still reads h->gram: order the caller's next work after it.
+  if (t_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_t, 0));
   HQ_TRY(hipGetLastError());
   return HQRRP_OK;
 }
@@ -1118,6 +1149,8 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
       HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, hipEventDisableTiming));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_sketch, hipEventDisableTiming));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_t, hipEventDisableTiming));
     }
   }
   int rc = ensure(h->words, 64);
@@ -1142,7 +1175,8 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain, h->ev_in, h->ev_out})
+  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain, h->ev_sketch,
+                       h->ev_t, h->ev_in, h->ev_out})
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t st : {h->s_chain, h->s_upd})
     if (st) (void)hipStreamDestroy(st);
