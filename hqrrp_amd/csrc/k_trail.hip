@@ -505,46 +505,94 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   const long gc = c0 + cw;
   // Every load up front (T, V1, the top rows for R12, then the partial
   // slices summed in slice order, four slices' loads in flight at a time),
-  // then the LDS stores.
+  // then the LDS stores.  T and V1 go into LDS unmasked (T's upper triangle
+  // and V1's unit lower triangle are applied where they are read), so an
+  // interior tile with all 64 reflectors (fast) issues every load without a
+  // predicate.
+  const bool fast = b == 64 && c0 + 64 <= nc;
   constexpr int kPer = 64 * 64 / HQ_NT;
   hq_d4 top[4];
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
-    Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
-    V1s[p][c] = (p < b && c < b) ? vmask(a.V[p + (long)c * a.ldv], p, c) : 0.0;
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int i = q * 16 + l4 + 4 * e;
-      top[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
-    }
-  // The slices four at a time: 4 x kPer independent loads in flight, then
-  // added in slice order.
   double msum[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) msum[u] = 0.0;
   const long off0 = (long)(t & 63) + (v0 + (t >> 6)) * b;  // u = 0
-  const bool pok = (t & 63) < b;
-  for (int q0 = 0; q0 < a.S; q0 += 4) {
-    double x[4][kPer];
+  if (fast) {
+    double tv[kPer], vv[kPer];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const bool sok = q0 + k < a.S;
-      const double* src = a.part + (q0 + k) * slice + off0;
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int c = (t >> 6) + u * (HQ_NT / 64);
-        x[k][u] = (sok && pok && c0 + c < nc) ? src[(long)u * (HQ_NT / 64) * b]
-                                              : 0.0;
-      }
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      tv[u] = a.T[p + (long)c * a.ldt];
+      vv[u] = a.V[p + (long)c * a.ldv];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) msum[u] += x[k][u];
+      for (int e = 0; e < 4; ++e)
+        top[q][e] = Cb[q * 16 + l4 + 4 * e + gc * ldc];
+    const int S4 = a.S & ~3;
+    for (int q0 = 0; q0 < S4; q0 += 4) {
+      double x[4][kPer];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double* src = a.part + (q0 + k) * slice + off0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) x[k][u] = src[(long)u * (HQ_NT / 64) * b];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) msum[u] += x[k][u];
+    }
+    for (int q = S4; q < a.S; ++q) {
+      const double* src = a.part + q * slice + off0;
+      double x[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) x[u] = src[(long)u * (HQ_NT / 64) * b];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) msum[u] += x[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      Ts[p][c] = tv[u];
+      V1s[p][c] = vv[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
+      V1s[p][c] = (p < b && c < b) ? a.V[p + (long)c * a.ldv] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        top[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
+      }
+    // The slices four at a time: 4 x kPer independent loads in flight, then
+    // added in slice order.
+    const bool pok = (t & 63) < b;
+    for (int q0 = 0; q0 < a.S; q0 += 4) {
+      double x[4][kPer];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const bool sok = q0 + k < a.S;
+        const double* src = a.part + (q0 + k) * slice + off0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+          const int c = (t >> 6) + u * (HQ_NT / 64);
+          x[k][u] = (sok && pok && c0 + c < nc)
+                        ? src[(long)u * (HQ_NT / 64) * b]
+                        : 0.0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) msum[u] += x[k][u];
+    }
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
@@ -562,8 +610,9 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
       const double bf = Ms[kk + l4][cw];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[kk + l4][q * 16 + l15],
-                                                      bf, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            (kk + l4 <= q * 16 + l15) ? Ts[kk + l4][q * 16 + l15] : 0.0, bf,
+            acc[q], 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -586,8 +635,9 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
       const double bf = Ws[kk + l4][cw];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-V1s[q * 16 + l15][kk + l4],
-                                                      bf, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            -vmask(V1s[q * 16 + l15][kk + l4], q * 16 + l15, kk + l4), bf,
+            acc[q], 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
