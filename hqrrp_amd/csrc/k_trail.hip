@@ -662,31 +662,59 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
   const int b = a.b, r = a.nc2;
   const long c0 = (long)blockIdx.x * 64;
   // Every load in flight together (B', R12, the Y2 tile), then the LDS
-  // stores.
+  // stores.  An interior tile with all 64 reflectors (fast) loads without
+  // predicates: rows i >= r of B' and Y2 only feed accumulator rows that are
+  // never stored, so they read row r - 1 instead of being zeroed.
   constexpr int kPa = 64 * 80 / HQ_NT, kPb = 64 * 64 / HQ_NT;
   double va[kPa], vb[kPb];
-#pragma unroll
-  for (int u = 0; u < kPa; ++u) {
-    const int idx = t + u * HQ_NT, p = idx & 63, i = idx >> 6;
-    va[u] = (p < b && i < r) ? -a.C2[p + (long)i * a.ldc2] : 0.0;
-  }
-#pragma unroll
-  for (int u = 0; u < kPb; ++u) {
-    const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
-    vb[u] = (p < b && c0 + c < a.nc) ? a.R[p + (c0 + c) * a.ldr] : 0.0;
-  }
   const int cw = wave * 16 + l15;
   const long gc = c0 + cw;
   const bool cok = gc < a.nc;
   hq_d4 acc[5];
+  if (b == 64 && c0 + 64 <= a.nc) {
 #pragma unroll
-  for (int q = 0; q < 5; ++q)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int i = q * 16 + l4 + 4 * e;
-      const long at = i + gc * a.ldy;
-      acc[q][e] = (cok && i < r) ? (a.Y2f ? (double)a.Y2f[at] : a.Y2[at]) : 0.0;
+    for (int u = 0; u < kPa; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, i = idx >> 6;
+      va[u] = -a.C2[p + (long)min(i, r - 1) * a.ldc2];
     }
+#pragma unroll
+    for (int u = 0; u < kPb; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      vb[u] = a.R[p + (c0 + c) * a.ldr];
+    }
+    if (a.Y2f) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[q][e] = (double)a.Y2f[min(q * 16 + l4 + 4 * e, r - 1) + gc * a.ldy];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[q][e] = a.Y2[min(q * 16 + l4 + 4 * e, r - 1) + gc * a.ldy];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPa; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, i = idx >> 6;
+      va[u] = (p < b && i < r) ? -a.C2[p + (long)i * a.ldc2] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPb; ++u) {
+      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      vb[u] = (p < b && c0 + c < a.nc) ? a.R[p + (c0 + c) * a.ldr] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = q * 16 + l4 + 4 * e;
+        const long at = i + gc * a.ldy;
+        acc[q][e] = (cok && i < r) ? (a.Y2f ? (double)a.Y2f[at] : a.Y2[at]) : 0.0;
+      }
+  }
 #pragma unroll
   for (int u = 0; u < kPa; ++u) {
     const int idx = t + u * HQ_NT;
