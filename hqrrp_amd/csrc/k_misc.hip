@@ -98,28 +98,22 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
   const long mw = (long)blockIdx.x * 128 + wave * 32;  // this wave's rows
   // acc[i][j] element e: P'(n, m), n = 16i + l/16 + 4e, m = mw + 16j + l%16.
   hq_d4 acc[4][2];
-  const bool full = b2 == 64 && mw + 32 <= rows;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int n = 16 * i + l4 + 4 * e;
-      const double* cc = C + r0 + (long)(n < b2 ? src[n] : 0) * ldc;
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const long m = mw + 16 * j + l15;
-        if (full)
-          acc[i][j][e] = cc[m];
-        else
-          acc[i][j][e] = (n < b2 && m < rows) ? cc[m] : 0.0;
-      }
-    }
   int wsrc[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) wsrc[i] = (16 * i + l15 < b2) ? src[16 * i + l15] : -1;
   if (b == 64 && b2 == 64 && mw + 32 <= rows) {
-    // Every operand exists: predicate-free loads through a four-step
-    // register ring (the same products in the same order as below).
+    // Every operand exists: C and the first operand steps are loaded
+    // without predicates, in flight together, and a four-step register ring
+    // keeps later steps' loads in flight (the same products in the same
+    // order as below).
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double* cc = C + r0 + (long)src[16 * i + l4 + 4 * e] * ldc;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j][e] = cc[mw + 16 * j + l15];
+      }
     auto load = [&](double (&af)[4], double (&bf)[2], int st) {
       const int k = 4 * st + l4;
 #pragma unroll
@@ -150,7 +144,28 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
       if (st + 6 < 16) load(ra[2], rb[2], st + 6);
       mma(ra[3], rb[3]);
     }
-  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int n = 16 * i + l4 + 4 * e;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) P[mw + 16 * j + l15 + (long)n * ldp] = acc[i][j][e];
+      }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int n = 16 * i + l4 + 4 * e;
+      const double* cc = C + r0 + (long)(n < b2 ? src[n] : 0) * ldc;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const long m = mw + 16 * j + l15;
+        acc[i][j][e] = (n < b2 && m < rows) ? cc[m] : 0.0;
+      }
+    }
 #pragma unroll
   for (int kk = 0; kk < 64; kk += 4) {
     const int k = kk + l4;
@@ -169,7 +184,6 @@ __global__ __launch_bounds__(HQ_NT) void select_update_kernel(
       for (int j = 0; j < 2; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
                                                           acc[i][j], 0, 0, 0);
-  }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
