@@ -30,6 +30,9 @@
 namespace {
 
 constexpr int kSlotHead = 8;  // slot: [key, pos, alpha, ||x||^2, pad, rows]
+// Slots start 1 KB apart, so no cache line holds parts of two workgroups'
+// slots (lines written by several workgroups slow the polling).
+constexpr long slot_stride(long used) { return (used + 127) / 128 * 128; }
 constexpr int kThreads = 512;  // 8 wavefronts, 32 columns each
 constexpr int kCols = 256;     // sketch columns per workgroup
 constexpr int kMaxWg = 256;    // workgroups of one launch at most
@@ -64,6 +67,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   const bool own = c < a.ncols;
   const int ibase = half * H;  // first active row (relative) of this lane
   const long slot = kSlotHead + RMAX;
+  const long stride = slot_stride(slot);
 
   __shared__ T s_v[RMAX];  // reflector on the active rows (relative)
   __shared__ double s_slot[kSlotHead + RMAX];  // this workgroup's slot
@@ -101,8 +105,8 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   __syncthreads();
 
   for (int k = 0; k < a.stages; ++k) {
-    const double* cbuf = a.xcur + (long)k * a.nw * slot;
-    double* mine = a.xcur + ((long)k * a.nw + w) * slot;
+    const double* cbuf = a.xcur + (long)k * a.nw * stride;
+    double* mine = a.xcur + ((long)k * a.nw + w) * stride;
     const int span = r - k;  // active rows
     // 1. local argmax: largest norm, smallest position among equals; every
     //    thread combines the 8 wavefront winners identically and the
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       for (int q = lane; ok && q - lane < a.nw; q += 64) {
         double hv[4];
         ok = hq_await(hv, [&](int i) -> const double* {
-          return q < a.nw ? cbuf + (long)q * slot + i : nullptr;
+          return q < a.nw ? cbuf + (long)q * stride + i : nullptr;
         }, a.err);
         const int pq = (int)hv[1];
         if (q < a.nw && (hv[0] > bk || (hv[0] == bk && pq < bp))) {
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       ba = hq_readlane(ba, bl);
       bs = hq_readlane(bs, bl);
       // Lane l: relative rows l and l + 64 of the winner.
-      const double* src = cbuf + (long)bw * slot + kSlotHead;
+      const double* src = cbuf + (long)bw * stride + kSlotHead;
       const bool f0 = lane > 0 && lane < span;
       const bool f1 = lane + 64 < span;
       double cv[2];
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
 }  // namespace
 
 size_t sketch_xchg_doubles(int nw, int rmax, int stages) {
-  return (size_t)stages * nw * (kSlotHead + rmax);
+  return (size_t)stages * nw * slot_stride(kSlotHead + rmax);
 }
 
 int sketch_cols_per_wg() { return kCols; }
