@@ -183,8 +183,8 @@ __device__ __forceinline__ void hq_argmax_dpp(double& key, int& pos) {
 // the result.  Within each row of 16 lanes: xor 1, xor 2 (quad
 // permutations), then the half-row and row mirrors, each pairing every lane
 // with one holding the other half of its group; the four row results are
-// then combined from scalar reads of lanes 0, 16, 32, 48.  Positions must be
-// unique among lanes with equal keys unless those lanes hold the same pair.
+// then combined from scalar reads of lanes 0, 16, 32, 48.  The rule is a
+// lexicographic maximum, so the pairing order cannot change the result.
 // All 64 lanes must be active.
 __device__ __forceinline__ void hq_wave_argmax64(double& key, int& pos) {
   hq_argmax_dpp<0xB1>(key, pos);   // quad_perm [1,0,3,2]
