@@ -786,12 +786,25 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                   8.0 * ((double)m * n + (double)r * (m + n)), 1);
     // G holds Omega' (m x r, FP64); the mixed sketch forms Y in FP32
     // (optionally bf16-rounded operands) with FP32 accumulation.
-    if (h->sk32)
+    if (h->sk32) {
       HQ_RET(gemm32(h, true, false, r, n, m, 1.0f, G, m, 1, 0, dA, lda, 1, 0,
                     0.0f, static_cast<float*>(Y), r, h->skbf));
-    else
-      HQ_RET(gemm(h, true, false, r, n, m, 1.0, static_cast<double*>(G), m, 0,
-                  dA, lda, 0, 0.0, static_cast<double*>(Y), r));
+    } else {
+      // Y = G'' A is a V'C product with G' (unmasked) in the role of V: up
+      // to 64 sketch rows per pass, each pass one streaming read of A with
+      // the operands going straight into the matrix cores (split-K
+      // partials, then a fixed-order sum into Y's rows).
+      for (int p0 = 0; p0 < r; p0 += 64) {
+        const int pb = std::min(64, r - p0);
+        TrailArgs ta;
+        HQ_RET(trail_prepare(h, m, n, pb, static_cast<double*>(G) + (long)p0 * m,
+                             m, nullptr, 0, dA, lda, &ta));
+        ta.vmask = 0;
+        HQ_TRY(launch_trail_vtc(ta, h->cur));
+        HQ_TRY(launch_reduce_splitk(pb, n, ta.splits, ta.part, 0.0,
+                                    static_cast<double*>(Y) + p0, r, h->cur));
+      }
+    }
   }
 
   // G' (m x r, ld m, FP64) is updated as extra columns of the trailing
