@@ -143,7 +143,7 @@ enum {
   PH_TRAIL_TW,         // W = T'M; in the driver also R12, B' and Y2
   PH_TRAIL_UPD,        // trailing C -= V W
   PH_DOWNDATE,         // sketch downdate (component entry only)
-  PH_SELECT,           // look-ahead: next panel's columns selected + updated
+  PH_SELECT,           // (unused: the panel kernel selects and updates)
   HQ_PHASES
 };
 static const char* const kPhaseNames[HQ_PHASES] = {
@@ -424,11 +424,27 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
   return HQRRP_OK;
 }
 
-// dT == nullptr: T is left to the caller (larft from h->gram).
+// dT == nullptr: T is left to the caller (larft from h->gram).  sel: the
+// look-ahead selection and update of the panel's columns, done by the panel
+// kernel as it loads them (its sel_* fields; null: the panel is read from
+// dP).
 int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
-              int pivoting, double* dtau, int* dperm, double* dT, long ldt) {
+              int pivoting, double* dtau, int* dperm, double* dT, long ldt,
+              const PanelArgs* sel = nullptr) {
   if (b < 1 || b > 64 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
   PanelArgs a{};
+  if (sel) {
+    a.sel_moves = sel->sel_moves;
+    a.sel_b2 = sel->sel_b2;
+    a.sel_r0 = sel->sel_r0;
+    a.sel_b = sel->sel_b;
+    a.sel_C = sel->sel_C;
+    a.sel_ldc = sel->sel_ldc;
+    a.sel_V = sel->sel_V;
+    a.sel_ldv = sel->sel_ldv;
+    a.sel_W = sel->sel_W;
+    a.sel_ldw = sel->sel_ldw;
+  }
   HQ_RET(panel_geometry(h, mrows, &a.nw));
   a.P = dP;
   a.ldp = ldp;
@@ -910,17 +926,22 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       h->cur = h->s_chain;
     }
     auto chain = [&]() -> int {
-      {
-        // The next panel's columns, selected and updated in one pass (the
-        // same products, in the same order, as the full update's).
-        PhaseScope ps(h, PH_SELECT, 2.0 * b * (double)mnext * b2,
-                      16.0 * (double)mnext * b2, 1);
-        HQ_TRY(launch_select_update(last_next ? nullptr : h->moves.as<int>(),
-                                    b2, C, ldx[xc], b, mnext, Ajj + b, lda,
-                                    ta.W, ta.ldw, b, Pn, mnext, h->cur));
-      }
+      // The next panel's columns are selected and updated (the same
+      // products, in the same order, as the full update's) by the panel
+      // kernel as it loads them.
+      PanelArgs sel{};
+      sel.sel_moves = last_next ? nullptr : h->moves.as<int>();
+      sel.sel_b2 = b2;
+      sel.sel_C = C;
+      sel.sel_ldc = ldx[xc];
+      sel.sel_r0 = b;
+      sel.sel_V = Ajj + b;
+      sel.sel_ldv = lda;
+      sel.sel_W = ta.W;
+      sel.sel_ldw = ta.ldw;
+      sel.sel_b = b;
       HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
-                       h->perm.as<int>(), nullptr, nb));
+                       h->perm.as<int>(), nullptr, nb, &sel));
       HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
       {
         PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);

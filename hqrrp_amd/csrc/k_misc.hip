@@ -72,132 +72,6 @@ __global__ __launch_bounds__(HQ_NT) void copy_block_kernel(double* dst, long ldd
   }
 }
 
-// Look-ahead selection and update: the b2 <= 64 columns that the next
-// block's panel will hold (position q <- source position src(q), from the
-// composed pivot moves, or the identity when moves is null) are read from
-// rows [r0, r0+rows) of the not yet updated trailing matrix C, updated with
-// the block's reflectors -- P(:, q) = C(:, src(q)) - V W'(src(q), :)' --
-// and written to P.  One 128-row tile per workgroup; wavefront w owns rows
-// 32w..32w+31 and every column, computed transposed (P' = C' + W'(-V')) on
-// the matrix cores with the operands loaded straight into MFMA registers.
-__global__ __launch_bounds__(HQ_NT) void select_update_kernel(
-    const int* moves, int b2, const double* C, long ldc, int r0, int rows,
-    const double* V, long ldv, const double* Wt, long ldw, int b, double* P,
-    long ldp) {
-  __shared__ int src[64];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int l15 = lane & 15, l4 = lane >> 4;
-  if (t < 64) src[t] = t;
-  __syncthreads();
-  // The composed moves have distinct destinations: one thread per move.
-  if (moves && t < moves[0]) {  // at most 2 * 80 moves < HQ_NT
-    const int dst = moves[1 + 2 * t];
-    if (dst < b2) src[dst] = moves[2 + 2 * t];
-  }
-  __syncthreads();
-  const long mw = (long)blockIdx.x * 128 + wave * 32;  // this wave's rows
-  // acc[i][j] element e: P'(n, m), n = 16i + l/16 + 4e, m = mw + 16j + l%16.
-  hq_d4 acc[4][2];
-  int wsrc[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) wsrc[i] = (16 * i + l15 < b2) ? src[16 * i + l15] : -1;
-  if (b == 64 && b2 == 64 && mw + 32 <= rows) {
-    // Every operand exists: C and the first operand steps are loaded
-    // without predicates, in flight together, and a four-step register ring
-    // keeps later steps' loads in flight (the same products in the same
-    // order as below).
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const double* cc = C + r0 + (long)src[16 * i + l4 + 4 * e] * ldc;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j][e] = cc[mw + 16 * j + l15];
-      }
-    auto load = [&](double (&af)[4], double (&bf)[2], int st) {
-      const int k = 4 * st + l4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = Wt[wsrc[i] + (long)k * ldw];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = -V[mw + 16 * j + l15 + (long)k * ldv];
-    };
-    auto mma = [&](const double (&af)[4], const double (&bf)[2]) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
-                                                            acc[i][j], 0, 0, 0);
-    };
-    double ra[4][4], rb[4][2];
-    load(ra[0], rb[0], 0);
-    load(ra[1], rb[1], 1);
-    load(ra[2], rb[2], 2);
-#pragma unroll
-    for (int st = 0; st < 16; st += 4) {
-      load(ra[3], rb[3], st + 3);
-      mma(ra[0], rb[0]);
-      if (st + 4 < 16) load(ra[0], rb[0], st + 4);
-      mma(ra[1], rb[1]);
-      if (st + 5 < 16) load(ra[1], rb[1], st + 5);
-      mma(ra[2], rb[2]);
-      if (st + 6 < 16) load(ra[2], rb[2], st + 6);
-      mma(ra[3], rb[3]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int n = 16 * i + l4 + 4 * e;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) P[mw + 16 * j + l15 + (long)n * ldp] = acc[i][j][e];
-      }
-    return;
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int n = 16 * i + l4 + 4 * e;
-      const double* cc = C + r0 + (long)(n < b2 ? src[n] : 0) * ldc;
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const long m = mw + 16 * j + l15;
-        acc[i][j][e] = (n < b2 && m < rows) ? cc[m] : 0.0;
-      }
-    }
-#pragma unroll
-  for (int kk = 0; kk < 64; kk += 4) {
-    const int k = kk + l4;
-    double af[4], bf[2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      af[i] = (k < b && wsrc[i] >= 0) ? Wt[wsrc[i] + (long)k * ldw] : 0.0;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const long m = mw + 16 * j + l15;
-      bf[j] = (k < b && m < rows) ? -V[m + (long)k * ldv] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
-                                                          acc[i][j], 0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int n = 16 * i + l4 + 4 * e;
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const long m = mw + 16 * j + l15;
-        if (n < b2 && m < rows) P[m + (long)n * ldp] = acc[i][j][e];
-      }
-    }
-}
-
 __global__ __launch_bounds__(HQ_NT) void fill_sentinel_kernel(double* p,
                                                               size_t n) {
   const double v = hq_sentinel();
@@ -243,18 +117,6 @@ hipError_t launch_copy_block(double* dst, long ldd, const double* src,
   dim3 grid(hq_cdiv(rows, kRowsPerBlock), cols < 32768 ? cols : 32768);
   hipLaunchKernelGGL(copy_block_kernel, grid, dim3(HQ_NT), 0, s, dst, ldd, src,
                      lds, rows, cols);
-  return hipGetLastError();
-}
-
-hipError_t launch_select_update(const int* moves, int b2, const double* C,
-                                long ldc, int r0, int rows, const double* V,
-                                long ldv, const double* Wt, long ldw, int b,
-                                double* P, long ldp, hipStream_t s) {
-  if (b2 <= 0 || b2 > 64 || b > 64) return hipErrorInvalidValue;
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(select_update_kernel, dim3(hq_cdiv(rows, 128)),
-                     dim3(HQ_NT), 0, s, moves, b2, C, ldc, r0, rows, V, ldv,
-                     Wt, ldw, b, P, ldp);
   return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@
 // written back in pivoted order.
 #include "common.h"
 #include "kernels.h"
+#include "select_tile.h"
 
 namespace {
 
@@ -56,6 +57,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   __shared__ double s_rowin[kMaxB];     // row k, as received
   __shared__ double s_pv[kPG][kRT];     // each group's rows of the pivot column
   __shared__ int s_abort;
+  __shared__ int s_src[kMaxB];          // look-ahead: source column per position
 
   const int t = threadIdx.x, w = blockIdx.x;
   const int b = a.b, nw = a.nw;
@@ -77,8 +79,45 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   };
   if (t == 0) s_abort = 0;
 
-  // ---- load, 128 rows per round, coalesced along the rows of a column.
   double x[kRT];
+  if (a.sel_C) {
+    // ---- look-ahead load: every wavefront forms its 32 rows of the
+    // selected, updated columns on the matrix cores (select_tile.h), then
+    // the tiles reach the column-per-thread layout through `stage`, 128 rows
+    // per round.
+    if (t < kMaxB) s_src[t] = t;
+    __syncthreads();
+    // The composed moves have distinct destinations: one thread per move.
+    if (a.sel_moves && t < a.sel_moves[0]) {  // at most 2 * 80 moves < kPT
+      const int dst = a.sel_moves[1 + 2 * t];
+      if (dst < a.sel_b2) s_src[dst] = a.sel_moves[2 + 2 * t];
+    }
+    __syncthreads();
+    hq_d4 acc[4][2];
+    select_tile(acc, s_src, a.sel_b2, a.sel_C, a.sel_ldc, a.sel_r0, a.mrows,
+                a.sel_V, a.sel_ldv, a.sel_W, a.sel_ldw, a.sel_b, glo);
+    const int l15 = cc & 15, l4 = cc >> 4;
+#pragma unroll
+    for (int rd = 0; rd < kPR / 128; ++rd) {
+      if (g / (128 / kRT) == rd) {
+        const int rb = (g % (128 / kRT)) * kRT;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              stage[16 * i + l4 + 4 * e][rb + 16 * j + l15] = acc[i][j][e];
+      }
+      __syncthreads();
+      if (g / (128 / kRT) == rd) {
+#pragma unroll
+        for (int i = 0; i < kRT; ++i) x[i] = stage[cc][(g % (128 / kRT)) * kRT + i];
+      }
+      __syncthreads();
+    }
+  } else {
+  // ---- load, 128 rows per round, coalesced along the rows of a column.
 #pragma unroll
   for (int rd = 0; rd < kPR / 128; ++rd) {
 #pragma unroll
@@ -93,6 +132,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
       for (int i = 0; i < kRT; ++i) x[i] = stage[cc][(g % (128 / kRT)) * kRT + i];
     }
     __syncthreads();
+  }
   }
 
   // Exchange e's slot of workgroup q.  e = 0: initial norms; step k: 1 + 2k

@@ -33,6 +33,18 @@ struct PanelArgs {
   double* xprev;
   long xprev_len;
   int* err;
+  // Look-ahead (sel_C set): the panel's columns are formed on load as
+  // P(m, q) = C(sel_r0 + m, src(q)) - V(m, :) W'(src(q), :)' (select_tile.h;
+  // src from the composed moves, the identity when sel_moves is null)
+  // instead of read from P; P then only receives the factored panel.
+  const int* sel_moves;
+  int sel_b2, sel_r0, sel_b;
+  const double* sel_C;
+  long sel_ldc;
+  const double* sel_V;
+  long sel_ldv;
+  const double* sel_W;
+  long sel_ldw;
 };
 
 struct GemmArgs {
@@ -205,13 +217,6 @@ hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
                             double* dst, long ldd, hipStream_t s);
 hipError_t launch_copy_block(double* dst, long ldd, const double* src,
                              long lds, int rows, int cols, hipStream_t s);
-// Look-ahead: P(:, q) = C(r0.., src(q)) - V W'(src(q), :)' for the b2
-// columns the next panel will hold (src from the composed moves; null:
-// identity).
-hipError_t launch_select_update(const int* moves, int b2, const double* C,
-                                long ldc, int r0, int rows, const double* V,
-                                long ldv, const double* Wt, long ldw, int b,
-                                double* P, long ldp, hipStream_t s);
 // Moves m rows of A's columns (A may point at any row), r rows of Y's
 // columns (r = 0: none; Y's elements are ysz = 4 or 8 bytes, ldy in
 // elements) and jpvt (null: none).
