@@ -957,14 +957,15 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
     t_pending = true;
-    // Join: block jn's factored panel over its gathered columns, and the
-    // row permutation of the rows above it.
+    // Join: block jn's factored panel over its gathered columns and the
+    // rows above it permuted, in one launch.
     {
-      PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mnext * b2, 1);
-      HQ_TRY(launch_copy_block(dA + jn + (long)jn * lda, lda, Pn, mnext, mnext,
-                               b2, h->cur));
+      PhaseScope ps(h, PH_PANEL_POST, 0.0,
+                    16.0 * (double)mnext * b2 + 16.0 * jn * b2, 1);
+      HQ_TRY(launch_panel_permute(dA + (long)jn * lda, lda, jn, b2,
+                                  h->perm.as<int>(), djpvt + jn, h->cur, Pn,
+                                  mnext, mnext));
     }
-    HQ_RET(block_permute(h, jn, b2, dA, lda, djpvt));
     xc = xn;
     have_panel = true;
   }

@@ -451,11 +451,31 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
 
 // Applies the panel's column permutation to rows [0, nrows) of the b panel
 // columns (the rows above the panel, A01 in the reference) and to jpvt.
+// With P: blocks past the permutation's also copy the factored panel
+// (prows x b, ld ldp, already in pivoted order) into A's rows nrows.. of the
+// same columns -- the look-ahead's write-back, in the same launch.
 __global__ __launch_bounds__(HQ_NT) void panel_permute_kernel(
-    double* A, long lda, int nrows, int b, const int* perm, int* jpvt) {
+    double* A, long lda, int nrows, int b, const int* perm, int* jpvt,
+    const double* P, long ldp, int prows, int nperm) {
   __shared__ double buf[kMaxB][65];
   __shared__ int sj[kMaxB], sp[kMaxB];
   const int t = threadIdx.x;
+  if ((int)blockIdx.x >= nperm) {
+    const long i0 = (long)(blockIdx.x - nperm) * 64;
+    constexpr int kPer = 64 * kMaxB / HQ_NT;
+    double v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, i = idx & 63, c = idx >> 6;
+      v[u] = (c < b && i0 + i < prows) ? P[(i0 + i) + (long)c * ldp] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int idx = t + u * HQ_NT, i = idx & 63, c = idx >> 6;
+      if (c < b && i0 + i < prows) A[(nrows + i0 + i) + (long)c * lda] = v[u];
+    }
+    return;
+  }
   const long r0 = (long)blockIdx.x * 64;
   const bool jp = blockIdx.x == 0 && jpvt;
   if (t < b) {
@@ -508,9 +528,12 @@ hipError_t launch_larft(int b, const double* gram, const double* tau,
 
 hipError_t launch_panel_permute(double* A, long lda, int nrows, int b,
                                 const int* perm, int* jpvt,
-                                hipStream_t stream) {
-  int blocks = nrows > 0 ? hq_cdiv(nrows, 64) : 1;
-  hipLaunchKernelGGL(panel_permute_kernel, dim3(blocks), dim3(HQ_NT), 0,
-                     stream, A, lda, nrows, b, perm, jpvt);
+                                hipStream_t stream, const double* P, long ldp,
+                                int prows) {
+  const int nperm = nrows > 0 ? hq_cdiv(nrows, 64) : 1;
+  const int ncopy = (P && prows > 0) ? hq_cdiv(prows, 64) : 0;
+  hipLaunchKernelGGL(panel_permute_kernel, dim3(nperm + ncopy), dim3(HQ_NT), 0,
+                     stream, A, lda, nrows, b, perm, jpvt, P, ldp, prows,
+                     nperm);
   return hipGetLastError();
 }

@@ -200,8 +200,13 @@ int panel_rows_per_wg();
 size_t panel_xchg_doubles(int nw, int b);
 hipError_t launch_larft(int b, const double* gram, const double* tau,
                         double* T, long ldt, hipStream_t s);
+// Rows [0, nrows) of the b panel columns follow the panel's pivots (and
+// jpvt); with P, the same launch also copies the factored panel (prows x b,
+// ld ldp) into rows nrows.. of those columns.
 hipError_t launch_panel_permute(double* A, long lda, int nrows, int b,
-                                const int* perm, int* jpvt, hipStream_t s);
+                                const int* perm, int* jpvt, hipStream_t s,
+                                const double* P = nullptr, long ldp = 0,
+                                int prows = 0);
 
 // k_gemm.hip
 hipError_t launch_gemm(const GemmArgs& g, bool ta, bool tb, int splits,
