@@ -306,6 +306,89 @@ def bench_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_classic(args):
+    """dgeqp4's small-size path: the classic greedy QRCP (dgeqp3 semantics)
+    of a 999 x 999 Gaussian, the largest size the reference hands to LAPACK
+    dgeqp3; GFLOP/s of 2mn^2 - 2/3 n^3 from HIP events around the device
+    entry (the input copy stays outside the events)."""
+    import numpy as np
+    import torch
+
+    import hqrrp_amd
+    from hqrrp_amd import matgen
+
+    m = n = 999
+    A_host = matgen.gaussian(m, n, 0)
+    A0 = hqrrp_amd.to_colmajor(A_host)
+    W = torch.empty_like(A0)
+    ids = torch.arange(1, n + 1, dtype=torch.int32, device="cuda")
+    jp = torch.empty_like(ids)
+    tau = torch.empty(n, dtype=torch.float64, device="cuda")
+    h = hqrrp_amd.Handle(0)
+    h.use_stream(torch.cuda.current_stream())
+
+    def step():
+        W.copy_(A0)
+        jp.copy_(ids)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record()
+        hqrrp_amd.geqp3_device(h, W, jp, tau, m, n)
+        e[1].record()
+        return e
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e = step()
+        torch.cuda.synchronize()
+        ms += e[0].elapsed_time(e[1])
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms /= args.steps
+    h.close()
+    fl = qr_flops(m, n)
+    # Algorithmic HBM bytes: step i reads and writes the trailing block
+    # A(i:m, i:n) once, 16 bytes per element (the kernels move whole columns,
+    # rows above i included, so they move more than this).
+    byts = sum(16.0 * (m - i) * (n - i) for i in range(min(m, n)))
+    ach = byts / (ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0,
+            "unit": "GB/s", "frac": round(ach / 8000.0, 4), "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        use_ref = oracle.ref() is not None
+        ts = []
+        for _ in range(4):
+            t1 = time.perf_counter()
+            oracle.dgeqp4(A_host, use_ref=use_ref)
+            ts.append(time.perf_counter() - t1)
+        dt = float(np.median(ts[1:]))
+        cpu = {"value": round(fl / dt / 1e9, 2), "unit": "GFLOP/s",
+               "cores": threads, "kind": "reference" if use_ref else "port",
+               "sample": (f"dgeqp4 on the same {m}x{n} matrix (its LAPACK "
+                          f"dgeqp3 branch, OpenBLAS, {threads} threads): "
+                          f"median of 3 after a warm-up, {dt * 1e3:.1f} ms")}
+    line = {
+        "metric": "classic QRCP GFLOP/s (2mn^2-2/3n^3), dgeqp4 small-size "
+                  "branch",
+        "value": round(fl / (ms / 1e3) / 1e9, 2), "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "wall_ms_per_step": round(wall, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "classic QRCP (dgeqp3 semantics) of a "
+                               "999x999 Gaussian", "m": m, "n": n},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,10 +414,13 @@ def main():
                     default="fp64",
                     help="sketch precision (panel, T and trailing update are "
                          "FP64 in every case)")
-    ap.add_argument("--workload", choices=("single", "batched", "adjoint"),
+    ap.add_argument("--workload",
+                    choices=("single", "batched", "adjoint", "classic"),
                     default="single",
                     help="single: one large HQRRP (the headline); batched: "
-                         "many small matrices in one launch")
+                         "many small matrices in one launch; adjoint: QR at "
+                         "fixed pivots and its adjoint; classic: dgeqp4's "
+                         "small-size classic QRCP")
     ap.add_argument("--count", type=int, default=8192,
                     help="matrices of the batched workload")
     ap.add_argument("--batch-nb", type=int, default=None,
@@ -369,6 +455,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "classic":
+        bench_classic(args)
         return
 
     import numpy as np

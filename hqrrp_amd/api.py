@@ -202,6 +202,16 @@ def geqrf_device(handle: Handle, At, tau_t, m: int, n: int,
           "hqrrp_geqrf_dev")
 
 
+def geqp3_device(handle: Handle, At, jpvt_t, tau_t, m: int, n: int,
+                 lda: int | None = None) -> None:
+    """Classic greedy QRCP (LAPACK dgeqp3 semantics) of the column-major
+    m x n matrix held by At, in place; jpvt_t (int32, n) permuted in place,
+    tau_t float64 (min(m,n)).  Stream-ordered on the handle's stream."""
+    check(lib().hqrrp_geqp3_dev(handle.ptr, m, n, _vp(At),
+                                m if lda is None else lda, _vp(jpvt_t),
+                                _vp(tau_t)), "hqrrp_geqp3_dev")
+
+
 def orgqr_device(handle: Handle, At, tau_t, m: int, n: int, k: int,
                  lda: int | None = None):
     """Q (column-major m x n, returned as a torch (n, m) tensor) from the

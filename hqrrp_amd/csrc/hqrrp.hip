@@ -179,6 +179,7 @@ struct hqrrp_handle_s {
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
+  DevBuf vn;              // classic QRCP: partial and reference norms
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
   // Batched path: block size per 64 x 64 shape bucket chosen by
@@ -997,6 +998,55 @@ int run_geqrf(hqrrp_handle_t h, int m, int n, int kmax, double* dA, long lda,
   return HQRRP_OK;
 }
 
+// Classic greedy QRCP of A(k0:m, k0:n) with LAPACK dgeqpf / dgeqp3 semantics
+// (the small-size branches of NoFLA_HQRRP_WY_blk_var4.c:189-199): partial
+// norms of rows k0.., then one pivot + update per column.  Swaps move whole
+// columns (all m rows) and jpvt entries, as LAPACK's dswap does, so rows
+// above k0 follow the pivots with no re-pivot pass.
+int run_classic_qrcp(hqrrp_handle_t h, int m, int n, int k0, double* dA,
+                     long lda, int* djpvt, double* dtau) {
+  const int mn = std::min(m, n);
+  if (k0 >= mn) return HQRRP_OK;
+  if (classic_fused_ok(m, n)) {
+    // One launch per step, ping-ponging X <-> Y (matrix, jpvt, norms).
+    const size_t mn2 = (size_t)m * n;
+    HQ_RET(ensure(h->vn, sizeof(double) * (2 * mn2 + 4 * (size_t)n) +
+                             sizeof(int) * 2 * (size_t)n));
+    double* X = h->vn.as<double>();
+    double* Y = X + mn2;
+    double* vX = Y + mn2;
+    double* vY = vX + 2 * (size_t)n;
+    int* jX = reinterpret_cast<int*>(vY + 2 * (size_t)n);
+    int* jY = jX + n;
+    HQ_TRY(launch_copy_block(X, m, dA, lda, m, n, h->cur));
+    HQ_TRY(hipMemcpyAsync(jX, djpvt, sizeof(int) * n,
+                          hipMemcpyDeviceToDevice, h->cur));
+    HQ_TRY(launch_classic_norms(X, m, m, n, k0, vX, vX + n, h->cur));
+    for (int i = k0; i < mn; ++i) {
+      HQ_TRY(launch_classic_fused(X, Y, m, m, n, i, jX, jY, vX, vY, dA, lda,
+                                  djpvt, dtau, h->cur));
+      std::swap(X, Y);
+      std::swap(jX, jY);
+      std::swap(vX, vY);
+    }
+    if (n > mn) {  // m < n: columns m.. are finished in X
+      HQ_TRY(launch_copy_block(dA + (long)mn * lda, lda, X + (size_t)mn * m,
+                               m, m, n - mn, h->cur));
+      HQ_TRY(hipMemcpyAsync(djpvt + mn, jX + mn, sizeof(int) * (n - mn),
+                            hipMemcpyDeviceToDevice, h->cur));
+    }
+    return HQRRP_OK;
+  }
+  HQ_RET(ensure(h->vn, sizeof(double) * 2 * (size_t)n));
+  double* vn1 = h->vn.as<double>();
+  double* vn2 = vn1 + n;
+  HQ_TRY(launch_classic_norms(dA, lda, m, n, k0, vn1, vn2, h->cur));
+  for (int i = k0; i < mn; ++i)
+    HQ_TRY(launch_classic_step(dA, lda, m, n, i, djpvt, dtau, vn1, vn2,
+                               h->cur));
+  return HQRRP_OK;
+}
+
 // Q (m x n) = the first n columns of H_1 ... H_k, the reflectors held in the
 // first k columns of A below the diagonal (LAPACK dorgqr, written to Q):
 // Q = [I; 0], then block reflectors from the last to the first applied to
@@ -1204,7 +1254,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->moves,     &h->piv,       &h->gram,
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
-                    &h->hA,    &h->hJ,    &h->hT,     &h->hW,     
+                    &h->hA,    &h->hJ,    &h->hT,     &h->hW,    &h->vn,
                     &h->Pn,    &h->Xb,     &h->adjAP,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
@@ -1365,6 +1415,15 @@ extern "C" int hqrrp_geqrf_dev(hqrrp_handle_t h, int m, int n, double* dA,
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
   return run_geqrf(h, m, n, std::min(m, n), dA, lda, dtau);
+}
+
+extern "C" int hqrrp_geqp3_dev(hqrrp_handle_t h, int m, int n, double* dA,
+                               int lda, int* djpvt, double* dtau) {
+  if (!h || m < 0 || n < 0 || lda < std::max(1, m)) return HQRRP_ERR_ARG;
+  if (std::min(m, n) == 0) return HQRRP_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau);
 }
 
 extern "C" int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k,
@@ -1808,6 +1867,42 @@ int host_dgeqp4_fixed(int m, int n, double* A, int lda, int* jpvt, double* tau,
   return HQRRP_OK;
 }
 
+// dgeqp4 for min(m,n) < 1000, where the reference calls LAPACK dgeqpf or
+// dgeqp3 (NoFLA_HQRRP_WY_blk_var4.c:189-199) with the fixed columns already
+// at the front: their QR with the reflectors applied to the rest (dgeqpf's
+// dgeqr2 + dorm2r), then the classic QRCP of the free columns.
+int host_dgeqp4_classic(int m, int n, double* A, int lda, int* jpvt,
+                        double* tau, int nf) {
+  hqrrp_handle_t h = nullptr;
+  HQ_RET(default_handle(&h));
+  std::lock_guard<std::mutex> lk(h->mu);
+  const int mn = std::min(m, n);
+  HQ_RET(ensure(h->hA, sizeof(double) * (size_t)std::max(1, m) * n));
+  HQ_RET(ensure(h->hJ, sizeof(int) * (size_t)std::max(1, n)));
+  HQ_RET(ensure(h->hT, sizeof(double) * (size_t)std::max(1, mn)));
+  double* dA = h->hA.as<double>();
+  int* dJ = h->hJ.as<int>();
+  double* dT = h->hT.as<double>();
+  h->cur = h->stream;
+  HQ_TRY(hipMemcpy2DAsync(dA, sizeof(double) * m, A, sizeof(double) * lda,
+                          sizeof(double) * m, n, hipMemcpyHostToDevice,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(dJ, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
+                        h->stream));
+  if (nf > 0) HQ_RET(run_geqrf(h, m, n, nf, dA, m, dT));
+  HQ_RET(run_classic_qrcp(h, m, n, nf, dA, m, dJ, dT));
+  HQ_RET(check_err(h));
+  HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
+                          sizeof(double) * m, n, hipMemcpyDeviceToHost,
+                          h->stream));
+  HQ_TRY(hipMemcpyAsync(jpvt, dJ, sizeof(int) * n, hipMemcpyDeviceToHost,
+                        h->stream));
+  HQ_TRY(hipMemcpyAsync(tau, dT, sizeof(double) * mn, hipMemcpyDeviceToHost,
+                        h->stream));
+  HQ_TRY(hipStreamSynchronize(h->stream));
+  return HQRRP_OK;
+}
+
 }  // namespace
 
 extern "C" int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double* buff_A,
@@ -1895,8 +1990,11 @@ extern "C" void dgeqp4(int* m, int* n, double* A, int* lda, int* jpvt,
   const int nf = std::min(mA, nfixed);
   hqrrp_options o;
   hqrrp_default_options(&o);
-  int rc = nf > 0 ? host_dgeqp4_fixed(mA, nA, A, ld, jpvt, tau, nf, o)
-                  : host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
+  // min(m,n) < 250 goes to dgeqpf and < 1000 to dgeqp3 in the reference
+  // (WY.c:189-199); both are the classic greedy QRCP here.
+  int rc = mn < 1000 ? host_dgeqp4_classic(mA, nA, A, ld, jpvt, tau, nf)
+           : nf > 0  ? host_dgeqp4_fixed(mA, nA, A, ld, jpvt, tau, nf, o)
+                     : host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
   if (rc != HQRRP_OK) {
     fprintf(stderr, "dgeqp4: device factorisation failed (%d)\n", rc);
     *info = rc;

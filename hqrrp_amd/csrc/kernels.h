@@ -245,6 +245,25 @@ hipError_t launch_permute_cols(const double* src, long lds, int m, int ncols,
                                const int* p, int scatter, double* dst,
                                long ldd, hipStream_t s);
 
+// k_classic.hip: classic greedy QRCP (LAPACK dgeqpf / dgeqp3), one step per
+// launch_classic_step; vn1/vn2: partial and reference norms (n each).
+hipError_t launch_classic_norms(const double* A, long lda, int m, int n,
+                                int k0, double* vn1, double* vn2,
+                                hipStream_t s);
+hipError_t launch_classic_step(double* A, long lda, int m, int n, int i,
+                               int* jpvt, double* tau, double* vn1,
+                               double* vn2, hipStream_t s);
+// Small shapes: one launch per step, X -> Y ping-pong (m x n, ld ldx each;
+// jpvt jX -> jY; norms vX -> vY with vn1 at [0, n), vn2 at [n, 2n)).
+constexpr int kClassicMaxRows = 1024;
+constexpr int kClassicMaxCols = 2048;
+int classic_fused_ok(int m, int n);
+hipError_t launch_classic_fused(const double* X, double* Y, long ldx, int m,
+                                int n, int i, const int* jX, int* jY,
+                                const double* vX, double* vY, double* A,
+                                long lda, int* jpvt, double* tau,
+                                hipStream_t s);
+
 // k_gemm32.hip
 struct Gemm32Args {
   int M, N, K;

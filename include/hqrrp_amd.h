@@ -138,6 +138,16 @@ void hqrrp_reference_omega(int r, int m, double *G, unsigned seed);
 int hqrrp_geqrf_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
                     double *dtau);
 
+/* Classic greedy column-pivoted QR, LAPACK dgeqp3 semantics (the first
+ * column of largest partial norm at every step, LAWN 176 norm downdating
+ * with recomputation): A (m x n, ld lda) factored in place, djpvt (n)
+ * permuted in place like hqrrp_factor_dev's, dtau (min(m,n)).  Replaces the
+ * dgeqpf_ / dgeqp3_ calls dgeqp4 makes for min(m,n) < 1000
+ * (NoFLA_HQRRP_WY_blk_var4.c:189-199); dgeqp4 itself routes those sizes
+ * here after moving its fixed columns to the front. */
+int hqrrp_geqp3_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
+                    int *djpvt, double *dtau);
+
 /* Q (m x n) = the first n columns of H_1 ... H_k, with the reflectors in the
  * first k columns of A below the diagonal and tau (LAPACK dorgqr semantics,
  * written out of place to Q); k <= n <= m.  Replaces the dorgqr_ call of
