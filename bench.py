@@ -46,7 +46,7 @@ PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 # reads for a plain copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE
 # x1.0).  Keyed by (phase, m, n, nb); other shapes report null.
 MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64):
-                    round((530206.2 * 2.0 + 702421.3) * 1024)}
+                    round((479734.3 * 2.0 + 702445.5) * 1024)}
 
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {

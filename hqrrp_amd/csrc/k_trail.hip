@@ -81,7 +81,8 @@ __device__ __forceinline__ void update_quadrant(const TrailArgs& a, long m0,
       for (int r = 0; r < 4; ++r) {
         const double* cc = cbase + (long)(16 * i + 4 * r) * ldc;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j][r] = cc[16 * j];
+        for (int j = 0; j < 4; ++j)
+          acc[i][j][r] = __builtin_nontemporal_load(cc + 16 * j);
       }
   } else {
 #pragma unroll
@@ -161,7 +162,8 @@ __device__ __forceinline__ void update_quadrant(const TrailArgs& a, long m0,
       for (int r = 0; r < 4; ++r) {
         double* cc = obase + (long)(16 * i + 4 * r) * ldo;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) cc[16 * j] = acc[i][j][r];
+        for (int j = 0; j < 4; ++j)
+          __builtin_nontemporal_store(acc[i][j][r], cc + 16 * j);
       }
   } else {
 #pragma unroll
@@ -273,7 +275,8 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
     if (VEC && k0 + kCK <= kend) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        cr[i] = cok[i] ? *reinterpret_cast<const hq_d4*>(cp[i] + k0)
+        cr[i] = cok[i] ? __builtin_nontemporal_load(
+                             reinterpret_cast<const hq_d4*>(cp[i] + k0))
                        : hq_d4{0.0, 0.0, 0.0, 0.0};
         vr[i] = vok[i] ? *reinterpret_cast<const hq_d4*>(vp[i] + k0)
                        : hq_d4{0.0, 0.0, 0.0, 0.0};
