@@ -86,6 +86,9 @@ CASES = [
     (64, 2000, "gaussian", 5), (257, 257, "lowrank", 6),
     (400, 300, "defective", 7), (1, 40, "gaussian", 8),
     (40, 1, "gaussian", 9), (2, 2, "gaussian", 10),
+    # the one-launch path's limits (1024 rows, 2048 columns) and past them
+    (1024, 700, "graded", 11), (700, 2048, "gaussian", 12),
+    (100, 3000, "gaussian", 13), (1025, 300, "graded", 14),
 ]
 
 
