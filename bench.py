@@ -23,10 +23,17 @@ Also reported:
                    are bracketed by HIP events (recorded on the stream they
                    run on), and its algorithmic flops (or bytes) per launch
                    are divided by its average launch duration;
-  cpu_baseline  -- the oracle (a CPU restatement of the reference driver
-                   making the same LAPACK calls, OpenBLAS) timed on the host
-                   cores on the first --cpu-blocks blocks of the same matrix,
-                   rated with the flops of those columns.
+  cpu_baseline  -- the reference's dgeqp4 compiled from its source against
+                   OpenBLAS (oracle/_ref; the oracle's restatement where it
+                   was not built) on the whole of the same matrix, timed on
+                   the host cores; cpu_dgeqp3 beside it: LAPACK dgeqp3 from
+                   the same OpenBLAS on a matrix of the same family at
+                   --cpu-dgeqp3-n (its 16384 run takes minutes);
+  quality       -- the result's backward error ||AP - QR||_F / ||A||_F,
+                   orthogonality max|Q'Q - I| (Q from the library's dorgqr,
+                   products in FP64) and the rank-revealing ratio
+                   sigma_k(A) / |R_kk| (sigma_k known exactly for the
+                   generated matrices).
 """
 import argparse
 import json
@@ -69,29 +76,94 @@ def qr_flops(m, n):
     return 2.0 * n * m * m - 2.0 / 3.0 * m ** 3
 
 
-def partial_qr_flops(m, n, k):
-    """Householder QR flops of the first k columns of an m x n matrix."""
-    return 4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 / 3.0 * k ** 3
-
-
-def cpu_baseline(A_host, nb, pp, blocks, threads):
+def _median_time(fn, reps):
+    """Median wall time of `reps` calls after one warm-up (reps > 1), or of
+    a single call."""
     import numpy as np
 
+    if reps > 1:
+        fn()
+    ts = []
+    for _ in range(max(1, reps)):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baseline(A_host, nb, pp, threads, reps):
+    """The reference's dgeqp4 (compiled from its source, OpenBLAS) on the
+    whole matrix; the oracle's restatement of its driver where the compiled
+    reference is absent."""
     import oracle
 
     oracle.set_threads(threads)
     m, n = A_host.shape
-    t0 = time.perf_counter()
-    oracle.hqrrp(A_host, nb, pp, 1, max_blocks=blocks)
-    dt = time.perf_counter() - t0
-    k = min(blocks * nb, min(m, n))
-    fl = partial_qr_flops(m, n, k)
+    use_ref = oracle.ref() is not None and (nb, pp) == (64, 10)
+    if use_ref:
+        fn = lambda: oracle.dgeqp4(A_host, use_ref=True)  # noqa: E731
+        what = ("the reference's dgeqp4 (NoFLA_HQRRP_WY_blk_var4.c compiled "
+                "from source, OpenBLAS")
+    else:
+        fn = lambda: oracle.hqrrp(A_host, nb, pp, 1)  # noqa: E731
+        what = ("the oracle's restatement of NoFLA_HQRRP_WY_blk_var4 "
+                "(OpenBLAS")
+    dt, runs = _median_time(fn, reps)
+    fl = qr_flops(m, n)
+    how = (f"median of {runs} after a warm-up" if runs > 1 else "one run")
     return {"value": round(fl / dt / 1e9, 2), "unit": "GFLOP/s",
-            "cores": threads, "kind": "port",
-            "sample": (f"oracle restatement of NoFLA_HQRRP_WY_blk_var4 "
-                       f"(OpenBLAS, {threads} threads) on the first {k} "
-                       f"columns ({blocks} blocks) of the same {m}x{n} "
-                       f"matrix: {dt:.2f} s for {fl:.3e} flops")}
+            "cores": threads, "kind": "reference" if use_ref else "port",
+            "sample": (f"{what}, {threads} threads) on the whole {m}x{n} "
+                       f"matrix: {dt:.2f} s ({how})")}
+
+
+def cpu_dgeqp3(A_host, threads, reps):
+    import oracle
+
+    oracle.set_threads(threads)
+    m, n = A_host.shape
+    dt, runs = _median_time(lambda: oracle.dgeqp3(A_host), reps)
+    return {"value": round(qr_flops(m, n) / dt / 1e9, 2), "unit": "GFLOP/s",
+            "cores": threads, "kind": "lapack",
+            "sample": (f"LAPACK dgeqp3 (OpenBLAS, {threads} threads) on a "
+                       f"{m}x{n} matrix of the same family: {dt:.2f} s "
+                       f"({'median of %d after a warm-up' % runs if runs > 1 else 'one run'})")}
+
+
+def factor_quality(h, A0, Af, jp, tau, m, n, sigma=None, kq=None):
+    """Backward error, orthogonality and sigma_k / |R_kk| of a factorisation
+    on the device (FP64 products)."""
+    import torch
+
+    import hqrrp_amd
+
+    k = min(m, n)
+    Qt = hqrrp_amd.orgqr_device(h, Af, tau, m, k, k)
+    h.synchronize()
+    torch.cuda.synchronize()
+    R = torch.triu(Af.t()[:k, :])
+    AP = A0[(jp.long() - 1)].t()
+    res = (torch.linalg.matrix_norm(AP - Qt.t() @ R)
+           / torch.linalg.matrix_norm(AP)).item()
+    del AP
+    orth = (Qt @ Qt.t() - torch.eye(k, dtype=torch.float64, device=Af.device)
+            ).abs().max().item()
+    eps = 2.0 ** -52
+    out = {"backward_error": res, "orthogonality": orth,
+           "backward_error_over_n_eps": round(res / (n * eps), 4),
+           "orthogonality_over_n_eps": round(orth / (n * eps), 4),
+           "bound_over_n_eps": 50}
+    if sigma is not None:
+        d = torch.diagonal(R).abs()
+        ratio = sigma[:k] / d
+        out["rank_ratio"] = {
+            "sigma_k_over_abs_Rkk_min": round(ratio.min().item(), 4),
+            "sigma_k_over_abs_Rkk_max": round(ratio.max().item(), 4),
+            "sigma_k_over_abs_Rkk_median": round(ratio.median().item(), 4)}
+        if kq is not None:
+            out["rank_ratio"]["k"] = kq
+            out["rank_ratio"]["at_k"] = round(ratio[kq - 1].item(), 4)
+    return out
 
 
 def _oracle_slice(job):
@@ -398,8 +470,13 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--pp", type=int, default=10)
-    ap.add_argument("--cpu-blocks", type=int, default=16)
+    ap.add_argument("--cpu-reps", type=int, default=1,
+                    help="CPU baseline runs (> 1: a warm-up, then the median)")
+    ap.add_argument("--cpu-dgeqp3-n", type=int, default=8192,
+                    help="size of LAPACK dgeqp3's sample (0: skip; the "
+                         "benchmark size takes minutes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--lookahead", type=int, default=1,
                     help="overlap the next panel with the trailing update")
@@ -470,10 +547,19 @@ def main():
     dev = "cuda:0"
     print(f"# generating {args.input} {m}x{n} input on the GPU",
           file=sys.stderr, flush=True)
+    sigma, kq = None, None
     if args.input == "graded":
         A0 = matgen.graded_device(m, n, seed=0, device=dev)
+        sigma = 10.0 ** (-12.0 * torch.arange(min(m, n), device=dev,
+                                              dtype=torch.float64) / min(m, n))
+        kq = min(m, n) // 2
     elif args.input == "lowrank":
-        A0 = matgen.lowrank_device(m, n, rank=512, seed=0, device=dev)
+        A0, sigma = matgen.lowrank_device(m, n, rank=512, seed=0, device=dev,
+                                          core_sigma=True)
+        sigma = torch.cat([sigma, torch.zeros(min(m, n) - sigma.numel(),
+                                              device=dev,
+                                              dtype=torch.float64)])
+        kq = 512
     else:
         A0 = matgen.gaussian_device(m, n, seed=0, device=dev)
     sketch_prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.sketch]
@@ -560,13 +646,34 @@ def main():
                     "launches": launches,
                     "avg_launch_us": round(avg_s * 1e6, 2)}
 
-    cpu = None
+    quality = None
+    if not args.no_quality:
+        quality = factor_quality(h, A0, A, jp, tau, m, n, sigma, kq)
+        if args.input == "lowrank" and quality.get("rank_ratio"):
+            # sigma_k of A differs from the noise-free core's by at most
+            # ||noise||_2 <= 1e-8 (sqrt(m) + sqrt(n)) (Weyl); only k <= 512
+            # is meaningful, so report at k = 512.
+            rr = quality["rank_ratio"]
+            quality["rank_ratio"] = {"k": rr["k"], "at_k": rr["at_k"]}
+        print(f"# quality {quality}", file=sys.stderr, flush=True)
+    cpu, cpu3 = None, None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         print(f"# cpu baseline on {threads} threads", file=sys.stderr,
               flush=True)
         A_host = np.asfortranarray(A0.cpu().numpy().T)
-        cpu = cpu_baseline(A_host, nb, pp, args.cpu_blocks, threads)
+        cpu = cpu_baseline(A_host, nb, pp, threads, args.cpu_reps)
+        print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
+        del A_host
+        if args.cpu_dgeqp3_n > 0:
+            q = min(args.cpu_dgeqp3_n, m, n)
+            Aq = (matgen.graded_device(q, q, seed=0, device=dev)
+                  if args.input == "graded" else
+                  matgen.gaussian_device(q, q, seed=0, device=dev))
+            Aq_host = np.asfortranarray(Aq.cpu().numpy().T)
+            del Aq
+            cpu3 = cpu_dgeqp3(Aq_host, threads, args.cpu_reps)
+            print(f"# cpu dgeqp3 {cpu3}", file=sys.stderr, flush=True)
 
     h.close()
     line = {
@@ -586,7 +693,9 @@ def main():
                    "m": m, "n": n, "nb": nb, "pp": pp, "panel_pivoting": 1,
                    "sketch": args.sketch, "omega": "reference stream seed 12",
                    "lookahead": args.lookahead},
-        "roofline": roof, "cpu_baseline": cpu, "phase_ms": phase_ms,
+        "roofline": roof, "cpu_baseline": cpu, "cpu_dgeqp3": cpu3,
+        "vs_cpu_baseline": (round(value / cpu["value"], 2) if cpu else None),
+        "quality": quality, "phase_ms": phase_ms,
         "gemm_roofline": gemm_roof,
     }
     print(json.dumps(line), flush=True)

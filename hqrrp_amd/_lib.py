@@ -16,6 +16,17 @@ _i = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
 
+class Trace(C.Structure):
+    """Mirror of ``hqrrp_trace`` (device pointers or NULL)."""
+
+    _fields_ = [
+        ("sketch_cols", _vp),
+        ("y0", _vp),
+        ("y", _vp),
+        ("g", _vp),
+    ]
+
+
 class Options(C.Structure):
     """Mirror of ``hqrrp_options``."""
 
@@ -29,6 +40,7 @@ class Options(C.Structure):
         ("omega", _vp),
         ("max_rank", C.c_int),
         ("lookahead", C.c_int),
+        ("trace", C.POINTER(Trace)),
     ]
 
 

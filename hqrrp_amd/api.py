@@ -18,7 +18,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import Options, check, lib
+from ._lib import Options, Trace, check, lib
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int)
@@ -159,7 +159,9 @@ class Handle:
 
 
 def options(nb=64, pp=10, panel_pivoting=1, seed=12, omega=None,
-            max_rank=0, sketch_prec=0, lookahead=None) -> Options:
+            max_rank=0, sketch_prec=0, lookahead=None, trace=None) -> Options:
+    """Options struct; `trace` is a dict of device tensors with any of the
+    keys of ``hqrrp_trace`` (sketch_cols, y0, y, g) to read back."""
     o = Options()
     lib().hqrrp_default_options(C.byref(o))
     o.nb, o.pp, o.panel_pivoting, o.seed = nb, pp, panel_pivoting, seed
@@ -170,6 +172,12 @@ def options(nb=64, pp=10, panel_pivoting=1, seed=12, omega=None,
     if omega is not None:
         o.omega_kind = 1
         o.omega = C.c_void_p(omega.data_ptr())
+    if trace:
+        t = Trace()
+        for k, v in trace.items():
+            setattr(t, k, C.c_void_p(v.data_ptr()))
+        o._trace_keep = t  # the struct must outlive the call
+        o.trace = C.pointer(t)
     return o
 
 

@@ -395,6 +395,14 @@ int gemm32(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K,
   return HQRRP_OK;
 }
 
+// The mixed-precision sketch Y = Omega A (r x n, FP32) from G' = Omega'
+// (m x r, FP64) and A (m x n, ld lda, FP64).
+int sketch_gemm_mixed(hqrrp_handle_t h, int r, int m, int n, const double* Gt,
+                      const double* dA, long lda, float* Y) {
+  return gemm32(h, true, false, r, n, m, 1.0f, Gt, m, 1, 0, dA, lda, 1, 0,
+                0.0f, Y, r, h->skbf);
+}
+
 int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
   int n = hq_cdiv(mrows, panel_rows_per_wg());
   if (n > h->panel_max_wg) return HQRRP_ERR_UNSUPPORTED;
@@ -804,8 +812,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     // G holds Omega' (m x r, FP64); the mixed sketch forms Y in FP32
     // (optionally bf16-rounded operands) with FP32 accumulation.
     if (h->sk32) {
-      HQ_RET(gemm32(h, true, false, r, n, m, 1.0f, G, m, 1, 0, dA, lda, 1, 0,
-                    0.0f, static_cast<float*>(Y), r, h->skbf));
+      HQ_RET(sketch_gemm_mixed(h, r, m, n, static_cast<const double*>(G), dA,
+                               lda, static_cast<float*>(Y)));
     } else {
       // Y = G'' A is a V'C product with G' (unmasked) in the role of V: up
       // to 64 sketch rows per pass, each pass one streaming read of A with
@@ -824,14 +832,35 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     }
   }
 
+  const hqrrp_trace* tr = o.trace;
+  const size_t ysz = h->sk32 ? sizeof(float) : sizeof(double);
+  if (tr && tr->y0)
+    HQ_TRY(hipMemcpyAsync(tr->y0, Y, ysz * (size_t)r * n,
+                          hipMemcpyDeviceToDevice, h->cur));
+  // Block j's sketch choices, in the sketch's order: jpvt(j:j+b) right after
+  // the block's column gather, before the panel reorders them.
+  auto trace_sketch = [&](int j, int b) -> int {
+    if (tr && tr->sketch_cols)
+      HQ_TRY(hipMemcpyAsync(tr->sketch_cols + j, djpvt + j, sizeof(int) * b,
+                            hipMemcpyDeviceToDevice, h->cur));
+    return HQRRP_OK;
+  };
+
   // G' (m x r, ld m, FP64) is updated as extra columns of the trailing
   // matrix and the downdate of Y follows the block tail (block_tail); Y is
   // FP64 or, for the mixed sketch, FP32.
   double* Gt = static_cast<double*>(G);
   bool have_panel = false;  // block j's panel already factored by look-ahead
   bool t_pending = false;   // its T still being formed on s_chain (ev_t)
+  // Width of the block starting at j: nb columns, cut to end at kmax unless
+  // it is the reference's final block (the panel of all that remains).
+  auto block_width = [&](int j) -> int {
+    const int w = std::min(nb, std::min(n - j, m - j));
+    const bool fin = (j + nb >= m) || (j + nb >= n);
+    return fin ? w : std::min(w, kmax - j);
+  };
   for (int j = 0; j < kmax; j += nb) {
-    const int b = std::min(nb, std::min(n - j, m - j));
+    const int b = block_width(j);
     const bool last = (j + nb >= m) || (j + nb >= n);
     const int mrows = m - j, nc = n - j - b;
     double* Ajj = dA + j + (long)j * lda;
@@ -839,6 +868,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       if (!last) {
         HQ_RET(block_sketch(h, r, j, n, b, Y));
         HQ_RET(block_gather(h, r, j, m, b, dA, lda, Y, djpvt));
+        HQ_RET(trace_sketch(j, b));
       }
       HQ_RET(run_panel(h, mrows, b, Ajj, lda, o.panel_pivoting, dtau + j,
                        h->perm.as<int>(), T, nb));
@@ -881,7 +911,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     const int xn = xc ^ 1;
     double* Pn = h->Pn.as<double>();
     const int mnext = m - jn;
-    const int b2 = std::min(nb, std::min(n - jn, m - jn));
+    const int b2 = block_width(jn);
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
     ta.Cout = X[xn] + j + (long)jn * ldx[xn];
     ta.ldco = ldx[xn];
@@ -917,7 +947,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                                   h->tmpY.as<double>(), h->tmpJ.as<int>(),
                                   h->cur));
       }
-      return HQRRP_OK;
+      return trace_sketch(jn, b2);
     };
     if (rc == HQRRP_OK) {
       h->cur = h->s_upd;
@@ -973,6 +1003,12 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   // A last T formed on s_chain but never consumed (no trailing columns)
   // still reads h->gram: order the caller's next work after it.
   if (t_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_t, 0));
+  if (tr && tr->y)
+    HQ_TRY(hipMemcpyAsync(tr->y, Y, ysz * (size_t)r * n,
+                          hipMemcpyDeviceToDevice, h->cur));
+  if (tr && tr->g)
+    HQ_TRY(hipMemcpyAsync(tr->g, Gt, sizeof(double) * (size_t)m * r,
+                          hipMemcpyDeviceToDevice, h->cur));
   HQ_TRY(hipGetLastError());
   return HQRRP_OK;
 }
@@ -1997,7 +2033,7 @@ extern "C" void dgeqp4(int* m, int* n, double* A, int* lda, int* jpvt,
                      : host_hqrrp(mA, nA, A, ld, jpvt, tau, o);
   if (rc != HQRRP_OK) {
     fprintf(stderr, "dgeqp4: device factorisation failed (%d)\n", rc);
-    *info = rc;
+    *info = HQRRP_INFO_DEVICE - rc;
     return;
   }
   work[0] = iws;

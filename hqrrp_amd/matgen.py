@@ -76,10 +76,16 @@ def gaussian_device(m: int, n: int, seed: int = 0, device: str = "cuda"):
 
 
 def lowrank_device(m: int, n: int, rank: int = 512, seed: int = 0,
-                   noise: float = 1e-8, device: str = "cuda"):
+                   noise: float = 1e-8, device: str = "cuda",
+                   core_sigma: bool = False):
     """Same family as :func:`lowrank_noise` (rank `rank`, sigma_i = 1/i,
     plus `noise` N(0,1)), built with torch on the GPU; returned column-major
-    as a (n, m) tensor like :func:`gaussian_device`."""
+    as a (n, m) tensor like :func:`gaussian_device`.
+
+    core_sigma=True also returns the singular values of the noise-free core
+    L diag(1/i) R' (exactly, from the SVD of R_L diag(1/i) R_R' with
+    L = Q_L R_L, R = Q_R R_R); by Weyl's inequality those of A differ from
+    them by at most ||noise||_2 <= noise (sqrt(m) + sqrt(n)) (1 + o(1))."""
     import torch
 
     g = torch.Generator(device=device)
@@ -90,7 +96,12 @@ def lowrank_device(m: int, n: int, rank: int = 512, seed: int = 0,
     sig = 1.0 / torch.arange(1, rank + 1, device=device, dtype=f64)
     At = (R * sig) @ L.t()  # (n, m): A' = R diag(sig) L'
     At += noise * torch.randn(n, m, generator=g, device=device, dtype=f64)
-    return At
+    if not core_sigma:
+        return At
+    RL = torch.linalg.qr(L, mode="r")[1]
+    RR = torch.linalg.qr(R, mode="r")[1]
+    s = torch.linalg.svdvals((RL * sig) @ RR.t())
+    return At, s
 
 
 # ---------------------------------------------------------------- batches

@@ -42,7 +42,11 @@ typedef enum {
  *  NoFLA_HQRRP_WY_blk_var4.c:127-300).  Host pointers; A is overwritten with
  * R and the Householder vectors (dgeqrf storage), jpvt/tau as in dgeqp3.
  * info: 0, -1 (m<0), -2 (n<0), -4 (lda<max(1,m)), -8 (lwork too small);
- * lwork == -1 is a workspace query answered in work[0]. */
+ * lwork == -1 is a workspace query answered in work[0].  A failure of the
+ * device (no counterpart in LAPACK, whose negative codes name arguments)
+ * is reported as info = HQRRP_INFO_DEVICE - status (> 0), e.g. 1002 for an
+ * allocation failure; A, jpvt and tau are then unspecified. */
+#define HQRRP_INFO_DEVICE 1000
 void dgeqp4(int *m, int *n, double *A, int *lda, int *jpvt, double *tau,
             double *work, int *lwork, int *info);
 
@@ -76,6 +80,23 @@ typedef enum {
   HQRRP_OMEGA_USER = 1       /* caller-supplied device matrix (r x m, ld r) */
 } hqrrp_omega_kind;
 
+/* Optional read-back of the driver's internal state, for tests of the
+ * pieces the reference checks only in debug builds (its
+ * CHECK_DOWNDATING_OF_Y block, NoFLA_HQRRP_WY_blk_var4.c:457-490) and of
+ * the pivot decisions.  Every pointer is a device pointer or NULL (not
+ * written); r = nb + pp. */
+typedef struct {
+  int *sketch_cols; /* min(m,n) ints: entry j+s = the column (jpvt value)
+                       that block j's sketch chose at step s, before the
+                       panel reorders the block                          */
+  void *y0;         /* r x n, ld r: the sketch Y = Omega*A right after it is
+                       formed (FP64, or FP32 for a mixed sketch)         */
+  void *y;          /* r x n, ld r: the sketch when the factorisation
+                       stops (columns kmax.. hold the downdated Y_R)     */
+  double *g;        /* m x r, ld m: Omega' when it stops (rows kmax.. hold
+                       the downdated G_R')                               */
+} hqrrp_trace;
+
 typedef struct {
   int nb;              /* block size nb_alg (reference default 64)           */
   int pp;              /* oversampling (reference default 10)                */
@@ -84,9 +105,18 @@ typedef struct {
   int omega_kind;      /* hqrrp_omega_kind                                   */
   unsigned seed;       /* seed of the reference stream (reference: 12)       */
   const double *omega; /* device pointer when omega_kind == HQRRP_OMEGA_USER */
-  int max_rank;        /* stop after this many columns (<=0: min(m,n))       */
+  /* Truncation (no counterpart in the reference): stop after k = max_rank
+   * columns (<= 0: min(m,n)).  Blocks of nb columns run as in the full
+   * factorisation; the block that reaches k is cut to end at column k (its
+   * sketch takes k - j pivots, its panel has k - j columns), except the
+   * reference's final block (j + nb >= m or j + nb >= n), which is always
+   * factored whole.  On exit columns 0..k-1 hold R's first k rows and the
+   * reflectors, and A(k:m, k:n) holds the trailing matrix Q' A P with Q of
+   * those k reflectors. */
+  int max_rank;
   int lookahead;       /* 1: overlap the next block's panel with the trailing
                           update (results bitwise equal to 0)                */
+  const hqrrp_trace *trace; /* NULL: no read-back                            */
 } hqrrp_options;
 
 /* Fills *opt with the reference defaults (64, 10, 1, FP64, seed 12). */

@@ -106,8 +106,14 @@ def ref_omega(r: int, m: int) -> np.ndarray:
 
 # ---------------------------------------------------------------- drivers
 def hqrrp(A, nb=64, pp=10, panel_pivoting=1, jpvt=None, omega_in=None,
-          max_blocks=0, trace=False, use_ref=False):
-    """Factor a copy of A; returns (A_out, jpvt (1-based), tau[, traces])."""
+          max_blocks=0, trace=False, use_ref=False, max_cols=0):
+    """Factor a copy of A; returns (A_out, jpvt (1-based), tau[, traces]).
+
+    trace=True appends (sketch gaps, panel gaps, sketch order): per pivot
+    position, the (largest, second largest) candidate partial norms of the
+    sketch and of the panel decision, and the columns each block's sketch
+    chose in its own order (jpvt values).  max_cols > 0 truncates (see
+    orc_hqrrp_ex)."""
     A = fortran(A)
     m, n = A.shape
     jp = (np.arange(1, n + 1, dtype=np.int32) if jpvt is None
@@ -124,14 +130,17 @@ def hqrrp(A, nb=64, pp=10, panel_pivoting=1, jpvt=None, omega_in=None,
     mn = min(m, n)
     ts = np.zeros(2 * max(mn, 1)) if trace else None
     tp = np.zeros(2 * max(mn, 1)) if trace else None
+    sc = np.zeros(max(mn, 1), dtype=np.int32) if trace else None
     om = None if omega_in is None else fortran(omega_in)
     lib().orc_hqrrp_ex(
         C.c_int(m), C.c_int(n), _dp(A), C.c_int(m), _ip(jp), _dp(tau),
         C.c_int(nb), C.c_int(pp), C.c_int(panel_pivoting),
         _dp(om) if om is not None else None, C.c_int(max_blocks),
-        _dp(ts) if trace else None, _dp(tp) if trace else None, None, None)
+        _dp(ts) if trace else None, _dp(tp) if trace else None, None, None,
+        C.c_int(max_cols), _ip(sc) if trace else None)
     if trace:
-        return A, jp, tau[:mn], ts.reshape(-1, 2)[:mn], tp.reshape(-1, 2)[:mn]
+        return (A, jp, tau[:mn], ts.reshape(-1, 2)[:mn],
+                tp.reshape(-1, 2)[:mn], sc[:mn])
     return A, jp, tau[:mn]
 
 

@@ -426,10 +426,17 @@ int orc_downdate_y(int b, int mU21, const double *U11, int ldu,
 /* ------------------------------------------------------------------------- */
 /* Blocked HQRRP driver (WY.c:303-571).                                       */
 
+/* Beyond the reference (test instrumentation and truncation): max_blocks
+ * stops after that many blocks; max_cols > 0 stops after max_cols columns,
+ * the block that reaches max_cols cut to end there unless it is the
+ * reference's final block (last_iter), which is factored whole;
+ * sketch_cols[j + s] receives the column block j's sketch chose at step s
+ * (jpvt after the sketch QRCP, before the panel reorders the block). */
 int orc_hqrrp_ex(int m, int n, double *A, int lda, int *jpvt, double *tau,
                  int nb, int pp, int panel_pivoting, const double *omega,
                  int max_blocks, double *trace_sketch, double *trace_panel,
-                 double *omega_out, double *y_out) {
+                 double *omega_out, double *y_out, int max_cols,
+                 int *sketch_cols) {
   static const double one = 1.0, zero = 0.0;
   if (m < 0) fprintf(stderr, "ERROR in oracle HQRRP: m_A is < 0.\n");
   if (n < 0) fprintf(stderr, "ERROR in oracle HQRRP: n_A is < 0.\n");
@@ -463,9 +470,11 @@ int orc_hqrrp_ex(int m, int n, double *A, int lda, int *jpvt, double *tau,
   int blocks = 0;
   for (int j = 0; j < mn; j += nb) {
     if (max_blocks > 0 && blocks >= max_blocks) break;
+    if (max_cols > 0 && j >= max_cols) break;
     ++blocks;
     int b = OMIN(nb, OMIN(n - j, m - j));
     int last = (j + nb >= m) || (j + nb >= n);
+    if (max_cols > 0 && !last) b = OMIN(b, max_cols - j);
     double *A_R = A + (size_t)j * lda;
     double *Y_R = Y + (size_t)j * r;
     double *V_R = V + (size_t)j * r;
@@ -483,6 +492,7 @@ int orc_hqrrp_ex(int m, int n, double *A, int lda, int *jpvt, double *tau,
       LP.dlacpy("All", &r, &nVR, Y_R, &r, V_R, &r, 3);
       orc_qrpmod(1, b, r, nVR, V_R, r, jpvt + j, tau + j, 1, m, A_R, lda, 1, r,
                  Y_R, r, 0, Y, r, trace_sketch ? trace_sketch + 2 * j : NULL);
+      if (sketch_cols) memcpy(sketch_cols + j, jpvt + j, sizeof(int) * b);
     }
     orc_qrpmod(panel_pivoting, -1, m - j, b, A11, lda, jpvt + j, tau + j, 1, j,
                A_R, lda, 1, r, Y_R, r, 1, T1, nb,
@@ -503,7 +513,7 @@ int orc_hqrrp_ex(int m, int n, double *A, int lda, int *jpvt, double *tau,
 int orc_hqrrp(int m, int n, double *A, int lda, int *jpvt, double *tau, int nb,
               int pp, int panel_pivoting) {
   return orc_hqrrp_ex(m, n, A, lda, jpvt, tau, nb, pp, panel_pivoting, NULL, 0,
-                      NULL, NULL, NULL, NULL);
+                      NULL, NULL, NULL, NULL, 0, NULL);
 }
 
 /* ------------------------------------------------------------------------- */

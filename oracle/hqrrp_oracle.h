@@ -39,7 +39,8 @@ int orc_downdate_y(int b, int mU21, const double *U11, int ldu,
 int orc_hqrrp_ex(int m, int n, double *A, int lda, int *jpvt, double *tau,
                  int nb, int pp, int panel_pivoting, const double *omega,
                  int max_blocks, double *trace_sketch, double *trace_panel,
-                 double *omega_out, double *y_out);
+                 double *omega_out, double *y_out, int max_cols,
+                 int *sketch_cols);
 int orc_hqrrp(int m, int n, double *A, int lda, int *jpvt, double *tau, int nb,
               int pp, int panel_pivoting);
 void orc_dgeqp4(int *m, int *n, double *A, int *lda, int *jpvt, double *tau,

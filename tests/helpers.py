@@ -40,33 +40,64 @@ def rel_gap(tr_row, scale):
     return (best - second) / scale if scale > 0 else 0.0
 
 
-def check_pivots(jp_gpu, jp_ref, trace_sketch, trace_panel, nb):
+def check_pivots(jp_gpu, jp_ref, trace_sketch, trace_panel, nb,
+                 sketch_gpu=None, sketch_ref=None, panel_pivoting=1,
+                 label=""):
     """Assert equal pivots, or a first divergence at a documented near-tie.
 
-    Returns the index of the first divergence (None when identical).  A
-    divergence in block j is attributed to the sketch when the block's set of
-    columns differs, otherwise to the panel ordering; the oracle's relative
-    gap between the two largest candidates at the responsible steps must be
-    below NEAR_TIE.
+    Returns the index of the first divergence (None when identical) and
+    prints every divergence with the step it is attributed to.
+
+    The divergence at position i lies in block j = nb * (i // nb).  It is
+    attributed to the exact decision that made it:
+
+    * to block j's sketch when the block's set of columns differs (or the
+      panel does not pivot): the first step s at which the GPU's sketch
+      order (sketch_gpu, the library's trace) differs from the oracle's
+      (sketch_ref); the oracle's gap between the two largest candidate
+      sketch norms at step s must be below NEAR_TIE of the sketch's scale;
+    * otherwise to the panel decision at position i, with the oracle's
+      panel gap at i.
+
+    Where the GPU's sketch order is not available (the batched kernel), the
+    sketch step is bounded: the first oracle step t whose column is not in
+    the GPU's block, the GPU having diverged at some step j..t; the smallest
+    oracle gap over j..t must then be below NEAR_TIE.
     """
+    jp_gpu = np.asarray(jp_gpu)
+    jp_ref = np.asarray(jp_ref)
     i = first_divergence(jp_gpu, jp_ref)
     if i is None:
         return None
     j = (i // nb) * nb
-    blk_g = set(np.asarray(jp_gpu)[j:j + nb].tolist())
-    blk_r = set(np.asarray(jp_ref)[j:j + nb].tolist())
-    if blk_g != blk_r:
+    e = min(j + nb, len(jp_ref))
+    blk_g = set(jp_gpu[j:e].tolist())
+    blk_r = set(jp_ref[j:e].tolist())
+    if blk_g != blk_r or not panel_pivoting:
         scale = trace_sketch[0][0]
-        gaps = [rel_gap(trace_sketch[k], scale)
-                for k in range(j, min(j + nb, len(trace_sketch)))]
-        where = "sketch"
+        if sketch_gpu is not None:
+            sg = np.asarray(sketch_gpu)[j:e]
+            sr = np.asarray(sketch_ref)[j:e]
+            d = np.nonzero(sg != sr)[0]
+            assert d.size, (f"{label}: block {j} differs but its sketch "
+                            "orders agree")
+            s = j + int(d[0])
+            g = rel_gap(trace_sketch[s], scale)
+            where = f"sketch step {s}"
+        else:
+            sr = np.asarray(sketch_ref)[j:e]
+            out = [k for k in range(e - j) if int(sr[k]) not in blk_g]
+            t = j + (out[0] if out else e - j - 1)
+            g = min(rel_gap(trace_sketch[k], scale) for k in range(j, t + 1))
+            where = f"sketch steps {j}..{t}"
     else:
-        gaps = [rel_gap(trace_panel[i], trace_panel[0][0])]
-        where = "panel"
-    g = min(gaps)
+        g = rel_gap(trace_panel[i], trace_panel[0][0])
+        where = f"panel step {i}"
+    print(f"{label}: pivots diverge at {i} ({where}), oracle gap {g:.3e} "
+          f"of the scale", flush=True)
     assert g < NEAR_TIE, (
-        f"pivots diverge at {i} ({where} decision of block {j}) where the "
-        f"oracle's smallest gap is {g:.3e} of the scale (>= {NEAR_TIE})")
+        f"{label}: pivots diverge at {i} ({where}) where the oracle's gap is "
+        f"{g:.3e} of the scale (>= {NEAR_TIE})")
     return i
 
 

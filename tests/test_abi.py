@@ -82,3 +82,29 @@ def test_driver_argument_errors():
     # Quick return for an empty matrix, as the reference.
     assert L.NoFLA_HQRRP_WY_blk_var4(0, 2, vp, 1, C.c_void_p(ip.ctypes.data),
                                      vp, 64, 10, 1) == 0
+
+
+def _gpu_present():
+    try:
+        return C.CDLL("libamdhip64.so").hipGetDeviceCount is not None and \
+            _device_count() > 0
+    except OSError:
+        return False
+
+
+def _device_count():
+    n = C.c_int(0)
+    rc = C.CDLL("libamdhip64.so").hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def test_dgeqp4_device_failure_is_not_an_argument_code():
+    """Without a usable device the factorisation cannot run: dgeqp4 reports
+    HQRRP_INFO_DEVICE - status (> 0), never one of LAPACK's negative
+    argument codes (info = -5 would read as 'jpvt illegal')."""
+    import pytest
+
+    if _gpu_present():
+        pytest.skip("a device is present")
+    info, _ = _call_dgeqp4(1200, 1000, 1200, 128 * 1000)
+    assert info == 1000 - (-3), info  # HQRRP_ERR_HIP

@@ -1,8 +1,10 @@
 """End-to-end parity of the GPU HQRRP with the oracle (a CPU restatement of
 NoFLA_HQRRP_WY_blk_var4.c that is bitwise equal to the compiled reference).
 
-* jpvt must equal the oracle's, or first diverge at a documented near-tie
-  (relative gap of the two largest partial norms < 1e-12);
+* jpvt must equal the oracle's; every case here does, so a divergence fails
+  the test even where tests.helpers.check_pivots would attribute it to a
+  near-tie (relative gap of the two largest partial norms < 1e-12 at the
+  exact sketch or panel decision that diverged, which it prints);
 * ||AP - QR||/||A|| and ||Q'Q - I|| <= 50 n eps;
 * |diag R| within 1e-10 relative of the oracle's wherever |R_kk| >= 1e-5 |R_00|
   (two correct LAPACKs already differ beyond that range on graded spectra).
@@ -20,19 +22,27 @@ torch = pytest.importorskip("torch")
 EPS = np.finfo(np.float64).eps
 
 
-def run_gpu(handle, A, nb, pp, pv, **kw):
+def run_gpu(handle, A, nb, pp, pv, sketch_order=False, **kw):
+    """Factor A on the device; with sketch_order, also returns the library's
+    trace of each block's sketch choices (hqrrp_trace.sketch_cols)."""
     import hqrrp_amd
 
     m, n = A.shape
     dA = hqrrp_amd.to_colmajor(A)
     jp = torch.arange(1, n + 1, dtype=torch.int32, device="cuda")
     tau = torch.zeros(max(1, min(m, n)), dtype=torch.float64, device="cuda")
+    sc = torch.zeros(max(1, min(m, n)), dtype=torch.int32, device="cuda")
+    if sketch_order:
+        kw["trace"] = {"sketch_cols": sc}
     hqrrp_amd.hqrrp_device(handle, dA, jp, tau, m, n, nb=nb, pp=pp,
                            panel_pivoting=pv, **kw)
     handle.synchronize()
     torch.cuda.synchronize()
-    return (hqrrp_amd.from_colmajor(dA), jp.cpu().numpy(),
-            tau.cpu().numpy()[:min(m, n)])
+    out = (hqrrp_amd.from_colmajor(dA), jp.cpu().numpy(),
+           tau.cpu().numpy()[:min(m, n)])
+    if sketch_order:
+        return out + (sc.cpu().numpy()[:min(m, n)],)
+    return out
 
 
 CASES = [
@@ -43,9 +53,28 @@ CASES = [
     (800, 1200, 64, 10, 1, "gaussian", 4),
     (1000, 1000, 32, 8, 1, "graded", 5),
     (1000, 700, 16, 8, 0, "gaussian", 6),
-    (3000, 3000, 64, 10, 1, "graded", 7),
+    (3000, 3000, 64, 10, 1, "graded", 7),  # a near-tie at 2820 (TIES)
     (513, 257, 64, 10, 1, "lowrank", 8),
 ]
+
+
+# Cases whose pivots first diverge from the oracle's at a near-tie (the
+# oracle's gap at that sketch decision is 3e-16 of the sketch's scale, deep
+# in the graded spectrum): the divergence is accepted there and nowhere else.
+TIES = {(3000, 3000, "graded", 7)}
+
+
+def sketched_columns(m, n, nb, kmax=None):
+    """Columns chosen by a sketch: those of every block but the reference's
+    final one (j + nb >= m or j + nb >= n, NoFLA_HQRRP_WY_blk_var4.c:
+    413-416), up to kmax."""
+    k = min(m, n) if kmax is None else kmax
+    tot = 0
+    for j in range(0, k, nb):
+        if j + nb >= m or j + nb >= n:
+            break
+        tot = min(j + nb, k)
+    return tot
 
 
 def make(family, m, n, seed):
@@ -60,10 +89,16 @@ def make(family, m, n, seed):
 def test_factorization_parity(handle, m, n, nb, pp, pv, family, seed):
     print(f"factor {family} {m}x{n} nb={nb} pp={pp} pv={pv}", flush=True)
     A = make(family, m, n, seed)
-    Ar, jr, tr, ts, tp = oracle.hqrrp(A, nb, pp, pv, trace=True)
-    Ag, jg, tg = run_gpu(handle, A, nb, pp, pv)
+    Ar, jr, tr, ts, tp, sr = oracle.hqrrp(A, nb, pp, pv, trace=True)
+    Ag, jg, tg, sg = run_gpu(handle, A, nb, pp, pv, sketch_order=True)
     k = min(m, n)
-    div = check_pivots(jg, jr, ts, tp, nb)
+    div = check_pivots(jg, jr, ts, tp, nb, sketch_gpu=sg, sketch_ref=sr,
+                       panel_pivoting=pv, label=f"{family} {m}x{n}")
+    if div is None:
+        # Every sketched block (all but the final one) chose the oracle's
+        # columns in the oracle's order.
+        nsk = sketched_columns(m, n, nb)
+        assert np.array_equal(sg[:nsk], sr[:nsk])
     res, orth = oracle.residuals(A, Ag, jg, tg)
     assert res <= 50 * max(m, n) * EPS, res
     assert orth <= 50 * max(m, n) * EPS, orth
@@ -74,7 +109,8 @@ def test_factorization_parity(handle, m, n, nb, pp, pv, family, seed):
     if upto:
         rel = np.abs(dg - dr)[keep] / dr[keep]
         assert rel.max() <= 1e-10, rel.max()
-    assert div is None or div > 0
+    assert div is None or (m, n, family, seed) in TIES, \
+        f"pivots diverge at {div}"
 
 
 def test_host_entry_matches_device(handle):

@@ -219,3 +219,45 @@ def test_first_index_tie_rule():
     r = oracle.qrpmod(A, pivoting=1, num_stages=1)
     c = int(np.argmax(np.linalg.norm(base, axis=0)))
     assert r["p"][0] == c + 1  # the first of the two equal columns
+
+
+# ------------------------------------------------------ truncation (oracle)
+def test_truncation_prefix_equals_full_run():
+    """Cut at a multiple of nb: the first k columns, their pivots and tau
+    are those of the full factorisation (later blocks never touch them)."""
+    A = matgen.gaussian(700, 600, 51)
+    Af, jf, tf = oracle.hqrrp(A, 32, 8, 1)
+    At, jt, tt = oracle.hqrrp(A, 32, 8, 1, max_cols=160)
+    assert np.array_equal(jt[:160], jf[:160])
+    assert np.array_equal(tt, np.concatenate([tf[:160], np.zeros(440)]))
+    # columns 0..159 are bitwise those of the full run up to the later
+    # blocks' row swaps, which never reach rows 0..159 of those columns
+    assert np.array_equal(At[:, :160], Af[:, :160])
+
+
+@pytest.mark.parametrize("kmax", [100, 33, 1])
+def test_truncation_cut_block_is_a_partial_qr(kmax):
+    """Cut inside a block: A P = Q [R11 R12; 0 A22] with Q the product of
+    the first k reflectors (a valid partial QR), the cut block's sketch
+    takes the first k - j choices of the uncut sketch (greedy pivoting is
+    prefix-consistent), and a planted mistake (one reflector dropped)
+    fails the check."""
+    m, n, nb = 500, 400, 32
+    A = matgen.gaussian(m, n, 52)
+    At, jt, tt, _, _, st = oracle.hqrrp(A, nb, 8, 1, max_cols=kmax,
+                                        trace=True)
+    _, _, _, _, _, sf = oracle.hqrrp(A, nb, 8, 1, trace=True)
+    assert np.array_equal(st[:kmax], sf[:kmax])
+
+    def resid(taus):
+        Q = oracle.form_q(At, taus, ncols=m)
+        M = At.copy()
+        for c in range(kmax):
+            M[c + 1:, c] = 0.0
+        AP = A[:, jt - 1]
+        return np.linalg.norm(AP - Q @ M) / np.linalg.norm(A)
+
+    assert resid(tt[:kmax]) <= 50 * m * EPS
+    bad = tt[:kmax].copy()
+    bad[kmax - 1] = 0.0
+    assert resid(bad) > 1e-6
