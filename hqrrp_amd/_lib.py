@@ -79,6 +79,8 @@ SIGNATURES = {
     "hqrrp_geqrf_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp]),
     "hqrrp_geqp3_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp,
                                  _vp]),
+    "hqrrp_geqp3_noalloc_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp,
+                                         C.c_int, _vp, _vp]),
     "hqrrp_orgqr_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
                                   C.c_int, _vp, _vp, C.c_int]),
     "hqrrp_qr_fixed_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,

@@ -102,6 +102,34 @@ struct Xchg {
   bool dirty = false;  // a launch aborted: both buffers are refilled
 };
 
+// Workspace of the allocation-free classic QRCP (hqrrp_geqp3_noalloc_dev):
+// a file-scope device array, one copy per GPU, held until the process ends.
+// Host code never uses its name as an address: hqrrp_create looks up this
+// device's copy with hipGetSymbolAddress and the handle keeps that pointer.
+constexpr size_t kScratchBytes = HQRRP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];  // per device, from hipGetSymbolAddress
+
+// Device address of this device's scratch (the device must be current).
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= 64) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "hqrrp: hipGetSymbolAddress(scratch) failed: %s\n",
+              hipGetErrorString(e));
+      return HQRRP_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return HQRRP_OK;
+}
+
 }  // namespace
 
 extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
@@ -180,6 +208,7 @@ struct hqrrp_handle_s {
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
   DevBuf vn;              // classic QRCP: partial and reference norms
+  void* scratch = nullptr;  // this device's allocation-free workspace
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
   // Batched path: block size per 64 x 64 shape bucket chosen by
@@ -1039,16 +1068,29 @@ int run_geqrf(hqrrp_handle_t h, int m, int n, int kmax, double* dA, long lda,
 // norms of rows k0.., then one pivot + update per column.  Swaps move whole
 // columns (all m rows) and jpvt entries, as LAPACK's dswap does, so rows
 // above k0 follow the pivots with no re-pivot pass.
+// Workspace bytes of run_classic_qrcp for an m x n matrix.
+size_t classic_ws_bytes(int m, int n) {
+  if (classic_fused_ok(m, n))
+    return sizeof(double) * (2 * (size_t)m * n + 4 * (size_t)n) +
+           sizeof(int) * 2 * (size_t)n;
+  return sizeof(double) * 2 * (size_t)n;
+}
+
+// ws: caller-provided workspace of classic_ws_bytes(m, n) bytes, or null
+// for the handle's own (grown on demand).
 int run_classic_qrcp(hqrrp_handle_t h, int m, int n, int k0, double* dA,
-                     long lda, int* djpvt, double* dtau) {
+                     long lda, int* djpvt, double* dtau,
+                     double* ws = nullptr) {
   const int mn = std::min(m, n);
   if (k0 >= mn) return HQRRP_OK;
+  if (!ws) {
+    HQ_RET(ensure(h->vn, classic_ws_bytes(m, n)));
+    ws = h->vn.as<double>();
+  }
   if (classic_fused_ok(m, n)) {
     // One launch per step, ping-ponging X <-> Y (matrix, jpvt, norms).
     const size_t mn2 = (size_t)m * n;
-    HQ_RET(ensure(h->vn, sizeof(double) * (2 * mn2 + 4 * (size_t)n) +
-                             sizeof(int) * 2 * (size_t)n));
-    double* X = h->vn.as<double>();
+    double* X = ws;
     double* Y = X + mn2;
     double* vX = Y + mn2;
     double* vY = vX + 2 * (size_t)n;
@@ -1073,8 +1115,7 @@ int run_classic_qrcp(hqrrp_handle_t h, int m, int n, int k0, double* dA,
     }
     return HQRRP_OK;
   }
-  HQ_RET(ensure(h->vn, sizeof(double) * 2 * (size_t)n));
-  double* vn1 = h->vn.as<double>();
+  double* vn1 = ws;
   double* vn2 = vn1 + n;
   HQ_TRY(launch_classic_norms(dA, lda, m, n, k0, vn1, vn2, h->cur));
   for (int i = k0; i < mn; ++i)
@@ -1277,6 +1318,10 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
   int rc = ensure(h->words, 64);
   if (rc) return rc;
   HQ_TRY(hipMemset(h->words.p, 0, h->words.bytes));
+  // The allocation-free entry's workspace on this device (looked up once
+  // per device, with the device current).
+  rc = scratch_address(device, &h->scratch);
+  if (rc) return rc;
   *out = h;
   return HQRRP_OK;
 }
@@ -1460,6 +1505,26 @@ extern "C" int hqrrp_geqp3_dev(hqrrp_handle_t h, int m, int n, double* dA,
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
   return run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau);
+}
+
+extern "C" int hqrrp_geqp3_noalloc_dev(hqrrp_handle_t h, int m, int n,
+                                       double* dA, int lda, int* djpvt,
+                                       double* dtau) {
+  if (!h || m < 0 || n < 0 || lda < std::max(1, m)) return HQRRP_ERR_ARG;
+  if (std::min(m, n) == 0) return HQRRP_OK;
+  const size_t need = classic_ws_bytes(m, n);
+  if (need > kScratchBytes) {
+    fprintf(stderr,
+            "hqrrp_geqp3_noalloc_dev: a %d x %d matrix needs %zu bytes of "
+            "workspace, more than the %zu of the static scratch\n",
+            m, n, need, kScratchBytes);
+    return HQRRP_ERR_UNSUPPORTED;
+  }
+  if (!h->scratch) return HQRRP_ERR_HIP;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau,
+                          static_cast<double*>(h->scratch));
 }
 
 extern "C" int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k,

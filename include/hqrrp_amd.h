@@ -178,6 +178,22 @@ int hqrrp_geqrf_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
 int hqrrp_geqp3_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
                     int *djpvt, double *dtau);
 
+/* Allocation-free variant of hqrrp_geqp3_dev (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  Same
+ * arguments, kernels and results (bitwise) as hqrrp_geqp3_dev, but the
+ * workspace is a static device array of HQRRP_NOALLOC_SCRATCH_BYTES per GPU
+ * (held from the first use of the library on that GPU until the process
+ * ends) instead of handle-owned memory; the call performs no allocation.
+ * Shapes whose workspace exceeds it -- 16 (m n + 2 n) + 8 n bytes for
+ * m <= 1024 and n <= 2048, else 16 n bytes -- return HQRRP_ERR_UNSUPPORTED
+ * with a message on stderr.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (other streams, other handles, other threads)
+ * race; calls on one stream are ordered. */
+#define HQRRP_NOALLOC_SCRATCH_BYTES ((size_t)34 << 20)
+int hqrrp_geqp3_noalloc_dev(hqrrp_handle_t h, int m, int n, double *dA,
+                            int lda, int *djpvt, double *dtau);
+
 /* Q (m x n) = the first n columns of H_1 ... H_k, with the reflectors in the
  * first k columns of A below the diagonal and tau (LAPACK dorgqr semantics,
  * written out of place to Q); k <= n <= m.  Replaces the dorgqr_ call of
