@@ -204,6 +204,8 @@ struct hqrrp_handle_s {
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
   DevBuf Pn;  // look-ahead: the next panel
+  DevBuf gen, genT, genM;  // generic block step: sketch copy / norms, T
+                           // products, trailing products
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
   DevBuf adjAP, adjTau, adjK, adjX;  // fixed-pivot QR and its adjoint
   DevBuf hA, hJ, hT, hW;  // device copies used by the host entry points
@@ -425,12 +427,134 @@ int gemm32(hqrrp_handle_t h, bool ta, bool tb, int M, int N, int K,
 }
 
 // The mixed-precision sketch Y = Omega A (r x n, FP32) from G' = Omega'
-// (m x r, FP64) and A (m x n, ld lda, FP64).
+// (m x r, FP64, already rounded to the sketch precision's values) and A
+// (m x n, ld lda, FP64): on the matrix cores (k_mixed.hip), split K so the
+// grid fills the CUs twice; beyond the kernel's r, an FP32 VALU product
+// of the same (rounded) operator.
 int sketch_gemm_mixed(hqrrp_handle_t h, int r, int m, int n, const double* Gt,
                       const double* dA, long lda, float* Y) {
-  return gemm32(h, true, false, r, n, m, 1.0f, Gt, m, 1, 0, dA, lda, 1, 0,
-                0.0f, Y, r, h->skbf);
+  if (r > mixed_sketch_rmax())
+    return gemm32(h, true, false, r, n, m, 1.0f, Gt, m, 1, 0, dA, lda, 1, 0,
+                  0.0f, Y, r, 0);
+  MixedSketchArgs a{};
+  a.G = Gt;
+  a.ldg = m;
+  a.A = dA;
+  a.lda = lda;
+  a.m = m;
+  a.n = n;
+  a.r = r;
+  a.Y = Y;
+  a.ldy = r;
+  const long tiles = hq_cdiv(n, mixed_sketch_cols_per_wg());
+  const long kb = mixed_sketch_kbk();
+  long S = std::max(1L, std::min((2L * h->cu_count + tiles - 1) / tiles,
+                                 (long)m / (4 * kb)));
+  long kc = (m + S - 1) / S;
+  kc = (kc + kb - 1) / kb * kb;
+  S = std::max(1L, (m + kc - 1) / kc);
+  a.kchunk = kc;
+  const int prec = h->skbf ? 2 : 1;
+  if (S == 1) {
+    HQ_TRY(launch_sketch_mixed(a, prec, 1, h->cur));
+    return HQRRP_OK;
+  }
+  HQ_RET(ensure(h->split, sizeof(float) * (size_t)S * r * n));
+  a.part = h->split.as<float>();
+  HQ_TRY(launch_sketch_mixed(a, prec, (int)S, h->cur));
+  HQ_TRY(launch_reduce32(r, n, (int)S, a.part, 1.0f, 0.0f, Y, r, h->cur));
+  return HQRRP_OK;
 }
+
+// T (b x b, ld ldt) of b reflectors from their inner products (gram:
+// strictly upper part, ld ldg) and tau, for any b: larft on 64-column
+// diagonal blocks, then the off-diagonal blocks by doubling,
+//   T(B1, B2) = -T(B1, B1) G(B1, B2) T(B2, B2)  (dlarft's recurrence in
+// block form; the same matrix in exact arithmetic).
+int larft_any(hqrrp_handle_t h, int b, const double* gram, long ldg,
+              const double* tau, double* T, long ldt) {
+  if (b <= 64) {
+    HQ_TRY(launch_larft(b, gram, ldg, tau, T, ldt, h->cur));
+    return HQRRP_OK;
+  }
+  HQ_TRY(launch_fill_eye(T, ldt, b, b, 0.0, h->cur));
+  for (int q0 = 0; q0 < b; q0 += 64)
+    HQ_TRY(launch_larft(std::min(64, b - q0), gram + q0 + (long)q0 * ldg, ldg,
+                        tau + q0, T + q0 + (long)q0 * ldt, ldt, h->cur));
+  HQ_RET(ensure(h->genT, sizeof(double) * (size_t)b * b));
+  double* X = h->genT.as<double>();
+  for (int sz = 64; sz < b; sz *= 2) {
+    for (int c0 = 0; c0 + sz < b; c0 += 2 * sz) {
+      const int c1 = c0 + sz, s2 = std::min(sz, b - c1);
+      HQ_RET(gemm(h, false, false, sz, s2, s2, 1.0, gram + c0 + (long)c1 * ldg,
+                  ldg, 0, T + c1 + (long)c1 * ldt, ldt, 0, 0.0, X, sz, false));
+      HQ_RET(gemm(h, false, false, sz, s2, sz, -1.0, T + c0 + (long)c0 * ldt,
+                  ldt, 0, X, sz, 0, 0.0, T + c0 + (long)c1 * ldt, ldt, false));
+    }
+  }
+  return HQRRP_OK;
+}
+
+// Greedy QRCP for `stages` steps on a copy of the sketch (r x ncols)
+// without the persistent kernel's limits (r, columns, stages): the
+// reference's NoFLA_QRPmod_WY_unb_var4 on V_R = Y_R (NoFLA_HQRRP_WY_blk_var4
+// .c:500-506), two launches per step, then the composed moves.
+int run_sketch_generic(hqrrp_handle_t h, int r, int ncols, const void* dY,
+                       long ldy, int stages, int* dpiv, int* dmoves, int f32) {
+  const size_t rc = (size_t)r * ncols;
+  HQ_RET(ensure(h->gen, sizeof(double) * (rc + 2 * (size_t)ncols + stages) +
+                            sizeof(int) * (size_t)ncols));
+  double* V = h->gen.as<double>();
+  double* vn1 = V + rc;
+  double* vn2 = vn1 + ncols;
+  double* tw = vn2 + ncols;
+  int* pos = reinterpret_cast<int*>(tw + stages);
+  if (f32)
+    HQ_TRY(launch_widen_block(V, r, static_cast<const float*>(dY), ldy, r,
+                              ncols, h->cur));
+  else
+    HQ_TRY(launch_copy_block(V, r, static_cast<const double*>(dY), ldy, r,
+                             ncols, h->cur));
+  HQ_TRY(launch_iota(pos, ncols, h->cur));
+  HQ_TRY(launch_classic_norms(V, r, r, ncols, 0, vn1, vn2, h->cur));
+  for (int k = 0; k < stages; ++k)
+    HQ_TRY(launch_qrpmod_step(V, r, r, ncols, k, 1, pos, dpiv, tw, vn1, vn2,
+                              h->cur));
+  HQ_TRY(launch_perm_moves(pos, ncols, dmoves, h->cur));
+  return HQRRP_OK;
+}
+
+// Householder QR with optional pivoting of an mrows x b panel without the
+// persistent kernel's limits (b, rows): NoFLA_QRPmod_WY_unb_var4 with
+// num_stages = -1 and build_T (NoFLA_HQRRP_WY_blk_var4.c:520-524), two
+// launches per step; the reflectors' inner products by one product, T by
+// larft_any.
+int run_panel_generic(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
+                      int pivoting, double* dtau, int* dperm, double* dT,
+                      long ldt) {
+  HQ_RET(ensure(h->gen, sizeof(double) * 2 * (size_t)b));
+  HQ_RET(ensure(h->gram, sizeof(double) * (size_t)b * b));
+  double* vn1 = h->gen.as<double>();
+  double* vn2 = vn1 + b;
+  HQ_TRY(launch_iota(dperm, b, h->cur));
+  if (pivoting) HQ_TRY(launch_classic_norms(dP, ldp, mrows, b, 0, vn1, vn2,
+                                            h->cur));
+  {
+    PhaseScope ps(h, PH_PANEL, 2.0 * mrows * b * b - 2.0 / 3.0 * b * b * b,
+                  16.0 * mrows * b, 2 * b);
+    for (int k = 0; k < std::min(b, mrows); ++k)
+      HQ_TRY(launch_qrpmod_step(dP, ldp, mrows, b, k, pivoting, dperm, nullptr,
+                                dtau, vn1, vn2, h->cur));
+  }
+  double* gram = h->gram.as<double>();
+  HQ_RET(gemm(h, true, false, b, b, mrows, 1.0, dP, ldp, 1, dP, ldp, 1, 0.0,
+              gram, b));
+  if (!dT) return HQRRP_OK;
+  PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
+  return larft_any(h, b, gram, b, dtau, dT, ldt);
+}
+
+int panel_geometry(hqrrp_handle_t h, int mrows, int* nw);
 
 int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
   int n = hq_cdiv(mrows, panel_rows_per_wg());
@@ -442,7 +566,9 @@ int panel_geometry(hqrrp_handle_t h, int mrows, int* nw) {
 int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
                int stages, int* dpiv, int* dmoves, int f32 = 0) {
   int rmax = sketch_rmax(r);
-  if (rmax < 0) return HQRRP_ERR_UNSUPPORTED;
+  if (rmax < 0 || stages > sketch_max_stages() ||
+      hq_cdiv(ncols, sketch_cols_per_wg()) > h->sketch_max_wg)
+    return run_sketch_generic(h, r, ncols, dY, ldy, stages, dpiv, dmoves, f32);
   SketchArgs a{};
   a.Y = dY;
   a.ldy = ldy;
@@ -469,7 +595,14 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
 int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
               int pivoting, double* dtau, int* dperm, double* dT, long ldt,
               const PanelArgs* sel = nullptr) {
-  if (b < 1 || b > 64 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
+  if (b < 1 || mrows < b) return HQRRP_ERR_UNSUPPORTED;
+  int nw_fit = 0;
+  if (b > 64 || panel_geometry(h, mrows, &nw_fit) != HQRRP_OK) {
+    // The look-ahead only runs where the persistent kernels fit.
+    if (sel) return HQRRP_ERR_UNSUPPORTED;
+    return run_panel_generic(h, mrows, b, dP, ldp, pivoting, dtau, dperm, dT,
+                             ldt);
+  }
   PanelArgs a{};
   if (sel) {
     a.sel_moves = sel->sel_moves;
@@ -505,7 +638,7 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
   xchg_end(h->xp, need);
   if (!dT) return HQRRP_OK;
   PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
-  HQ_TRY(launch_larft(b, a.gram, dtau, dT, ldt, h->cur));
+  HQ_TRY(launch_larft(b, a.gram, b, dtau, dT, ldt, h->cur));
   return HQRRP_OK;
 }
 
@@ -695,13 +828,60 @@ int block_gather(hqrrp_handle_t h, int r, int j, int m, int b, double* dA,
   return HQRRP_OK;
 }
 
-// Rows 0..j-1 of the panel columns and jpvt follow the panel's pivots.
+// Rows 0..j-1 of the panel columns and jpvt follow the panel's pivots (for
+// panels beyond 64 columns: as the permutation's moves, through the column
+// gathers).
 int block_permute(hqrrp_handle_t h, int j, int b, double* dA, long lda,
                   int* djpvt) {
   PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * j * b, 1);
-  HQ_TRY(launch_panel_permute(dA + (long)j * lda, lda, j, b, h->perm.as<int>(),
-                              djpvt + j, h->cur));
+  if (b <= 64) {
+    HQ_TRY(launch_panel_permute(dA + (long)j * lda, lda, j, b,
+                                h->perm.as<int>(), djpvt + j, h->cur));
+    return HQRRP_OK;
+  }
+  int* mv = h->moves.as<int>();  // free between the gathers and the next sketch
+  HQ_TRY(launch_perm_moves(h->perm.as<int>(), b, mv, h->cur));
+  HQ_TRY(launch_gather_cols(mv, b, dA + (long)j * lda, lda, j, nullptr, 0, 0,
+                            8, djpvt + j, h->tmpA.as<double>(),
+                            h->tmpY.as<double>(), h->tmpJ.as<int>(), h->cur));
   return HQRRP_OK;
+}
+
+// Block tail and trailing update with generic FP64 products, for blocks
+// beyond the fused kernels (b > 64 reflectors or r > 80 sketch rows):
+// dlarfb (NoFLA_HQRRP_WY_blk_var4.c:529-541, 711-740) on the trailing
+// matrix C (mrows x nc, rows 0..b-1 become R12) and, with Gt, the sketch
+// downdate as in block_tail: G' -= V (T'(V'G')) on G' rows j.., whose first
+// b rows are then the reference's B (617-686), and Y2 -= B R12 (691-693).
+int block_trail_generic(hqrrp_handle_t h, int j, int jn, int mrows, int nc,
+                        int b, const double* V, long ldv, const double* T,
+                        long ldt, double* C, long ldc, double* Gt, int m,
+                        int r, void* Y) {
+  const int ncg = Gt ? r : 0;
+  HQ_RET(ensure(h->genM, sizeof(double) * 2 * (size_t)b * (nc + ncg)));
+  double* M = h->genM.as<double>();          // b x (nc + ncg): V'[C G']
+  double* W = M + (size_t)b * (nc + ncg);    // b x (nc + ncg): T'M
+  PhaseScope ps(h, PH_TRAIL_VTC, 4.0 * b * (double)mrows * (nc + ncg),
+                16.0 * (double)mrows * (nc + ncg), 6);
+  HQ_RET(gemm(h, true, false, b, nc, mrows, 1.0, V, ldv, 1, C, ldc, 0, 0.0, M,
+              b));
+  if (Gt)
+    HQ_RET(gemm(h, true, false, b, r, mrows, 1.0, V, ldv, 1, Gt + j, m, 0, 0.0,
+                M + (size_t)b * nc, b));
+  HQ_RET(gemm(h, true, false, b, nc + ncg, b, 1.0, T, ldt, 0, M, b, 0, 0.0, W,
+              b, false));
+  HQ_RET(gemm(h, false, false, mrows, nc, b, -1.0, V, ldv, 1, W, b, 0, 1.0, C,
+              ldc));
+  if (!Gt) return HQRRP_OK;
+  HQ_RET(gemm(h, false, false, mrows, r, b, -1.0, V, ldv, 1,
+              W + (size_t)b * nc, b, 0, 1.0, Gt + j, m));
+  const double* R12 = C;  // rows 0..b-1 of the updated trailing matrix
+  if (h->sk32)
+    return gemm32(h, true, false, r, nc, b, -1.0f, Gt + j, m, 1, 0, R12, ldc,
+                  1, 0, 1.0f, static_cast<float*>(Y) + (long)jn * r, r, 0,
+                  false);
+  return gemm(h, true, false, r, nc, b, -1.0, Gt + j, m, 0, R12, ldc, 0, 1.0,
+              static_cast<double*>(Y) + (long)jn * r, r, false);
 }
 
 
@@ -791,14 +971,12 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
 
 int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
                   int* djpvt, double* dtau, const hqrrp_options& o) {
-  const int nb = o.nb, pp = o.pp, r = nb + pp;
   const int mn = std::min(m, n);
   h->cur = h->stream;
   if (mn == 0) return HQRRP_OK;
   const int kmax = (o.max_rank > 0) ? std::min(mn, o.max_rank) : mn;
   if (o.sketch_prec < HQRRP_SKETCH_FP64 || o.sketch_prec > HQRRP_SKETCH_BF16)
     return HQRRP_ERR_ARG;
-  if (sketch_rmax(r) < 0 || nb > 64) return HQRRP_ERR_UNSUPPORTED;
 
   HQ_TRY(hipSetDevice(h->device));
   h->sk32 = o.sketch_prec != HQRRP_SKETCH_FP64;
@@ -814,13 +992,18 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   HQ_RET(ensure(h->Y, sizeof(double) * (size_t)r * n));
   HQ_RET(ensure(h->moves, sizeof(int) * (size_t)(1 + 4 * nb)));
   HQ_RET(ensure(h->piv, sizeof(int) * (size_t)nb));
-  HQ_RET(ensure(h->perm, sizeof(int) * 64));
+  HQ_RET(ensure(h->perm, sizeof(int) * (size_t)std::max(64, nb)));
+  HQ_RET(ensure(h->gram, sizeof(double) * (size_t)std::max(64, nb) *
+                             std::max(64, nb)));
   HQ_RET(ensure(h->T, sizeof(double) * (size_t)nb * nb));
   HQ_RET(ensure(h->tmpA, sizeof(double) * (size_t)2 * nb * m));
   HQ_RET(ensure(h->tmpY, sizeof(double) * (size_t)2 * nb * r));
   HQ_RET(ensure(h->tmpJ, sizeof(int) * (size_t)2 * nb));
+  // Blocks beyond the fused kernels (more than 64 reflectors or 80 sketch
+  // rows) run the generic block step, which has no look-ahead.
+  const bool gen_trail = nb > 64 || r > 80;
   const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd &&
-                      kmax > nb && n > nb;
+                      kmax > nb && n > nb && !gen_trail;
   if (use_la) {
     HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
     HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
@@ -834,6 +1017,12 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   const long ldx[2] = {lda, (long)m};
   int xc = 0;
 
+  // The mixed sketch's operator: Omega' rounded to the sketch precision's
+  // values once, so that the FP64 downdate below is that of exactly the
+  // operator the mixed product uses.
+  if (h->sk32)
+    HQ_TRY(launch_round_omega(static_cast<double*>(G), (long)m * r,
+                              h->skbf ? 2 : 1, h->cur));
   // Y = Omega * A (NoFLA_HQRRP_WY_blk_var4.c:407-409).
   {
     PhaseScope ps(h, PH_SKETCH_GEMM, 2.0 * r * (double)m * n,
@@ -922,9 +1111,14 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       xc = 0;
     }
     double* C = X[xc] + j + (long)jn * ldx[xc];
-    TrailArgs ta;
     // With a next block to pivot, G' rows j.. ride along.
     const bool gcols = !last;
+    if (gen_trail) {
+      HQ_RET(block_trail_generic(h, j, jn, mrows, nc, b, Ajj, lda, T, nb, C,
+                                 ldx[xc], gcols ? Gt : nullptr, m, r, Y));
+      continue;
+    }
+    TrailArgs ta;
     HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta,
                          gcols ? Gt + j : nullptr, m, gcols ? r : 0));
     HQ_RET(trail_vtc_only(h, ta));
@@ -1005,7 +1199,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
       {
         PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
-        HQ_TRY(launch_larft(b2, h->gram.as<double>(), dtau + jn, T, nb,
+        HQ_TRY(launch_larft(b2, h->gram.as<double>(), b2, dtau + jn, T, nb,
                             h->cur));
       }
       HQ_TRY(hipEventRecord(h->ev_t, h->s_chain));
@@ -1141,7 +1335,7 @@ int run_orgqr(hqrrp_handle_t h, int m, int n, int k, const double* dA,
     double* gram = h->gram.as<double>();
     HQ_RET(gemm(h, true, false, b, b, mrows, 1.0, V, lda, 1, V, lda, 1, 0.0,
                 gram, b));
-    HQ_TRY(launch_larft(b, gram, dtau + j, h->T.as<double>(), nb, h->cur));
+    HQ_TRY(launch_larft(b, gram, b, dtau + j, h->T.as<double>(), nb, h->cur));
     TrailArgs ta;
     HQ_RET(trail_prepare(h, mrows, n - j, b, V, lda, h->T.as<double>(), nb,
                          dQ + j + (long)j * ldq, ldq, &ta));
@@ -1336,7 +1530,8 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
                     &h->hA,    &h->hJ,    &h->hT,     &h->hW,    &h->vn,
-                    &h->Pn,    &h->Xb,     &h->adjAP,
+                    &h->Pn,    &h->Xb,     &h->adjAP, &h->gen,
+                    &h->genT,  &h->genM,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);

@@ -57,19 +57,22 @@ __global__ __launch_bounds__(HQ_NT) void classic_norms_kernel(
 // vn2(pvt) = vn2(i); then the reflector of A(i:m, i): beta into A(i, i), the
 // scaled vector into A(i+1:m, i), tau(i).  Each thread reads back only the
 // rows it wrote itself; A(i, i) and the sum of squares go through LDS.
+// pivoting = 0: no choice (p = i).  piv (optional): piv[i] = p.
 __global__ __launch_bounds__(HQ_NT) void classic_pivot_kernel(
     double* A, long lda, int m, int n, int i, int* jpvt, double* tau,
-    double* vn1, double* vn2) {
+    double* vn1, double* vn2, int pivoting, int* piv) {
   __shared__ double s_key[kWaves], s_sum[kWaves], s_alpha;
   __shared__ int s_pos[kWaves];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double key = -1.0;
   int pos = 0x7fffffff;
-  for (int j = i + t; j < n; j += HQ_NT) {
-    const double v = vn1[j];
-    if (v > key) {  // increasing j: the first of equal maxima stays
-      key = v;
-      pos = j;
+  if (pivoting) {
+    for (int j = i + t; j < n; j += HQ_NT) {
+      const double v = vn1[j];
+      if (v > key) {  // increasing j: the first of equal maxima stays
+        key = v;
+        pos = j;
+      }
     }
   }
   hq_wave_argmax64(key, pos);
@@ -85,6 +88,7 @@ __global__ __launch_bounds__(HQ_NT) void classic_pivot_kernel(
   // In [i, n) since vn1 >= 0 > the initial key; a NaN norm (NaN in A) is
   // never taken, and an all-NaN range keeps column i.
   const int p = (pos < n) ? pos : i;
+  if (piv && t == 0) piv[i] = p;
   double* ci = A + (long)i * lda;
   if (p != i) {
     double* cp = A + (long)p * lda;
@@ -124,10 +128,15 @@ __global__ __launch_bounds__(HQ_NT) void classic_pivot_kernel(
 }
 
 // Step i, part 2 (dgeqpf's DLARF and DO 30): columns j in (i, n), one wave
-// each.  With tau = 0 the column is left untouched, as dlarf does.
+// each.  With tau = 0 the column is left untouched, as dlarf does.  REF:
+// the downdate of the reference's NoFLA_QRP_downdate_partial_norms
+// (NoFLA_HQRRP_WY_blk_var4.c:925-951: t = max(0, (1 + t)(1 - t))) instead
+// of LAPACK dlaqp2's 1 - t^2; the two differ in rounding only.  pivoting = 0
+// skips the norms.
+template <bool REF>
 __global__ __launch_bounds__(HQ_NT) void classic_update_kernel(
     double* A, long lda, int m, int n, int i, const double* tau, double* vn1,
-    double* vn2) {
+    double* vn2, int pivoting) {
   const int lane = threadIdx.x & 63;
   const long j = (long)i + 1 + (long)blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (j >= n) return;  // wave-uniform
@@ -151,11 +160,11 @@ __global__ __launch_bounds__(HQ_NT) void classic_update_kernel(
     for (int r = i + 1 + lane; r < m; r += 64) s2 = fma(c[r], c[r], s2);
   }
   s2 = hq_wave_sum(s2);
-  if (lane != 0) return;
+  if (lane != 0 || !pivoting) return;
   const double d = vn1[j];
   if (d == 0.0) return;
   const double q = fabs(aij) / d;
-  double temp = 1.0 - q * q;
+  double temp = REF ? (1.0 + q) * (1.0 - q) : 1.0 - q * q;
   temp = temp > 0.0 ? temp : 0.0;
   const double ratio = d / vn2[j];
   const double temp2 = temp * (ratio * ratio);
@@ -324,11 +333,26 @@ hipError_t launch_classic_step(double* A, long lda, int m, int n, int i,
                                double* vn2, hipStream_t s) {
   if (i < 0 || i >= m || i >= n) return hipErrorInvalidValue;
   hipLaunchKernelGGL(classic_pivot_kernel, dim3(1), dim3(HQ_NT), 0, s, A, lda,
-                     m, n, i, jpvt, tau, vn1, vn2);
+                     m, n, i, jpvt, tau, vn1, vn2, 1, nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || i + 1 >= n) return e;
-  hipLaunchKernelGGL(classic_update_kernel, dim3(hq_cdiv(n - i - 1, kWaves)),
-                     dim3(HQ_NT), 0, s, A, lda, m, n, i, tau, vn1, vn2);
+  hipLaunchKernelGGL(classic_update_kernel<false>,
+                     dim3(hq_cdiv(n - i - 1, kWaves)), dim3(HQ_NT), 0, s, A,
+                     lda, m, n, i, tau, vn1, vn2, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_qrpmod_step(double* A, long lda, int m, int n, int i,
+                              int pivoting, int* perm, int* piv, double* tau,
+                              double* vn1, double* vn2, hipStream_t s) {
+  if (i < 0 || i >= m || i >= n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(classic_pivot_kernel, dim3(1), dim3(HQ_NT), 0, s, A, lda,
+                     m, n, i, perm, tau, vn1, vn2, pivoting, piv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || i + 1 >= n) return e;
+  hipLaunchKernelGGL(classic_update_kernel<true>,
+                     dim3(hq_cdiv(n - i - 1, kWaves)), dim3(HQ_NT), 0, s, A,
+                     lda, m, n, i, tau, vn1, vn2, pivoting);
   return hipGetLastError();
 }
 

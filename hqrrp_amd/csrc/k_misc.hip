@@ -100,7 +100,82 @@ __global__ __launch_bounds__(HQ_NT) void transpose_kernel(
   }
 }
 
+__global__ __launch_bounds__(HQ_NT) void iota_kernel(int* p, int n) {
+  const int i = blockIdx.x * HQ_NT + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// One workgroup walks the positions in chunks of 1024, compacting the moved
+// ones in order: a ballot per wavefront, the wavefront offsets through LDS.
+constexpr int kPM = 1024;
+__global__ __launch_bounds__(kPM) void perm_moves_kernel(const int* perm,
+                                                         int n, int* moves) {
+  __shared__ int s_cnt[kPM / 64];
+  __shared__ int s_base;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) s_base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += kPM) {
+    const int pos = c0 + t;
+    const int src = pos < n ? perm[pos] : pos;
+    const bool mv = pos < n && src != pos;
+    const unsigned long long b = __ballot(mv);
+    if (lane == 0) s_cnt[w] = __popcll(b);
+    __syncthreads();
+    int off = s_base;
+    for (int q = 0; q < w; ++q) off += s_cnt[q];
+    if (mv) {
+      const int idx = off + __popcll(b & ((1ull << lane) - 1ull));
+      moves[1 + 2 * idx] = pos;
+      moves[2 + 2 * idx] = src;
+    }
+    __syncthreads();
+    if (t == 0) {
+      int tot = 0;
+      for (int q = 0; q < kPM / 64; ++q) tot += s_cnt[q];
+      s_base += tot;
+    }
+    __syncthreads();
+  }
+  if (t == 0) moves[0] = s_base;
+}
+
+__global__ __launch_bounds__(HQ_NT) void widen_block_kernel(
+    double* dst, long ldd, const float* src, long lds, int rows, int cols) {
+  const long i0 = (long)blockIdx.x * kRowsPerBlock;
+  for (long c = blockIdx.y; c < cols; c += gridDim.y) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      long i = i0 + q * HQ_NT + threadIdx.x;
+      if (i < rows) dst[i + c * ldd] = (double)src[i + c * lds];
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_iota(int* p, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(iota_kernel, dim3(hq_cdiv(n, HQ_NT)), dim3(HQ_NT), 0, s,
+                     p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_perm_moves(const int* perm, int n, int* moves,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(perm_moves_kernel, dim3(1), dim3(kPM), 0, s, perm, n,
+                     moves);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen_block(double* dst, long ldd, const float* src,
+                              long lds, int rows, int cols, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  dim3 grid(hq_cdiv(rows, kRowsPerBlock), cols < 32768 ? cols : 32768);
+  hipLaunchKernelGGL(widen_block_kernel, grid, dim3(HQ_NT), 0, s, dst, ldd,
+                     src, lds, rows, cols);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;

@@ -391,6 +391,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
 // triangular products instead of LAPACK's b-step recurrence (same matrix in
 // exact arithmetic; a zero tau gives a zero column, as LAPACK's).
 __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
+                                                      long ldg,
                                                       const double* tau,
                                                       double* T, long ldt) {
   __shared__ double sT[kMaxB][kMaxB + 1];
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
   for (int u = 0; u < kMaxB * kMaxB / HQ_NT; ++u) {
     const int idx = t + u * HQ_NT, i = idx & (kMaxB - 1), j = idx >> 6;
     const bool in = i < b && j < b;
-    sG[i][j] = (in && i < j) ? gram[i + j * b] : 0.0;
+    sG[i][j] = (in && i < j) ? gram[i + j * ldg] : 0.0;
     sT[i][j] = (in && i == j) ? tau[i] : 0.0;
   }
   __syncthreads();
@@ -519,10 +520,11 @@ size_t panel_xchg_doubles(int nw, int b) {
   return (size_t)(2 * b + 1) * nw * kXS;
 }
 
-hipError_t launch_larft(int b, const double* gram, const double* tau,
+hipError_t launch_larft(int b, const double* gram, long ldg, const double* tau,
                         double* T, long ldt, hipStream_t stream) {
+  if (b < 1 || b > kMaxB) return hipErrorInvalidValue;
   hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(HQ_NT), 0, stream, b, gram,
-                     tau, T, ldt);
+                     ldg, tau, T, ldt);
   return hipGetLastError();
 }
 

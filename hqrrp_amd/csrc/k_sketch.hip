@@ -341,6 +341,8 @@ size_t sketch_xchg_doubles(int nw, int rmax, int stages) {
 
 int sketch_cols_per_wg() { return kCols; }
 
+int sketch_max_stages() { return kMaxStages; }
+
 int sketch_rmax(int r) {
   if (r <= 32) return 32;
   if (r <= 80) return 80;

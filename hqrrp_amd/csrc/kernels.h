@@ -192,13 +192,16 @@ int trail_vtc_cols();  // columns of C per wide trail_vtc workgroup
 size_t sketch_xchg_doubles(int nw, int rmax, int stages);
 int sketch_rmax(int r);
 int sketch_cols_per_wg();
+int sketch_max_stages();
 hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
 
 // k_panel.hip
 hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t s);
 int panel_rows_per_wg();
 size_t panel_xchg_doubles(int nw, int b);
-hipError_t launch_larft(int b, const double* gram, const double* tau,
+// T (b x b, ld ldt) from the reflector inner products gram (strictly upper
+// part, ld ldg) and tau; b <= 64.
+hipError_t launch_larft(int b, const double* gram, long ldg, const double* tau,
                         double* T, long ldt, hipStream_t s);
 // Rows [0, nrows) of the b panel columns follow the panel's pivots (and
 // jpvt); with P, the same launch also copies the factored panel (prows x b,
@@ -220,6 +223,16 @@ hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s);
 // dst (cols x rows, ld ldd) = src' (src: rows x cols, ld lds).
 hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
                             double* dst, long ldd, hipStream_t s);
+// p[i] = i for i < n.
+hipError_t launch_iota(int* p, int n, hipStream_t s);
+// The moves of a permutation (perm[pos] = the position whose column ends at
+// pos): moves[0] = count, then (dst = pos, src = perm[pos]) for every pos
+// with perm[pos] != pos, in increasing pos (one workgroup).
+hipError_t launch_perm_moves(const int* perm, int n, int* moves,
+                             hipStream_t s);
+// dst (rows x cols, ld ldd, FP64) = src (FP32, ld lds).
+hipError_t launch_widen_block(double* dst, long ldd, const float* src,
+                              long lds, int rows, int cols, hipStream_t s);
 hipError_t launch_copy_block(double* dst, long ldd, const double* src,
                              long lds, int rows, int cols, hipStream_t s);
 // Moves m rows of A's columns (A may point at any row), r rows of Y's
@@ -253,6 +266,14 @@ hipError_t launch_classic_norms(const double* A, long lda, int m, int n,
 hipError_t launch_classic_step(double* A, long lda, int m, int n, int i,
                                int* jpvt, double* tau, double* vn1,
                                double* vn2, hipStream_t s);
+// One step of NoFLA_QRPmod_WY_unb_var4 (NoFLA_HQRRP_WY_blk_var4.c:820-872)
+// on A (m x n, ld lda) with the same two kernels: the first position of the
+// largest vn1 (pivoting = 1; else position i), a whole-column swap (perm,
+// vn1, vn2 follow; piv[i] = the chosen position when piv is set), dlarfg,
+// dlarf on the columns to the right and the reference's LAWN 176 downdate.
+hipError_t launch_qrpmod_step(double* A, long lda, int m, int n, int i,
+                              int pivoting, int* perm, int* piv, double* tau,
+                              double* vn1, double* vn2, hipStream_t s);
 // Small shapes: one launch per step, X -> Y ping-pong (m x n, ld ldx each;
 // jpvt jX -> jY; norms vX -> vY with vn1 at [0, n), vn2 at [n, 2n)).
 constexpr int kClassicMaxRows = 1024;
@@ -263,6 +284,30 @@ hipError_t launch_classic_fused(const double* X, double* Y, long ldx, int m,
                                 const double* vX, double* vY, double* A,
                                 long lda, int* jpvt, double* tau,
                                 hipStream_t s);
+
+// k_mixed.hip: the mixed-precision sketch Y (r x n, FP32, ld ldy) =
+// G'' A with G' (m x r, ld ldg) and A (m x n, ld lda) in FP64, on the
+// matrix cores (prec 1: FP32, 2: BF16 with A split in two terms).  With
+// part set, split s of K (rows [s kchunk, (s+1) kchunk), kchunk a multiple
+// of mixed_sketch_kbk()) writes slice s of part (r x n each, ld r).
+struct MixedSketchArgs {
+  const double* G;
+  long ldg;
+  const double* A;
+  long lda;
+  int m, n, r;
+  long kchunk;
+  float* part;
+  float* Y;
+  long ldy;
+};
+int mixed_sketch_rmax();
+int mixed_sketch_cols_per_wg();
+int mixed_sketch_kbk();
+hipError_t launch_sketch_mixed(const MixedSketchArgs& a, int prec, int splits,
+                               hipStream_t s);
+// x[0, n) rounded to the nearest FP32 (prec 1) or bf16 (prec 2) value.
+hipError_t launch_round_omega(double* x, long n, int prec, hipStream_t s);
 
 // k_gemm32.hip
 struct Gemm32Args {

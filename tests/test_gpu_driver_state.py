@@ -66,16 +66,31 @@ def effective_rank(m, n, nb, kmax):
     return k
 
 
-# Rounding bounds of Y = Omega A per column, relative to ||Omega||_2 ||A_c||:
-# a sum of m products in FP64 or FP32 (typically sqrt(m) eps; 8 sqrt(m) eps
-# allowed), and for BF16 operands the rounding of each operand to 8 bits
-# (2^-9 relative per entry; 2^-7 allowed).
+# Rounding bounds of Y = Omega A per column, relative to ||Omega||_2 ||A_c||,
+# where Omega is the operator the sketch uses (the reference's, rounded to
+# FP32 or bf16 values for a mixed sketch): a sum of m products in FP64 or
+# FP32 (typically sqrt(m) eps; 8 sqrt(m) eps allowed), and for the BF16
+# sketch A's two-term split (16 significant bits: 2^-17 relative per entry;
+# 2^-14 allowed).
 def _tol(prec, m):
     if prec == 0:
         return 8 * np.sqrt(m) * EPS
     if prec == 1:
         return 8 * np.sqrt(m) * 2.0 ** -24
-    return 2.0 ** -7
+    return 2.0 ** -14
+
+
+def _round_operator(G, prec):
+    """Omega rounded as the mixed sketch rounds it (to FP32, or to bf16 by
+    round-to-nearest-even on the FP32 bits)."""
+    if prec == 0:
+        return G
+    g = G.astype(np.float32)
+    if prec == 2:
+        b = g.view(np.uint32).astype(np.uint64)
+        b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+        g = b.astype(np.uint32).view(np.float32)
+    return g.astype(np.float64)
 
 
 @pytest.mark.parametrize("m,n,prec", [(3000, 2000, 0), (1000, 3000, 0),
@@ -85,7 +100,7 @@ def test_initial_sketch(handle, m, n, prec):
     nb, pp = 64, 10
     print(f"initial sketch {m}x{n} prec={prec}", flush=True)
     A = matgen.gaussian(m, n, seed=11)
-    G = oracle.omega(nb + pp, m)
+    G = _round_operator(oracle.omega(nb + pp, m), prec)
     _, _, _, st = _run(handle, A, nb, pp, ("y0",), sketch_prec=prec,
                        max_rank=1)
     Y = st["y0"]
