@@ -83,6 +83,8 @@ SIGNATURES = {
                                          C.c_int, _vp, _vp]),
     "hqrrp_orgqr_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
                                   C.c_int, _vp, _vp, C.c_int]),
+    "hqrrp_ormqr_dev": (C.c_int, [_vp, C.c_char, C.c_char, C.c_int, C.c_int,
+                                  C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
     "hqrrp_qr_fixed_dev": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp,
                                      C.c_int, _vp, _vp, C.c_int, _vp,
                                      C.c_int]),

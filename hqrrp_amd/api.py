@@ -233,6 +233,17 @@ def orgqr_device(handle: Handle, At, tau_t, m: int, n: int, k: int,
     return Q
 
 
+def ormqr_device(handle: Handle, side: str, trans: str, At, tau_t, k: int,
+                 Ct, m: int, n: int, lda: int | None = None) -> None:
+    """op(Q) C or C op(Q) in place (LAPACK dormqr semantics) on the
+    column-major m x n matrix held by Ct, with Q from the first k reflectors
+    of At (column-major, ld lda: m for side 'L', n for 'R')."""
+    nq = m if side.upper() == "L" else n
+    check(lib().hqrrp_ormqr_dev(handle.ptr, side.encode(), trans.encode(), m,
+                                n, k, _vp(At), nq if lda is None else lda,
+                                _vp(tau_t), _vp(Ct), m), "hqrrp_ormqr_dev")
+
+
 def _colmajor_of(A):
     """torch m x n (any layout) -> contiguous (n, m): column-major m x n."""
     return A.detach().t().contiguous()

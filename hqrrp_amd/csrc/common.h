@@ -202,6 +202,14 @@ __device__ __forceinline__ void hq_wave_argmax64(double& key, int& pos) {
   pos = p0;
 }
 
+// Pivot key of a candidate's partial norm: a NaN norm (NaN or Inf entries
+// in its column) orders after every finite candidate and before the columns
+// already taken (key -1), so every lane's argmax agrees and such a column is
+// chosen last among the candidates, in position order.
+__device__ __forceinline__ double hq_pivot_key(double d) {
+  return d == d ? d : -0.5;
+}
+
 // Largest finite value of T (LAPACK's overflow threshold in dlapy2/slapy2).
 template <class T>
 __device__ __forceinline__ T hq_huge();

@@ -196,7 +196,8 @@ struct hqrrp_handle_s {
   int cu_count = 0;
   size_t lds_max = 0;
   int sketch_max_wg = 0, panel_max_wg = 0;
-  DevBuf words;  // [8] error word of the persistent kernels
+  DevBuf words;  // ints [8]: error word of the persistent kernels;
+                 // u64 [8, 14): the prescaling state (launch_scale_pre)
   DevBuf Y, G, Gcache, moves, piv, gram, T, perm;
   Xchg xs, xp;  // exchange buffers of the sketch and panel kernels
   DevBuf tmpA, tmpY, tmpJ, split, Mx, W2, Pp, Qm;
@@ -232,6 +233,11 @@ struct hqrrp_handle_s {
   long long ph_launch[HQ_PHASES] = {};
 
   int* err() const { return words.as<int>() + 8; }
+  // Prescaling state: slot 0 for a factorisation, slot 1 for a host entry
+  // that wraps one (dgeqp4 with fixed columns).
+  unsigned long long* scale_words(int slot = 0) const {
+    return words.as<unsigned long long>() + 8 + 8 * slot;
+  }
 };
 
 namespace {
@@ -982,7 +988,24 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   h->sk32 = o.sketch_prec != HQRRP_SKETCH_FP64;
   h->skbf = o.sketch_prec == HQRRP_SKETCH_BF16;
   StreamScope ss(h);
-  return factor_loop(h, m, n, kmax, dA, lda, djpvt, dtau, o);
+  // A matrix of extreme magnitude is factored as 2^p A (exact: the pivots,
+  // reflectors and tau do not change, R is scaled back): max |a| then lies
+  // in [0.5, 1), so squared norms neither overflow nor underflow, and a
+  // mixed sketch stays inside FP32's range.
+  const int lim = h->sk32 ? 60 : 256;
+  HQ_TRY(launch_scale_pre(dA, lda, m, n, h->scale_words(), -lim, lim, h->cur));
+  HQ_RET(factor_loop(h, m, n, kmax, dA, lda, djpvt, dtau, o));
+  // Columns factored: blocks of nb up to kmax, the final block whole.
+  int kdone = mn;
+  for (int j = 0; j < mn; j += o.nb) {
+    if (j + o.nb >= m || j + o.nb >= n) break;
+    if (j + o.nb >= kmax) {
+      kdone = kmax;
+      break;
+    }
+  }
+  HQ_TRY(launch_scale_post(dA, lda, m, n, kdone, h->scale_words(), h->cur));
+  return HQRRP_OK;
 }
 
 int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
@@ -1318,6 +1341,17 @@ int run_classic_qrcp(hqrrp_handle_t h, int m, int n, int k0, double* dA,
   return HQRRP_OK;
 }
 
+// The classic QRCP of a whole matrix with the prescaling of factor_locked
+// (an exact power of two when max |a| is extreme).
+int run_classic_scaled(hqrrp_handle_t h, int m, int n, double* dA, long lda,
+                       int* djpvt, double* dtau, double* ws) {
+  HQ_TRY(launch_scale_pre(dA, lda, m, n, h->scale_words(), -256, 256, h->cur));
+  HQ_RET(run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau, ws));
+  HQ_TRY(launch_scale_post(dA, lda, m, n, std::min(m, n), h->scale_words(),
+                           h->cur));
+  return HQRRP_OK;
+}
+
 // Q (m x n) = the first n columns of H_1 ... H_k, the reflectors held in the
 // first k columns of A below the diagonal (LAPACK dorgqr, written to Q):
 // Q = [I; 0], then block reflectors from the last to the first applied to
@@ -1342,6 +1376,54 @@ int run_orgqr(hqrrp_handle_t h, int m, int n, int k, const double* dA,
     ta.tnotrans = 1;
     HQ_RET(trail_w(h, ta));
     HQ_RET(trail_apply(h, ta));
+  }
+  return HQRRP_OK;
+}
+
+// op(Q) C (left) or C op(Q) (right), Q = H_1 ... H_k from the reflectors in
+// the first k columns of A below the diagonal and tau (LAPACK dormqr), by
+// block reflectors of 64: Q = Q_1 Q_2 ..., Q_b = I - V T V' with T rebuilt
+// from V'V.  Left: Q'C applies Q_1' first (W = T'V'C), QC the last block
+// first (W = T V'C), both through the trailing-update kernels on the rows
+// the block touches; right: C Q applies Q_1 first and C Q' the last block
+// first, C(:, j:) -= ((C(:, j:) V) T or T') V' by generic products.
+int run_ormqr(hqrrp_handle_t h, bool left, bool trans, int m, int n, int k,
+              const double* dA, long lda, const double* dtau, double* dC,
+              long ldc) {
+  const int nb = 64;
+  HQ_RET(ensure(h->T, sizeof(double) * nb * nb));
+  HQ_RET(ensure(h->gram, sizeof(double) * 64 * 64));
+  const int nq = left ? m : n;
+  const int nblk = (k + nb - 1) / nb;
+  // Left Q' and right Q take the blocks first to last.
+  const bool forward = left == trans;
+  for (int q = 0; q < nblk; ++q) {
+    const int j = (forward ? q : nblk - 1 - q) * nb;
+    const int b = std::min(nb, k - j), rows = nq - j;
+    const double* V = dA + j + (long)j * lda;
+    double* gram = h->gram.as<double>();
+    HQ_RET(gemm(h, true, false, b, b, rows, 1.0, V, lda, 1, V, lda, 1, 0.0,
+                gram, b));
+    HQ_TRY(launch_larft(b, gram, b, dtau + j, h->T.as<double>(), nb, h->cur));
+    if (left) {
+      TrailArgs ta;
+      HQ_RET(trail_prepare(h, rows, n, b, V, lda, h->T.as<double>(), nb,
+                           dC + j, ldc, &ta));
+      ta.tnotrans = trans ? 0 : 1;
+      HQ_RET(trail_w(h, ta));
+      HQ_RET(trail_apply(h, ta));
+    } else {
+      HQ_RET(ensure(h->genM, sizeof(double) * 2 * (size_t)m * b));
+      double* W = h->genM.as<double>();  // m x b: C(:, j:) V
+      double* W2 = W + (size_t)m * b;    // m x b: W T (or W T')
+      double* Cj = dC + (long)j * ldc;
+      HQ_RET(gemm(h, false, false, m, b, rows, 1.0, Cj, ldc, 0, V, lda, 1, 0.0,
+                  W, m));
+      HQ_RET(gemm(h, false, trans, m, b, b, 1.0, W, m, 0, h->T.as<double>(),
+                  nb, 0, 0.0, W2, m, false));
+      HQ_RET(gemm(h, false, true, m, rows, b, -1.0, W2, m, 0, V, lda, 1, 1.0,
+                  Cj, ldc, false));
+    }
   }
   return HQRRP_OK;
 }
@@ -1509,7 +1591,7 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
       HQ_TRY(hipEventCreateWithFlags(&h->ev_t, hipEventDisableTiming));
     }
   }
-  int rc = ensure(h->words, 64);
+  int rc = ensure(h->words, 256);
   if (rc) return rc;
   HQ_TRY(hipMemset(h->words.p, 0, h->words.bytes));
   // The allocation-free entry's workspace on this device (looked up once
@@ -1699,7 +1781,7 @@ extern "C" int hqrrp_geqp3_dev(hqrrp_handle_t h, int m, int n, double* dA,
   if (std::min(m, n) == 0) return HQRRP_OK;
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
-  return run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau);
+  return run_classic_scaled(h, m, n, dA, lda, djpvt, dtau, nullptr);
 }
 
 extern "C" int hqrrp_geqp3_noalloc_dev(hqrrp_handle_t h, int m, int n,
@@ -1718,8 +1800,8 @@ extern "C" int hqrrp_geqp3_noalloc_dev(hqrrp_handle_t h, int m, int n,
   if (!h->scratch) return HQRRP_ERR_HIP;
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
-  return run_classic_qrcp(h, m, n, 0, dA, lda, djpvt, dtau,
-                          static_cast<double*>(h->scratch));
+  return run_classic_scaled(h, m, n, dA, lda, djpvt, dtau,
+                            static_cast<double*>(h->scratch));
 }
 
 extern "C" int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k,
@@ -1732,6 +1814,22 @@ extern "C" int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k,
   std::lock_guard<std::mutex> lk(h->mu);
   StreamScope ss(h);
   return run_orgqr(h, m, n, k, dA, lda, dtau, dQ, ldq);
+}
+
+extern "C" int hqrrp_ormqr_dev(hqrrp_handle_t h, char side, char trans,
+                               int m, int n, int k, const double* dA, int lda,
+                               const double* dtau, double* dC, int ldc) {
+  const bool left = side == 'L' || side == 'l';
+  const bool tr = trans == 'T' || trans == 't';
+  const int nq = left ? m : n;
+  if (!h || !(left || side == 'R' || side == 'r') ||
+      !(tr || trans == 'N' || trans == 'n') || m < 0 || n < 0 || k < 0 ||
+      k > nq || lda < std::max(1, nq) || ldc < std::max(1, m))
+    return HQRRP_ERR_ARG;
+  if (m == 0 || n == 0 || k == 0) return HQRRP_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  StreamScope ss(h);
+  return run_ormqr(h, left, tr, m, n, k, dA, lda, dtau, dC, ldc);
 }
 
 extern "C" int hqrrp_qr_fixed_dev(hqrrp_handle_t h, int m, int n, int k,
@@ -2124,6 +2222,11 @@ int host_dgeqp4_fixed(int m, int n, double* A, int lda, int* jpvt, double* tau,
                           h->stream));
   HQ_TRY(hipMemcpyAsync(dJ, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
                         h->stream));
+  // The fixed columns' QR on the prescaled matrix (the free block's own
+  // prescaling inside factor_locked then finds nothing to do); R scaled back
+  // at the end.
+  HQ_TRY(launch_scale_pre(dA, m, m, n, h->scale_words(1), -256, 256,
+                          h->cur));
   HQ_RET(run_geqrf(h, m, n, nf, dA, m, dT));
   if (nf < mn) {
     std::vector<int> prev(jpvt, jpvt + n);
@@ -2153,6 +2256,7 @@ int host_dgeqp4_fixed(int m, int n, double* A, int lda, int* jpvt, double* tau,
     HQ_TRY(launch_copy_block(dA + (long)nf * m, m, top, nf, nf, n - nf,
                              h->stream));
   }
+  HQ_TRY(launch_scale_post(dA, m, m, n, mn, h->scale_words(1), h->stream));
   HQ_RET(check_err(h));
   HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
                           sizeof(double) * m, n, hipMemcpyDeviceToHost,
@@ -2185,8 +2289,10 @@ int host_dgeqp4_classic(int m, int n, double* A, int lda, int* jpvt,
                           h->stream));
   HQ_TRY(hipMemcpyAsync(dJ, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
                         h->stream));
+  HQ_TRY(launch_scale_pre(dA, m, m, n, h->scale_words(), -256, 256, h->cur));
   if (nf > 0) HQ_RET(run_geqrf(h, m, n, nf, dA, m, dT));
   HQ_RET(run_classic_qrcp(h, m, n, nf, dA, m, dJ, dT));
+  HQ_TRY(launch_scale_post(dA, m, m, n, mn, h->scale_words(), h->cur));
   HQ_RET(check_err(h));
   HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
                           sizeof(double) * m, n, hipMemcpyDeviceToHost,

@@ -478,7 +478,7 @@ __device__ __forceinline__ void panel_qrcp(Lds<NB>& L, double* __restrict__ A, l
     // 1. pivot slot p (uniform): largest partial norm, first position on ties
     int ppos;
     if (pivoting) {
-      double key = sdone ? -1.0 : dn;
+      double key = sdone ? -1.0 : hq_pivot_key(dn);
       int kp = sdone ? 0x7fffffff : spos;
       hq_wave_argmax(key, kp, 64);
       ppos = kp;

@@ -152,7 +152,104 @@ __global__ __launch_bounds__(HQ_NT) void widen_block_kernel(
   }
 }
 
+// Power-of-two prescaling of a matrix whose magnitude is extreme (words:
+// [0] the bits of max |a_ij| over the finite entries, [1] 1 if some entry
+// is NaN or Inf, [2] s, [3] 1/s, [4] 1 when the forward pass must run, [5]
+// 1 when the backward pass must).
+__global__ __launch_bounds__(64) void scale_init_kernel(
+    unsigned long long* w) {
+  if (threadIdx.x < 6) w[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(HQ_NT) void amax_kernel(const double* A, long lda,
+                                                     int m, int n,
+                                                     unsigned long long* w) {
+  const long i0 = (long)blockIdx.x * kRowsPerBlock;
+  double mx = 0.0;
+  bool bad = false;
+  for (long c = blockIdx.y; c < n; c += gridDim.y) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long i = i0 + q * HQ_NT + threadIdx.x;
+      if (i < m) {
+        const double v = fabs(A[i + c * lda]);
+        if (v <= 1.79769313486231571e308)
+          mx = fmax(mx, v);
+        else
+          bad = true;  // Inf or NaN
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  const bool any_bad = __any(bad);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(mx));
+    if (any_bad) atomicOr(&w[1], 1ull);
+  }
+}
+
+// s = 2^-(e+1) with e = exponent(max |a|) when that exponent lies outside
+// [lo, hi] (s max|a| in [0.5, 1)); else 1.  The forward pass also runs when
+// there are non-finite entries: multiplying by s (even 1) turns signalling
+// NaNs into quiet ones.
+__global__ __launch_bounds__(64) void scale_decide_kernel(unsigned long long* w,
+                                                          int lo, int hi) {
+  if (threadIdx.x != 0) return;
+  const double amax = __longlong_as_double((long long)w[0]);
+  double s = 1.0;
+  if (amax > 0.0) {
+    const int e = ilogb(amax);
+    if (e < lo || e > hi) s = ldexp(1.0, -e - 1);
+  }
+  w[2] = (unsigned long long)__double_as_longlong(s);
+  w[3] = (unsigned long long)__double_as_longlong(1.0 / s);
+  w[4] = (s != 1.0 || w[1]) ? 1ull : 0ull;
+  w[5] = (s != 1.0) ? 1ull : 0ull;
+}
+
+// Forward (fwd = 1): A *= s everywhere.  Backward: A *= 1/s on R and the
+// trailing matrix -- entries (i, j) with i <= j and i < k, or i >= k and
+// j >= k -- leaving the reflectors, which do not scale.
+__global__ __launch_bounds__(HQ_NT) void scale_apply_kernel(
+    double* A, long lda, int m, int n, int k, const unsigned long long* w,
+    int fwd) {
+  if (!w[fwd ? 4 : 5]) return;
+  const double s = __longlong_as_double((long long)w[fwd ? 2 : 3]);
+  const long i0 = (long)blockIdx.x * kRowsPerBlock;
+  for (long c = blockIdx.y; c < n; c += gridDim.y) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long i = i0 + q * HQ_NT + threadIdx.x;
+      if (i >= m) continue;
+      if (fwd || (i <= c && i < k) || (i >= k && c >= k)) A[i + c * lda] *= s;
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_scale_pre(double* A, long lda, int m, int n,
+                            unsigned long long* w, int lo, int hi,
+                            hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_init_kernel, dim3(1), dim3(64), 0, s, w);
+  dim3 grid(hq_cdiv(m, kRowsPerBlock), n < 2048 ? n : 2048);
+  hipLaunchKernelGGL(amax_kernel, grid, dim3(HQ_NT), 0, s, A, lda, m, n, w);
+  hipLaunchKernelGGL(scale_decide_kernel, dim3(1), dim3(64), 0, s, w, lo, hi);
+  hipLaunchKernelGGL(scale_apply_kernel, grid, dim3(HQ_NT), 0, s, A, lda, m, n,
+                     0, w, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_post(double* A, long lda, int m, int n, int k,
+                             const unsigned long long* w, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  dim3 grid(hq_cdiv(m, kRowsPerBlock), n < 2048 ? n : 2048);
+  hipLaunchKernelGGL(scale_apply_kernel, grid, dim3(HQ_NT), 0, s, A, lda, m, n,
+                     k, w, 0);
+  return hipGetLastError();
+}
 
 hipError_t launch_iota(int* p, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;

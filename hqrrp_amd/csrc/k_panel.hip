@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     int p = k;
     if (a.pivoting) {
       const bool cand = col_ok && pos >= k;
-      double key = cand ? dn : -1.0;
+      double key = cand ? hq_pivot_key(dn) : -1.0;
       int q = cand ? pos : 1 << 30;
       hq_wave_argmax64(key, q);
       p = q;

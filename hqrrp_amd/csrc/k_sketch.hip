@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    thread combines the 8 wavefront winners identically and the
     //    winning column's lanes publish the slot.
     {
-      double key = (pos >= 0) ? (double)d : -1.0;
+      double key = (pos >= 0) ? hq_pivot_key((double)d) : -1.0;
       int kp = (pos >= 0) ? pos : INT_MAX;
       hq_wave_argmax64(key, kp);  // lanes l, l+32 hold the same column
       if (lane == 0) {

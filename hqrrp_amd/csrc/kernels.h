@@ -223,6 +223,18 @@ hipError_t launch_fill_sentinel(double* p, size_t n, hipStream_t s);
 // dst (cols x rows, ld ldd) = src' (src: rows x cols, ld lds).
 hipError_t launch_transpose(const double* src, long lds, int rows, int cols,
                             double* dst, long ldd, hipStream_t s);
+// Exact power-of-two prescaling of an m x n matrix whose largest entry is
+// extreme (exponent outside [lo, hi]): pre records max |A| and s in w (6
+// words of device memory) and multiplies A by s (also when A holds NaN or
+// Inf, which quiets signalling NaNs); post multiplies R and the trailing
+// matrix of a factorisation truncated at k by 1/s.  Products, quotients and
+// square roots of power-of-two multiples are exact, so pivots, reflectors
+// and tau are unchanged and R scales exactly.
+hipError_t launch_scale_pre(double* A, long lda, int m, int n,
+                            unsigned long long* w, int lo, int hi,
+                            hipStream_t s);
+hipError_t launch_scale_post(double* A, long lda, int m, int n, int k,
+                             const unsigned long long* w, hipStream_t s);
 // p[i] = i for i < n.
 hipError_t launch_iota(int* p, int n, hipStream_t s);
 // The moves of a permutation (perm[pos] = the position whose column ends at

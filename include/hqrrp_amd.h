@@ -201,6 +201,17 @@ int hqrrp_geqp3_noalloc_dev(hqrrp_handle_t h, int m, int n, double *dA,
 int hqrrp_orgqr_dev(hqrrp_handle_t h, int m, int n, int k, const double *dA,
                     int lda, const double *dtau, double *dQ, int ldq);
 
+/* op(Q) C (side 'L') or C op(Q) (side 'R'), op = transpose for trans 'T',
+ * with Q = H_1 ... H_k from the reflectors in the first k columns of A below
+ * the diagonal and tau (LAPACK dormqr semantics, in place on C: m x n, ld
+ * ldc; A: nq x k, ld lda >= nq, nq = m for 'L', n for 'R').  Replaces the
+ * dormqr_ call dgeqp4 makes to apply the fixed columns' reflectors
+ * (NoFLA_HQRRP_WY_blk_var4.c:231-235) and the Q products callers form
+ * after a factorisation (the test driver's dorgqr, WYtest.c:89-93). */
+int hqrrp_ormqr_dev(hqrrp_handle_t h, char side, char trans, int m, int n,
+                    int k, const double *dA, int lda, const double *dtau,
+                    double *dC, int ldc);
+
 /* QR at fixed pivots, truncated to k (no counterpart in the reference):
  * with AP = A(:, jpvt-1) (jpvt 1-based, n entries), AP(:, 0:k) = Q R11 and
  * R = [R11 R12] = the first k rows of the R factor of AP.  Q: m x k (ld

@@ -250,6 +250,26 @@ def downdate_y(U, R12, Y2, T, G, use_ref=False):
     return Y2, G
 
 
+def dormqr(side, trans, Afac, tau, Cm):
+    """LAPACK dormqr on a copy of Cm: op(Q) C ('L') or C op(Q) ('R'), Q the
+    product of the len(tau) reflectors stored below Afac's diagonal."""
+    A = fortran(Afac)
+    Cm = fortran(Cm)
+    m, n = Cm.shape
+    k = len(tau)
+    lw = max(1, 64 * max(m, n))
+    work = np.zeros(lw)
+    info = C.c_int(0)
+    t = np.ascontiguousarray(tau, dtype=np.float64)
+    lib().orc_dormqr(side.encode(), trans.encode(), C.byref(C.c_int(m)),
+                     C.byref(C.c_int(n)), C.byref(C.c_int(k)), _dp(A),
+                     C.byref(C.c_int(A.shape[0])), _dp(t), _dp(Cm),
+                     C.byref(C.c_int(m)), _dp(work), C.byref(C.c_int(lw)),
+                     C.byref(info))
+    assert info.value == 0, info.value
+    return Cm
+
+
 def dlarft(V, tau):
     V = fortran(V)
     n, k = V.shape

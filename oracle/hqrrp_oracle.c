@@ -622,6 +622,13 @@ void orc_dorgqr(int *m, int *n, int *k, double *A, int *lda, const double *tau,
   LP.dorgqr(m, n, k, A, lda, tau, work, lwork, info);
 }
 
+void orc_dormqr(const char *side, const char *trans, int *m, int *n, int *k,
+                double *A, int *lda, const double *tau, double *C, int *ldc,
+                double *work, int *lwork, int *info) {
+  LP.dormqr(side, trans, m, n, k, A, lda, tau, C, ldc, work, lwork, info, 1,
+            1);
+}
+
 void orc_dlarft(int n, int k, const double *V, int ldv, const double *tau,
                 double *T, int ldt) {
   LP.dlarft("Forward", "Columnwise", &n, &k, V, &ldv, tau, T, &ldt, 7, 10);

@@ -49,6 +49,9 @@ void orc_dgeqp3(int *m, int *n, double *A, int *lda, int *jpvt, double *tau,
                 double *work, int *lwork, int *info);
 void orc_dorgqr(int *m, int *n, int *k, double *A, int *lda, const double *tau,
                 double *work, int *lwork, int *info);
+void orc_dormqr(const char *side, const char *trans, int *m, int *n, int *k,
+                double *A, int *lda, const double *tau, double *C, int *ldc,
+                double *work, int *lwork, int *info);
 void orc_dlarft(int n, int k, const double *V, int ldv, const double *tau,
                 double *T, int ldt);
 

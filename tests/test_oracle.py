@@ -261,3 +261,25 @@ def test_truncation_cut_block_is_a_partial_qr(kmax):
     bad = tt[:kmax].copy()
     bad[kmax - 1] = 0.0
     assert resid(bad) > 1e-6
+
+
+@pytest.mark.parametrize("side,trans", [("L", "N"), ("L", "T"), ("R", "N"),
+                                        ("R", "T")])
+def test_dormqr_equals_explicit_q(side, trans):
+    """The oracle's dormqr (LAPACK) applies the explicit Q that dorgqr forms
+    (Q' Q = I pinned separately): op(Q) C or C op(Q) to rounding; a planted
+    mistake (the opposite transpose) fails."""
+    rng = np.random.default_rng(71)
+    nq, k = 120, 70
+    Af, _, tau = oracle.hqrrp(rng.standard_normal((nq, 90)), 32, 8, 1)
+    tau = tau[:k]
+    Q = oracle.form_q(Af, tau, ncols=nq)  # nq x nq
+    Cm = rng.standard_normal((nq, 50)) if side == "L" else \
+        rng.standard_normal((40, nq))
+    got = oracle.dormqr(side, trans, Af, tau, Cm)
+    op = Q.T if trans == "T" else Q
+    want = op @ Cm if side == "L" else Cm @ op
+    wrong = (Q if trans == "T" else Q.T) @ Cm if side == "L" else \
+        Cm @ (Q if trans == "T" else Q.T)
+    assert np.abs(got - want).max() <= 1e-13
+    assert np.abs(got - wrong).max() > 1e-3
