@@ -228,7 +228,8 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 // double at a time, so both forms compute identical bits.
 constexpr int kVN = 256;  // columns of C per workgroup
 constexpr int kCK = 16;   // K rows per chunk
-template <bool VEC>
+constexpr int kRing = 4;  // chunks of the register ring (kRing - 1 in flight)
+template <bool VEC, int RING = kRing>
 __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l15 = lane & 15, g4 = lane >> 4;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
   const long nsteps = (kend - kbeg + kCK - 1) / kCK;
   const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
 
-  hq_d4 cb[3][4], vb[3][4];
+  hq_d4 cb[RING][4], vb[RING][4];
   auto load = [&](hq_d4 (&cr)[4], hq_d4 (&vr)[4], long it) {
     if (it >= nsteps) {
 #pragma unroll
@@ -311,17 +312,19 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[i][e], vr[j][e],
                                                             acc[i][j], 0, 0, 0);
   };
-  // Three-chunk ring, the step count padded to a multiple of three
-  // (padding chunks are zeros): constant ring indices, no early exit.
-  load(cb[0], vb[0], 0);
-  load(cb[1], vb[1], 1);
-  for (long it = 0; it < nsteps; it += 3) {
-    load(cb[2], vb[2], it + 2);
-    compute(cb[0], vb[0]);
-    load(cb[0], vb[0], it + 3);
-    compute(cb[1], vb[1]);
-    load(cb[1], vb[1], it + 4);
-    compute(cb[2], vb[2]);
+  // RING-chunk ring, the step count padded to a multiple of RING (padding
+  // chunks are zeros): constant ring indices, no early exit.  Before chunk q
+  // is multiplied, chunk q + RING - 1 is loaded into the slot chunk q - 1
+  // left.
+#pragma unroll
+  for (int q = 0; q + 1 < RING; ++q) load(cb[q], vb[q], q);
+  for (long it = 0; it < nsteps; it += RING) {
+#pragma unroll
+    for (int q = 0; q < RING; ++q) {
+      load(cb[(q + RING - 1) % RING], vb[(q + RING - 1) % RING],
+           it + q + RING - 1);
+      compute(cb[q], vb[q]);
+    }
   }
   // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
   // n = n0 + 16i + l/16 + 4r.

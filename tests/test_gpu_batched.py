@@ -212,3 +212,62 @@ def test_batched_autotune(handle):
     want = [table[(mi - 1) // 64, (ni - 1) // 64] for mi, ni in zip(m, n)]
     assert plan.nb.tolist() == want
     plan.close()
+
+
+@pytest.mark.timeout(600)
+def test_batched_benchmark_launch_in_graph(handle):
+    """The benchmark's whole 8192-matrix batch (seed 0, default block sizes,
+    pp = 8), factored by a graph replay: two replays equal the eager launch
+    bit for bit; every 64th matrix meets the backward-error bar and every
+    512th the oracle's pivots."""
+    import os
+
+    import hqrrp_amd
+
+    count = 8192
+    print(f"batched benchmark launch {count}", flush=True)
+    m, n = matgen.batch_shapes(count, seed=0)
+    mats = matgen.batch_matrices(m, n, seed=0,
+                                 workers=min(16, os.cpu_count() or 1))
+    plan = hqrrp_amd.BatchPlan(handle, m, n, nb=None, pp=8)
+    nbs = plan.nb
+    src = hqrrp_amd.PackedBatch(m, n)
+    src.load(mats)
+    work = hqrrp_amd.PackedBatch(m, n)
+    ident = src.jpvt.clone()
+
+    def step():
+        work.A.copy_(src.A)
+        work.jpvt.copy_(ident)
+        plan.factor(handle, work.A_ptrs, work.jpvt_ptrs, work.tau_ptrs)
+
+    step()
+    handle.synchronize()
+    torch.cuda.synchronize()
+    eagerA, eagerJ, eagerT = (work.A.clone(), work.jpvt.clone(),
+                              work.tau.clone())
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        handle.use_stream(s)
+        step()
+    handle.use_stream(torch.cuda.current_stream())
+    for rep in range(2):
+        work.A.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(work.A, eagerA), rep
+        assert torch.equal(work.jpvt, eagerJ), rep
+        assert torch.equal(work.tau, eagerT), rep
+    out = work.unpack()
+    for i in range(0, count, 64):
+        A = mats[i]
+        if i % 512 == 0:
+            check_one(A, out[i], int(nbs[i]), 8, 1, f"workload {i}")
+        else:
+            res, orth = oracle.residuals(A, *out[i])
+            bound = 50 * max(A.shape) * EPS
+            assert res <= bound and orth <= bound, (i, res, orth)
+    plan.close()
