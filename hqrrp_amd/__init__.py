@@ -7,11 +7,11 @@ library's HIP kernels; nothing here computes a factorization.
 """
 from .api import (BatchPlan, Handle, PackedBatch, dgeqp4, fixed_pivot_qr,
                   geqp3_device, geqrf_device, hqrrp, hqrrp_batched, hqrrp_device,
-                  orgqr_device, ormqr_device, qr_fixed, qr_fixed_adjoint, reference_omega,
-                  to_colmajor, from_colmajor)
+                  omega_carry, orgqr_device, ormqr_device, qr_fixed, qr_fixed_adjoint,
+                  reference_omega, set_omega_carry, to_colmajor, from_colmajor)
 from ._lib import HQRRPError, Options, lib
 
 __all__ = ["BatchPlan", "Handle", "PackedBatch", "hqrrp_batched", "dgeqp4", "fixed_pivot_qr", "geqp3_device", "geqrf_device", "hqrrp",
            "hqrrp_device", "orgqr_device", "ormqr_device", "qr_fixed", "qr_fixed_adjoint",
-           "reference_omega", "to_colmajor", "from_colmajor", "HQRRPError",
+           "reference_omega", "omega_carry", "set_omega_carry", "to_colmajor", "from_colmajor", "HQRRPError",
            "Options", "lib"]

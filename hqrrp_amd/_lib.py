@@ -64,6 +64,12 @@ SIGNATURES = {
     "hqrrp_factor_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp,
                                    _vp, C.POINTER(Options)]),
     "hqrrp_reference_omega": (None, [C.c_int, C.c_int, _vp, C.c_uint]),
+    "hqrrp_reference_omega_carry": (None, [C.c_int, C.c_int, _vp, C.c_uint,
+                                           C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]),
+    "hqrrp_omega_carry_get": (None, [C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]),
+    "hqrrp_omega_carry_set": (None, [C.c_int, C.c_double]),
     "hqrrp_sketch_qrcp_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int,
                                         C.c_int, _vp, _vp]),
     "hqrrp_panel_qrcp_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int,

@@ -28,11 +28,35 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data)
 
 
-def reference_omega(r: int, m: int, seed: int = 12) -> np.ndarray:
-    """The reference's Gaussian sketch matrix (r x m, column-major)."""
+def reference_omega(r: int, m: int, seed: int = 12, carry=None):
+    """The reference's Gaussian sketch matrix (r x m, column-major) as the
+    first call of a process draws it.  With carry=(have, value), the
+    generator starts from that cached second variate and the result is
+    (G, carry after the draw)."""
     G = np.zeros((r, m), dtype=np.float64, order="F")
-    lib().hqrrp_reference_omega(r, m, _ptr(G), seed)
-    return G
+    if carry is None:
+        lib().hqrrp_reference_omega(r, m, _ptr(G), seed)
+        return G
+    have = C.c_int(int(carry[0]))
+    val = C.c_double(float(carry[1]))
+    lib().hqrrp_reference_omega_carry(r, m, _ptr(G), seed, C.byref(have),
+                                      C.byref(val))
+    return G, (have.value, val.value)
+
+
+def omega_carry():
+    """The library's process-wide cached variate of the reference generator
+    (have, value), read and advanced by every factorisation that draws the
+    reference Omega."""
+    have = C.c_int(0)
+    val = C.c_double(0.0)
+    lib().hqrrp_omega_carry_get(C.byref(have), C.byref(val))
+    return have.value, val.value
+
+
+def set_omega_carry(have=0, value=0.0):
+    """Replace that state ((0, 0.0): as in a new process)."""
+    lib().hqrrp_omega_carry_set(int(have), float(value))
 
 
 def dgeqp4(A, jpvt=None, lwork=None):

@@ -132,15 +132,17 @@ int scratch_address(int device, void** out) {
 
 }  // namespace
 
-extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
-  // Marsaglia polar method exactly as NoFLA_Normal_random_number
-  // (NoFLA_HQRRP_WY_blk_var4.c:593-614): pairs of uniforms on (-1,1) with
-  // 0 < a < 1, returning c1*f and then the cached c2*f, column-major fill.
+namespace {
+
+// Marsaglia polar method exactly as NoFLA_Normal_random_number
+// (NoFLA_HQRRP_WY_blk_var4.c:593-614): pairs of uniforms on (-1,1) with
+// 0 < a < 1, returning c1*f and then the cached c2*f, column-major fill.
+// (have, cached) is the generator's cached second variate on entry and on
+// exit: the reference keeps it in function statics that survive srand().
+void polar_fill(long total, double* G, unsigned seed, bool& have,
+                double& cached) {
   RefRand rng(seed);
   const double rmax = 2147483647.0;
-  double cached = 0.0;
-  bool have = false;
-  long total = (long)r * m;
   for (long idx = 0; idx < total; ++idx) {
     if (have) {
       have = false;
@@ -158,6 +160,49 @@ extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
     have = true;
     G[idx] = 0.0 + 1.0 * (c1 * f);
   }
+}
+
+// The process-wide cached variate of the reference generator.  The
+// reference seeds rand() with srand(12) at the start of every HQRRP call
+// (NoFLA_HQRRP_WY_blk_var4.c:380) but its normal generator's statics
+// (lines 594-595) carry over: a call that draws an odd number r*m of
+// normals leaves the second variate of its last pair cached, and the next
+// call in the same process returns that value first.  Every factorisation
+// that draws the reference's Omega reads and advances this state the same
+// way, so a sequence of calls sees the sequence of Omegas the reference's
+// would.
+std::mutex g_carry_mu;
+bool g_carry_have = false;
+double g_carry_value = 0.0;
+
+}  // namespace
+
+extern "C" void hqrrp_reference_omega(int r, int m, double* G, unsigned seed) {
+  bool have = false;
+  double cached = 0.0;
+  polar_fill((long)r * m, G, seed, have, cached);
+}
+
+extern "C" void hqrrp_reference_omega_carry(int r, int m, double* G,
+                                            unsigned seed, int* have,
+                                            double* cached) {
+  bool hv = have && *have;
+  double cv = (hv && cached) ? *cached : 0.0;
+  polar_fill((long)r * m, G, seed, hv, cv);
+  if (have) *have = hv ? 1 : 0;
+  if (cached) *cached = cv;
+}
+
+extern "C" void hqrrp_omega_carry_get(int* have, double* value) {
+  std::lock_guard<std::mutex> lk(g_carry_mu);
+  if (have) *have = g_carry_have ? 1 : 0;
+  if (value) *value = g_carry_value;
+}
+
+extern "C" void hqrrp_omega_carry_set(int have, double value) {
+  std::lock_guard<std::mutex> lk(g_carry_mu);
+  g_carry_have = have != 0;
+  g_carry_value = have ? value : 0.0;
 }
 
 // Phases timed by the optional profiler (hqrrp_profile_*).
@@ -214,6 +259,9 @@ struct hqrrp_handle_s {
   void* scratch = nullptr;  // this device's allocation-free workspace
   int om_r = -1, om_m = -1;
   unsigned om_seed = 0;
+  // Cached-variate state the cached Omega was drawn with, and after it.
+  bool om_have = false, om_out_have = false;
+  double om_in = 0.0, om_out = 0.0;
   // Batched path: block size per 64 x 64 shape bucket chosen by
   // hqrrp_batch_tune for (bt_pp, bt_piv); 0 = not tuned.
   int bt_nb[8][8] = {};
@@ -793,16 +841,31 @@ int load_omega(hqrrp_handle_t h, int r, int m, const hqrrp_options& o) {
     HQ_TRY(launch_transpose(o.omega, r, r, m, h->G.as<double>(), m, h->cur));
     return HQRRP_OK;
   }
-  if (h->om_r != r || h->om_m != m || h->om_seed != o.seed) {
+  // The reference's Omega depends on the generator's cached variate as
+  // this call finds it (process-wide, advanced by every call).
+  std::lock_guard<std::mutex> lk(g_carry_mu);
+  const bool in_have = g_carry_have;
+  const double in_value = g_carry_value;
+  if (h->om_r != r || h->om_m != m || h->om_seed != o.seed ||
+      h->om_have != in_have ||
+      (in_have && __builtin_memcmp(&h->om_in, &in_value, sizeof(double)))) {
     std::vector<double> host((size_t)r * m);
-    hqrrp_reference_omega(r, m, host.data(), o.seed);
+    bool hv = in_have;
+    double cv = in_value;
+    polar_fill((long)r * m, host.data(), o.seed, hv, cv);
     HQ_RET(ensure(h->Gcache, sizeof(double) * (size_t)r * m));
     HQ_TRY(hipMemcpy(h->Gcache.p, host.data(), sizeof(double) * host.size(),
                      hipMemcpyHostToDevice));
     h->om_r = r;
     h->om_m = m;
     h->om_seed = o.seed;
+    h->om_have = in_have;
+    h->om_in = in_value;
+    h->om_out_have = hv;
+    h->om_out = cv;
   }
+  g_carry_have = h->om_out_have;
+  g_carry_value = h->om_out;
   HQ_TRY(launch_transpose(h->Gcache.as<double>(), r, r, m, h->G.as<double>(),
                           m, h->cur));
   return HQRRP_OK;

@@ -182,10 +182,27 @@ __global__ __launch_bounds__(HQ_NT) void amax_kernel(const double* A, long lda,
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  // One atomic per workgroup (the four wavefront maxima combined in LDS):
+  // thousands of workgroups updating one word is what this pass waits on.
+  __shared__ double s_mx[HQ_NT / 64];
+  __shared__ int s_bad[HQ_NT / 64];
   const bool any_bad = __any(bad);
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&w[0], (unsigned long long)__double_as_longlong(mx));
-    if (any_bad) atomicOr(&w[1], 1ull);
+    s_mx[threadIdx.x >> 6] = mx;
+    s_bad[threadIdx.x >> 6] = any_bad ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m4 = s_mx[0];
+    int b4 = s_bad[0];
+#pragma unroll
+    for (int q = 1; q < HQ_NT / 64; ++q) {
+      m4 = fmax(m4, s_mx[q]);
+      b4 |= s_bad[q];
+    }
+    // Non-negative doubles order like their bit patterns.
+    atomicMax(&w[0], (unsigned long long)__double_as_longlong(m4));
+    if (b4) atomicOr(&w[1], 1ull);
   }
 }
 
@@ -235,7 +252,9 @@ hipError_t launch_scale_pre(double* A, long lda, int m, int n,
   if (m <= 0 || n <= 0) return hipSuccess;
   hipLaunchKernelGGL(scale_init_kernel, dim3(1), dim3(64), 0, s, w);
   dim3 grid(hq_cdiv(m, kRowsPerBlock), n < 2048 ? n : 2048);
-  hipLaunchKernelGGL(amax_kernel, grid, dim3(HQ_NT), 0, s, A, lda, m, n, w);
+  // The max pass loops over more columns per workgroup (fewer atomics).
+  dim3 mgrid(grid.x, n < 256 ? n : 256);
+  hipLaunchKernelGGL(amax_kernel, mgrid, dim3(HQ_NT), 0, s, A, lda, m, n, w);
   hipLaunchKernelGGL(scale_decide_kernel, dim3(1), dim3(64), 0, s, w, lo, hi);
   hipLaunchKernelGGL(scale_apply_kernel, grid, dim3(HQ_NT), 0, s, A, lda, m, n,
                      0, w, 1);

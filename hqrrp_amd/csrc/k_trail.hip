@@ -222,7 +222,7 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 // MFMA substep e (0..3) lane l supplies row 4*(l/16) + e of the chunk for
 // both operands, so every lane reads 4 consecutive doubles (32 bytes) of one
 // column of C and of V per chunk, and 16 lanes cover a column's 128-byte
-// line.  A three-chunk register ring keeps the next two chunks' loads in
+// line.  A four-chunk register ring keeps the next three chunks' loads in
 // flight while the matrix cores work on the current one.  VEC: the 32-byte groups are
 // 32-byte aligned (vector loads); otherwise the same values are loaded one
 // double at a time, so both forms compute identical bits.

@@ -158,6 +158,22 @@ int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
  * rand() state.  (NoFLA_HQRRP_WY_blk_var4.c:380, 575-614) */
 void hqrrp_reference_omega(int r, int m, double *G, unsigned seed);
 
+/* The same generator started from a given cached second variate (*have,
+ * *cached) and returning the one it leaves: the reference's generator keeps
+ * it in function statics (NoFLA_HQRRP_WY_blk_var4.c:594-602) that srand()
+ * does not reset, so a call drawing an odd number r*m of normals hands its
+ * last pair's second value to the next call. */
+void hqrrp_reference_omega_carry(int r, int m, double *G, unsigned seed,
+                                 int *have, double *cached);
+
+/* The library's process-wide copy of that state, read and advanced by every
+ * factorisation that draws the reference Omega (HQRRP_OMEGA_REFERENCE:
+ * hqrrp_factor_dev, NoFLA_HQRRP_WY_blk_var4, dgeqp4's HQRRP branch), as the
+ * reference's statics are by its calls.  A new process starts with none
+ * (have = 0); _set restores any state, e.g. that of a fresh process. */
+void hqrrp_omega_carry_get(int *have, double *value);
+void hqrrp_omega_carry_set(int have, double value);
+
 /* ------------------------------- Q, fixed-pivot QR and its adjoint (device)
  * Device pointers, stream-ordered on the handle's stream. */
 

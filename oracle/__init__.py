@@ -90,11 +90,21 @@ class RandStream:
         return int(lib().orc_rand_next(self._st))
 
 
-def omega(r: int, m: int, seed: int = 12) -> np.ndarray:
-    """The reference's Gaussian sketch matrix G (r x m, column-major)."""
+def omega(r: int, m: int, seed: int = 12, carry=None):
+    """The reference's Gaussian sketch matrix G (r x m, column-major), as the
+    first call of a process draws it; with carry=(have, value) -- the
+    generator's statics as a later call finds them -- returns (G, the
+    statics the draw leaves)."""
     G = np.zeros((r, m), dtype=np.float64, order="F")
-    lib().orc_omega(C.c_int(r), C.c_int(m), _dp(G), C.c_int(r), C.c_uint(seed))
-    return G
+    if carry is None:
+        lib().orc_omega(C.c_int(r), C.c_int(m), _dp(G), C.c_int(r),
+                        C.c_uint(seed))
+        return G
+    h = C.c_int(int(carry[0]))
+    v = C.c_double(float(carry[1]))
+    lib().orc_omega_carry(C.c_int(r), C.c_int(m), _dp(G), C.c_int(r),
+                          C.c_uint(seed), C.byref(h), C.byref(v))
+    return G, (h.value, v.value)
 
 
 def ref_omega(r: int, m: int) -> np.ndarray:

@@ -25,6 +25,7 @@
  * Parity status of each function (what pins it):
  *   orc_rand_*         glibc rand() known answers after srand(12)   (pinned)
  *   orc_omega          reference Omega known answers                (pinned)
+ *   orc_omega_carry    compiled reference, two calls in one process  (pinned)
  *   orc_qrpmod         compiled reference helper, bitwise           (pinned)
  *   orc_downdate_y     compiled reference helper, bitwise;          (pinned)
  *                      invariant Y_R = G_R * A_R
@@ -229,14 +230,24 @@ static double orc_polar(orc_rand_state *s, double *cache, int *have) {
   return 0.0 + 1.0 * (c1 * factor);
 }
 
-void orc_omega(int r, int m, double *G, int ldg, unsigned int seed) {
+/* Omega as a call finds the generator's statics (*have, *cache) and leaves
+ * them (WY.c:594-595: they survive the srand(12) of every call, WY.c:380);
+ * NULL pointers: a fresh process. */
+void orc_omega_carry(int r, int m, double *G, int ldg, unsigned int seed,
+                     int *have, double *cache) {
   orc_rand_state s;
-  double cache = 0.0;
-  int have = 0;
+  double c = (have && *have && cache) ? *cache : 0.0;
+  int h = (have && *have) ? 1 : 0;
   orc_rand_seed(&s, seed);
   for (int j = 0; j < m; ++j)
     for (int i = 0; i < r; ++i)
-      G[i + (size_t)j * ldg] = orc_polar(&s, &cache, &have);
+      G[i + (size_t)j * ldg] = orc_polar(&s, &c, &h);
+  if (have) *have = h;
+  if (cache) *cache = c;
+}
+
+void orc_omega(int r, int m, double *G, int ldg, unsigned int seed) {
+  orc_omega_carry(r, m, G, ldg, seed, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -108,3 +108,27 @@ def test_dgeqp4_device_failure_is_not_an_argument_code():
         pytest.skip("a device is present")
     info, _ = _call_dgeqp4(1200, 1000, 1200, 128 * 1000)
     assert info == 1000 - (-3), info  # HQRRP_ERR_HIP
+
+
+def test_omega_carry_host_functions():
+    """hqrrp_reference_omega_carry against the oracle's restatement of the
+    reference's generator statics (itself pinned against the compiled
+    reference in test_oracle.py), and the process-wide state's get/set."""
+    import oracle
+
+    r, m = 21, 301  # r * m odd: the draw leaves a cached variate
+    G1, c1 = hqrrp_amd.reference_omega(r, m, carry=(0, 0.0))
+    assert np.array_equal(G1, hqrrp_amd.reference_omega(r, m))
+    assert np.array_equal(G1, oracle.omega(r, m))
+    H1, d1 = oracle.omega(r, m, carry=(0, 0.0))
+    assert c1 == d1 and c1[0] == 1
+    G2, c2 = hqrrp_amd.reference_omega(r, m, carry=c1)
+    H2, d2 = oracle.omega(r, m, carry=d1)
+    assert np.array_equal(G2, H2) and c2 == d2 and c2[0] == 0
+    # the carried draw starts with the cached variate, then the stream
+    assert G2[0, 0] == c1[1]
+    assert np.array_equal(G2.ravel(order="F")[1:], G1.ravel(order="F")[:-1])
+    hqrrp_amd.set_omega_carry(*c1)
+    assert hqrrp_amd.omega_carry() == c1
+    hqrrp_amd.set_omega_carry()
+    assert hqrrp_amd.omega_carry() == (0, 0.0)

@@ -168,6 +168,34 @@ def test_deterministic(handle):
     assert np.array_equal(j1, j2) and np.array_equal(a1, a2)
 
 
+def test_omega_carry_across_calls(handle):
+    """Three calls in a row with an odd r*m (nb = 16, pp = 5, m = 1301): as
+    in the reference, whose generator statics carry the last pair's second
+    variate into the next call, the second call factors with the shifted
+    Omega and the third with the first call's again.  Each call against the
+    oracle with that call's Omega (the oracle's carried Omega is pinned
+    against the compiled reference on the CPU)."""
+    import hqrrp_amd
+
+    m, n, nb, pp = 1301, 1100, 16, 5
+    A = matgen.gaussian(m, n, 57)
+    r = nb + pp
+    G1, c1 = oracle.omega(r, m, carry=(0, 0.0))
+    G2, c2 = oracle.omega(r, m, carry=c1)
+    assert hqrrp_amd.omega_carry() == (0, 0.0)
+    want = []
+    for G in (G1, G2):
+        want.append(oracle.hqrrp(A, nb, pp, 1, omega_in=G))
+    for call, (G, carry) in enumerate(((G1, c1), (G2, c2), (G1, c1))):
+        print(f"call {call + 1}: carry in {hqrrp_amd.omega_carry()}", flush=True)
+        Ag, jg, tg = run_gpu(handle, A, nb, pp, 1)
+        assert hqrrp_amd.omega_carry() == carry
+        Ar, jr, tr = want[call % 2]
+        assert np.array_equal(jg, jr), f"call {call + 1}: pivots differ"
+        dr, dg = np.abs(np.diag(Ar)), np.abs(np.diag(Ag))
+        assert (np.abs(dg - dr) / dr).max() <= 1e-10
+
+
 LOOKAHEAD_CASES = [
     # (m, n, nb, pp, panel_pivoting, max_rank, seed)
     (1300, 1300, 64, 10, 1, 0, 41),

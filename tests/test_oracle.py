@@ -283,3 +283,44 @@ def test_dormqr_equals_explicit_q(side, trans):
         Cm @ (Q if trans == "T" else Q.T)
     assert np.abs(got - want).max() <= 1e-13
     assert np.abs(got - wrong).max() > 1e-3
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built here")
+def test_omega_carry_matches_reference_statics(tmp_path):
+    """The reference's normal generator keeps its second variate in function
+    statics (WY.c:594-595) that the srand(12) of every call (WY.c:380) does
+    not reset: with an odd r*m the second call of a process starts from the
+    first call's cached variate.  A fresh interpreter runs the compiled
+    reference twice (its Omega, then the whole driver); the oracle's carried
+    Omega reproduces every call bitwise."""
+    import subprocess
+    import sys
+
+    m, n, nb, pp = 301, 200, 16, 5  # r * m = 21 * 301, odd
+    out = tmp_path / "ref.npz"
+    code = (
+        "import numpy as np, oracle\n"
+        f"A = np.random.default_rng(17).standard_normal(({m}, {n}))\n"
+        f"g1 = oracle.ref_omega({nb + pp}, {m}); g2 = oracle.ref_omega({nb + pp}, {m})\n"
+        f"r1 = oracle.hqrrp(A, {nb}, {pp}, 1, use_ref=True)\n"
+        f"r2 = oracle.hqrrp(A, {nb}, {pp}, 1, use_ref=True)\n"
+        f"np.savez(r'{out}', g1=g1, g2=g2, a1=r1[0], j1=r1[1], t1=r1[2],"
+        " a2=r2[0], j2=r2[1], t2=r2[2])\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", code], cwd=root, check=True,
+                   timeout=120)
+    d = np.load(out)
+    r = nb + pp
+    G1, c1 = oracle.omega(r, m, carry=(0, 0.0))
+    G2, c2 = oracle.omega(r, m, carry=c1)
+    assert c1[0] == 1 and c2[0] == 0
+    assert np.array_equal(d["g1"], G1) and np.array_equal(d["g2"], G2)
+    assert not np.array_equal(G1, G2)
+    A = np.random.default_rng(17).standard_normal((m, n))
+    # The two ref_omega calls left the statics empty again (two odd draws),
+    # so the driver calls see G1 and then G2.
+    for tag, G in (("1", G1), ("2", G2)):
+        a, j, t = oracle.hqrrp(A, nb, pp, 1, omega_in=G)
+        assert np.array_equal(a, d["a" + tag])
+        assert np.array_equal(j, d["j" + tag])
+        assert np.array_equal(t, d["t" + tag])
