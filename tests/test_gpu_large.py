@@ -1,6 +1,6 @@
 """The full-size factorizations (16384 x 16384 and ragged rectangles around
-10^4), checked through properties that do not need the oracle, which would
-take hours at these sizes:
+10^4), checked through size-independent properties (the headline's pivots
+are compared with the oracle itself in tests/test_gpu_headline.py):
 
 * jpvt is a permutation of 1..n;
 * ||AP - QR||_F / ||A||_F and max|Q'Q - I| <= 20 sqrt(max(m, n)) eps (the
