@@ -49,6 +49,10 @@ SIGNATURES = {
     "dgeqp4": (None, [_i, _i, _vp, _i, _vp, _vp, _vp, _i, _i]),
     "NoFLA_HQRRP_WY_blk_var4": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp,
                                           _vp, C.c_int, C.c_int, C.c_int]),
+    "hqrrp_ut_blk_var2": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp,
+                                    C.c_int, C.c_int, C.c_int]),
+    "hqrrp_ut_factor_dev": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int,
+                                      _vp, _vp, C.POINTER(Options)]),
     "hqrrp_default_options": (None, [C.POINTER(Options)]),
     "hqrrp_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "hqrrp_destroy": (C.c_int, [_vp]),

@@ -91,6 +91,20 @@ def hqrrp(A, nb_alg=64, pp=10, panel_pivoting=1, jpvt=None):
     return A, jp, tau[:min(m, n)]
 
 
+def hqrrp_ut(A, nb_alg=64, pp=10, panel_pivoting=1, p=None):
+    """The libflame UT-variant front end (FLA_HQRRP_UT_blk_var2) on a copy
+    of A; returns (A, p, s) in libflame's UT convention."""
+    A = np.asfortranarray(np.array(A, dtype=np.float64, copy=True))
+    m, n = A.shape
+    jp = (np.arange(1, n + 1, dtype=np.int32) if p is None
+          else np.array(p, dtype=np.int32, copy=True))
+    s = np.zeros(max(1, min(m, n)))
+    rc = lib().hqrrp_ut_blk_var2(m, n, _ptr(A), max(1, m), _ptr(jp), _ptr(s),
+                                 nb_alg, pp, panel_pivoting)
+    check(rc, "hqrrp_ut_blk_var2")
+    return A, jp, s[:min(m, n)]
+
+
 # ------------------------------------------------------------ device side
 def to_colmajor(A, device="cuda"):
     """numpy/torch m x n -> contiguous torch (n, m) = column-major m x n."""
@@ -218,6 +232,19 @@ def hqrrp_device(handle: Handle, At, jpvt_t, tau_t, m: int, n: int,
                                 C.c_void_p(jpvt_t.data_ptr()),
                                 C.c_void_p(tau_t.data_ptr()), C.byref(o))
     check(rc, "hqrrp_factor_dev")
+
+
+def hqrrp_ut_device(handle: Handle, At, p_t, s_t, m: int, n: int,
+                    lda: int | None = None, **kw) -> None:
+    """hqrrp_ut_factor_dev: hqrrp_device in libflame's UT convention (s_t
+    receives s = 1/tau, 1/2 where LAPACK's tau is 0, with R's row negated
+    there)."""
+    o = options(**kw)
+    rc = lib().hqrrp_ut_factor_dev(handle.ptr, m, n, C.c_void_p(At.data_ptr()),
+                                   m if lda is None else lda,
+                                   C.c_void_p(p_t.data_ptr()),
+                                   C.c_void_p(s_t.data_ptr()), C.byref(o))
+    check(rc, "hqrrp_ut_factor_dev")
 
 
 # ------------------------------------- Q and the fixed-pivot QR with adjoint

@@ -1038,6 +1038,18 @@ TrailArgs below_r12(const TrailArgs& ta) {
 int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                 long lda, int* djpvt, double* dtau, const hqrrp_options& o);
 
+// Columns a factorisation truncated at kmax leaves factored: blocks of nb
+// up to kmax, the block that would cross kmax cut there, except the
+// reference's final block (j + nb >= m or n), which is factored whole.
+int factored_columns(int m, int n, int nb, int kmax) {
+  const int mn = std::min(m, n);
+  for (int j = 0; j < mn; j += nb) {
+    if (j + nb >= m || j + nb >= n) break;
+    if (j + nb >= kmax) return kmax;
+  }
+  return mn;
+}
+
 int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
                   int* djpvt, double* dtau, const hqrrp_options& o) {
   const int mn = std::min(m, n);
@@ -1058,16 +1070,8 @@ int factor_locked(hqrrp_handle_t h, int m, int n, double* dA, long lda,
   const int lim = h->sk32 ? 60 : 256;
   HQ_TRY(launch_scale_pre(dA, lda, m, n, h->scale_words(), -lim, lim, h->cur));
   HQ_RET(factor_loop(h, m, n, kmax, dA, lda, djpvt, dtau, o));
-  // Columns factored: blocks of nb up to kmax, the final block whole.
-  int kdone = mn;
-  for (int j = 0; j < mn; j += o.nb) {
-    if (j + o.nb >= m || j + o.nb >= n) break;
-    if (j + o.nb >= kmax) {
-      kdone = kmax;
-      break;
-    }
-  }
-  HQ_TRY(launch_scale_post(dA, lda, m, n, kdone, h->scale_words(), h->cur));
+  HQ_TRY(launch_scale_post(dA, lda, m, n, factored_columns(m, n, o.nb, kmax),
+                           h->scale_words(), h->cur));
   return HQRRP_OK;
 }
 
@@ -1771,6 +1775,39 @@ extern "C" int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double* dA,
   return factor_locked(h, m, n, dA, lda, djpvt, dtau, o);
 }
 
+// The libflame UT-variant front end (FLA_HQRRP_UT_blk_var2,
+// libflame_sources/FLA_HQRRP_UT_blk_var2.c:124-401): the same algorithm,
+// its Householder transforms in libflame's UT form.  The factorisation runs
+// as HQRRP (WY form) and is then rewritten: s = 1/tau, and each step where
+// nothing remained below the diagonal (tau = 0 here; libflame's
+// FLA_Househ2_UT then negates alpha and sets tau = 1/2, the reflection
+// I - 2 e e') has its row of R negated.  Same pivots, same Householder
+// vectors below the diagonal.
+int ut_rewrite(hqrrp_handle_t h, int m, int n, double* dA, long lda,
+               double* ds, const hqrrp_options& o) {
+  const int mn = std::min(m, n);
+  if (mn == 0) return HQRRP_OK;
+  const int kmax = (o.max_rank > 0) ? std::min(mn, o.max_rank) : mn;
+  HQ_TRY(launch_ut_convert(dA, lda, m, n, factored_columns(m, n, o.nb, kmax),
+                           ds, h->stream));
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_ut_factor_dev(hqrrp_handle_t h, int m, int n, double* dA,
+                                   int lda, int* dp, double* ds,
+                                   const hqrrp_options* opt) {
+  if (!h || m < 0 || n < 0 || lda < std::max(1, m)) return HQRRP_ERR_ARG;
+  hqrrp_options o;
+  if (opt)
+    o = *opt;
+  else
+    hqrrp_default_options(&o);
+  if (o.nb < 1 || o.pp < 0) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  HQ_RET(factor_locked(h, m, n, dA, lda, dp, ds, o));
+  return ut_rewrite(h, m, n, dA, lda, ds, o);
+}
+
 extern "C" int hqrrp_sketch_qrcp_dev(hqrrp_handle_t h, int r, int ncols,
                                      const double* dY, int ldy, int stages,
                                      int* dpiv, int* dmoves) {
@@ -2232,8 +2269,9 @@ extern "C" int hqrrp_batch_destroy(hqrrp_batch_t b) {
 namespace {
 
 // Host-pointer HQRRP on the default handle of the current device.
+// ut: libflame's UT form of the result (hqrrp_ut_blk_var2).
 int host_hqrrp(int m, int n, double* A, int lda, int* jpvt, double* tau,
-               const hqrrp_options& o) {
+               const hqrrp_options& o, bool ut = false) {
   hqrrp_handle_t h = nullptr;
   HQ_RET(default_handle(&h));
   std::lock_guard<std::mutex> lk(h->mu);
@@ -2248,6 +2286,7 @@ int host_hqrrp(int m, int n, double* A, int lda, int* jpvt, double* tau,
   HQ_TRY(hipMemcpyAsync(h->hJ.p, jpvt, sizeof(int) * n, hipMemcpyHostToDevice,
                         h->stream));
   HQ_RET(factor_locked(h, m, n, dA, m, h->hJ.as<int>(), h->hT.as<double>(), o));
+  if (ut) HQ_RET(ut_rewrite(h, m, n, dA, m, h->hT.as<double>(), o));
   HQ_RET(check_err(h));
   HQ_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * m,
                           sizeof(double) * m, n, hipMemcpyDeviceToHost,
@@ -2399,6 +2438,24 @@ extern "C" int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double* buff_A,
   o.pp = pp;
   o.panel_pivoting = panel_pivoting;
   return host_hqrrp(m_A, n_A, buff_A, ldim_A, buff_jpvt, buff_tau, o);
+}
+
+extern "C" int hqrrp_ut_blk_var2(int m_A, int n_A, double* buff_A,
+                                 int ldim_A, int* buff_p, double* buff_s,
+                                 int nb_alg, int pp, int panel_pivoting) {
+  // FLA_HQRRP_UT_blk_var2 (FLA_HQRRP_UT_blk_var2.c:124-401) with its FLA_Obj
+  // arguments unpacked (buffer, length, width, column stride).  It has no
+  // argument checks of its own (FLA_Obj dimensions cannot be illegal);
+  // these reject what a caller could not have passed in an FLA_Obj.
+  if (m_A < 0 || n_A < 0 || ldim_A < std::max(1, m_A)) return HQRRP_ERR_ARG;
+  if (std::min(m_A, n_A) == 0) return 0;  // quick return, UT.c:177-180
+  if (nb_alg < 1 || pp < 0) return HQRRP_ERR_ARG;
+  hqrrp_options o;
+  hqrrp_default_options(&o);
+  o.nb = nb_alg;
+  o.pp = pp;
+  o.panel_pivoting = panel_pivoting;
+  return host_hqrrp(m_A, n_A, buff_A, ldim_A, buff_p, buff_s, o, true);
 }
 
 extern "C" void dgeqp4(int* m, int* n, double* A, int* lda, int* jpvt,

@@ -6,6 +6,8 @@
 // swaps are composed on the device into at most 2b (dst, src) moves and
 // applied in two coalesced passes: every source column is copied out to a
 // staging buffer, then every staging column is written to its destination.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -244,7 +246,36 @@ __global__ __launch_bounds__(HQ_NT) void scale_apply_kernel(
   }
 }
 
+// Row k of R negated (columns >= k) for every reflector k < nk with
+// tau = 0; one thread per column.
+__global__ __launch_bounds__(HQ_NT) void ut_rows_kernel(double* A, long lda,
+                                                        int n, int nk,
+                                                        const double* tau) {
+  const long c = (long)blockIdx.x * HQ_NT + threadIdx.x;
+  if (c >= n) return;
+  const int last = (int)min((long)nk - 1, c);
+  for (int k = 0; k <= last; ++k)
+    if (tau[k] == 0.0) A[k + c * lda] = -A[k + c * lda];
+}
+
+// tau -> the UT transform's s (1/tau; 1/2 for tau = 0).
+__global__ __launch_bounds__(HQ_NT) void ut_tau_kernel(double* tau, int nk) {
+  const int k = blockIdx.x * HQ_NT + threadIdx.x;
+  if (k < nk) tau[k] = (tau[k] == 0.0) ? 0.5 : 1.0 / tau[k];
+}
+
 }  // namespace
+
+hipError_t launch_ut_convert(double* A, long lda, int m, int n, int k,
+                             double* tau, hipStream_t s) {
+  const int nk = std::min(k, std::min(m, n));
+  if (nk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ut_rows_kernel, dim3(hq_cdiv(n, HQ_NT)), dim3(HQ_NT), 0,
+                     s, A, lda, n, nk, tau);
+  hipLaunchKernelGGL(ut_tau_kernel, dim3(hq_cdiv(nk, HQ_NT)), dim3(HQ_NT), 0,
+                     s, tau, nk);
+  return hipGetLastError();
+}
 
 hipError_t launch_scale_pre(double* A, long lda, int m, int n,
                             unsigned long long* w, int lo, int hi,

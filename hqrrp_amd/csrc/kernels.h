@@ -235,6 +235,14 @@ hipError_t launch_scale_pre(double* A, long lda, int m, int n,
                             hipStream_t s);
 hipError_t launch_scale_post(double* A, long lda, int m, int n, int k,
                              const unsigned long long* w, hipStream_t s);
+// The libflame UT-transform form of a factorisation's first k reflectors
+// (LAPACK's H = I - tau v v' becomes H = I - u u' / s with u = v): s =
+// 1 / tau, and where LAPACK's dlarfg found nothing below the diagonal
+// (tau = 0, H = I) the UT Householder is the reflection I - 2 e e' (s =
+// 1/2), so row k of R (columns k..n-1) changes sign.  In place on A and
+// on tau (which becomes s).
+hipError_t launch_ut_convert(double* A, long lda, int m, int n, int k,
+                             double* tau, hipStream_t s);
 // p[i] = i for i < n.
 hipError_t launch_iota(int* p, int n, hipStream_t s);
 // The moves of a permutation (perm[pos] = the position whose column ends at

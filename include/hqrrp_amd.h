@@ -62,6 +62,23 @@ int NoFLA_HQRRP_WY_blk_var4(int m_A, int n_A, double *buff_A, int ldim_A,
                             int *buff_jpvt, double *buff_tau, int nb_alg,
                             int pp, int panel_pivoting);
 
+/* libflame UT-variant front end.  Replaces the reference's
+ *   int FLA_HQRRP_UT_blk_var2(FLA_Obj A, FLA_Obj p, FLA_Obj s, int nb_alg,
+ *        int pp, int panel_pivoting)
+ * (libflame_sources/FLA_HQRRP_UT_blk_var2.h:3-4, FLA_HQRRP_UT_blk_var2.c:
+ * 124-401) with each FLA_Obj unpacked into its buffer and sizes: A (m_A x
+ * n_A, column stride ldim_A), p (n_A pivots, permuted in place; the caller
+ * initialises them), s (min(m_A, n_A) values).  Host pointers.  The same
+ * factorisation as NoFLA_HQRRP_WY_blk_var4 with libflame's UT Householder
+ * convention: H_k = I - u_k u_k' / s_k (u_k: 1 on the diagonal, the stored
+ * vector below), s_k = 1 / tau_k; where nothing remains below the diagonal
+ * (tau_k = 0 in LAPACK's convention, H_k = I) libflame reflects instead,
+ * H_k = I - 2 e_k e_k', s_k = 1/2, with row k of R negated.  Returns 0 or a
+ * negative hqrrp_status. */
+int hqrrp_ut_blk_var2(int m_A, int n_A, double *buff_A, int ldim_A,
+                      int *buff_p, double *buff_s, int nb_alg, int pp,
+                      int panel_pivoting);
+
 /* -------------------------------------------- 2. device-pointer interface */
 
 typedef struct hqrrp_handle_s *hqrrp_handle_t;
@@ -152,6 +169,12 @@ int hqrrp_profile_read(hqrrp_handle_t h, int nphase, double *ms,
  * dA (m x n, ld lda), djpvt (n, permuted in place), dtau (min(m,n)). */
 int hqrrp_factor_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
                      int *djpvt, double *dtau, const hqrrp_options *opt);
+
+/* Device form of hqrrp_ut_blk_var2 (the libflame UT-variant front end,
+ * FLA_HQRRP_UT_blk_var2.c:124-401): hqrrp_factor_dev followed by the
+ * rewrite into libflame's UT convention; ds (min(m,n)) receives s. */
+int hqrrp_ut_factor_dev(hqrrp_handle_t h, int m, int n, double *dA, int lda,
+                        int *dp, double *ds, const hqrrp_options *opt);
 
 /* Host generation of the reference's Omega (r x m, column-major, ld r) with
  * the stream srand(seed) + polar method, without touching the caller's

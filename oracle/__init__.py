@@ -154,6 +154,35 @@ def hqrrp(A, nb=64, pp=10, panel_pivoting=1, jpvt=None, omega_in=None,
     return A, jp, tau[:mn]
 
 
+def hqrrp_ut(A, nb=64, pp=10, panel_pivoting=1, p=None, carry=None):
+    """The libflame UT variant (FLA_HQRRP_UT_blk_var2) restated; returns
+    (A_out, p (1-based), s) in libflame's UT convention
+    (H_k = I - u_k u_k' / s_k)."""
+    A = fortran(A)
+    m, n = A.shape
+    jp = (np.arange(1, n + 1, dtype=np.int32) if p is None
+          else np.array(p, dtype=np.int32))
+    sv = np.zeros(max(min(m, n), 1))
+    h = C.c_int(int(carry[0]) if carry else 0)
+    v = C.c_double(float(carry[1]) if carry else 0.0)
+    lib().orc_hqrrp_ut(C.c_int(m), C.c_int(n), _dp(A), C.c_int(max(1, m)),
+                       _ip(jp), _dp(sv), C.c_int(nb), C.c_int(pp),
+                       C.c_int(panel_pivoting), C.byref(h), C.byref(v))
+    return A, jp, sv[:min(m, n)]
+
+
+def ut_q(Af, s):
+    """Q = I - U inv(T) U' of the UT transform (U unit lower trapezoidal
+    from Af, T upper triangular with diag(T) = s and T(i, j) = u_i'u_j)."""
+    m, n = Af.shape
+    k = len(s)
+    U = np.tril(Af[:, :k], -1)
+    U[np.arange(k), np.arange(k)] = 1.0
+    T = np.triu(U.T @ U, 1)
+    T[np.arange(k), np.arange(k)] = s
+    return np.eye(m) - U @ np.linalg.solve(T, U.T)
+
+
 def dgeqp4(A, jpvt=None, lwork=None, use_ref=False):
     """The dgeqp3-compatible front end; returns (A, jpvt, tau, info, work0)."""
     A = fortran(A)

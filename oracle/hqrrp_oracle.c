@@ -95,6 +95,22 @@ typedef void (*dorgqr_t)(const int *, const int *, const int *, double *,
                          const int *, const double *, double *, const int *,
                          int *);
 typedef void (*setthreads_t)(int);
+typedef void (*dtrsm_t)(const char *, const char *, const char *, const char *,
+                        const int *, const int *, const double *,
+                        const double *, const int *, double *, const int *,
+                        flen, flen, flen, flen);
+typedef void (*dgemv_t)(const char *, const int *, const int *,
+                        const double *, const double *, const int *,
+                        const double *, const int *, const double *, double *,
+                        const int *, flen);
+typedef void (*dger_t)(const int *, const int *, const double *,
+                       const double *, const int *, const double *,
+                       const int *, double *, const int *);
+typedef void (*dcopy_t)(const int *, const double *, const int *, double *,
+                        const int *);
+typedef void (*daxpy_t)(const int *, const double *, const double *,
+                        const int *, double *, const int *);
+typedef void (*dscal_t)(const int *, const double *, double *, const int *);
 
 static struct {
   void *h;
@@ -117,6 +133,12 @@ static struct {
   ilaenv_t ilaenv;
   xerbla_t xerbla;
   dorgqr_t dorgqr;
+  dtrsm_t dtrsm;
+  dgemv_t dgemv;
+  dger_t dger;
+  dcopy_t dcopy;
+  daxpy_t daxpy;
+  dscal_t dscal;
   setthreads_t set_threads;
 } LP;
 
@@ -164,6 +186,12 @@ int orc_init(const char *libpath) {
   GET(ilaenv, "ilaenv_");
   GET(xerbla, "xerbla_");
   GET(dorgqr, "dorgqr_");
+  GET(dtrsm, "dtrsm_");
+  GET(dgemv, "dgemv_");
+  GET(dger, "dger_");
+  GET(dcopy, "dcopy_");
+  GET(daxpy, "daxpy_");
+  GET(dscal, "dscal_");
 #undef GET
   *(void **)&LP.set_threads = orc_sym("openblas_set_num_threads");
   return 0;
@@ -643,4 +671,288 @@ void orc_dormqr(const char *side, const char *trans, int *m, int *n, int *k,
 void orc_dlarft(int n, int k, const double *V, int ldv, const double *tau,
                 double *T, int ldt) {
   LP.dlarft("Forward", "Columnwise", &n, &k, V, &ldv, tau, T, &ldt, 7, 10);
+}
+
+/* ------------------------------------------------------------------------- */
+/* The libflame UT variant, FLA_HQRRP_UT_blk_var2                             */
+/* (libflame_sources/FLA_HQRRP_UT_blk_var2.c, below UT.c), restated on plain  */
+/* column-major arrays.  Its FLA_Obj views become pointers and sizes; its     */
+/* libflame calls (FLA_Gemm, FLA_Trmm, FLA_Trsm, FLA_Copy, FLA_Axpy) become   */
+/* the BLAS calls they perform.  Its Householder vectors come from libflame's */
+/* FLA_Househ2_UT_l_opd, which is not in the reference tree (libflame is a    */
+/* third-party library; the reference pins no version); its published        */
+/* algorithm is restated in orc_househ2_ut.  libflame itself being absent,    */
+/* this driver is pinned by the mathematics of the UT transform and by the    */
+/* WY driver above (tests/test_oracle.py), not by a compiled reference.       */
+
+/* FLA_Househ2_UT_l_opd: chi_1 := alpha = -sign(chi_1) ||[chi_1; x2]||,
+ * x2 := u2 = x2 / (chi_1 - alpha), tau := (1 + u2'u2) / 2, so that
+ * (I - [1; u2][1; u2]' / tau) [chi_1; x2] = [alpha; 0]; when x2 = 0 it
+ * returns chi_1 := -chi_1, tau := 1/2 (the reflection I - 2 e1 e1'). */
+static void orc_househ2_ut(int m_x2, double *chi_1, double *x2, double *tau) {
+  static const int one = 1;
+  const double norm_x_2 = m_x2 > 0 ? LP.dnrm2(&m_x2, x2, &one) : 0.0;
+  if (norm_x_2 == 0.0) {
+    *chi_1 = -(*chi_1);
+    *tau = 0.5;
+    return;
+  }
+  const int two = 2;
+  double y[2] = {fabs(*chi_1), norm_x_2};
+  const double norm_x = LP.dnrm2(&two, y, &one);
+  const double alpha = (*chi_1 < 0.0 ? 1.0 : -1.0) * norm_x;
+  const double cma = *chi_1 - alpha;
+  const double inv = 1.0 / cma;
+  LP.dscal(&m_x2, &inv, x2, &one);
+  const double r = norm_x_2 / fabs(cma);
+  *tau = 0.5 + 0.5 * (r * r);
+  *chi_1 = alpha;
+}
+
+/* MyFLA_Apply_H2_UT_l_opd_var2 (UT.c:826-928): [a1t; A2] := H [a1t; A2]. */
+static void orc_apply_h2_ut(int m_u2, int n_a1t, const double *tau,
+                            const double *u2, double *a1t, int inc_a1t,
+                            double *A2, int lda2, double *w) {
+  static const int one = 1;
+  static const double d1 = 1.0, dm1 = -1.0;
+  if (n_a1t == 0) return;
+  LP.dcopy(&n_a1t, a1t, &inc_a1t, w, &one);
+  if (m_u2 > 0)
+    LP.dgemv("Transpose", &m_u2, &n_a1t, &d1, A2, &lda2, u2, &one, &d1, w, &one,
+             1);
+  const double rtau = 1.0 / *tau; /* tau >= 1/2 in this convention */
+  LP.dscal(&n_a1t, &rtau, w, &one);
+  LP.daxpy(&n_a1t, &dm1, w, &one, a1t, &inc_a1t);
+  if (m_u2 > 0) LP.dger(&m_u2, &n_a1t, &dm1, u2, &one, w, &one, A2, &lda2);
+}
+
+/* NoFLA_my_idamax (UT.c:1123-1166): first index of the largest |d|. */
+static int orc_my_idamax(int n, const double *d) {
+  if (n < 1) return -1;
+  int res = 0;
+  double dmax = fabs(d[0]);
+  for (int i = 1; i < n; ++i)
+    if (dmax < fabs(d[i])) {
+      res = i;
+      dmax = fabs(d[i]);
+    }
+  return res;
+}
+
+/* MyFLA_QRPmod_UT_unb_var2 (UT.c:633-823), with NoFLA_QRP_pivot_B_C
+ * (UT.c:986-1040), MyFLA_QRP_compute_norms (UT.c:931-983) and
+ * NoFLA_QRP_downdate_partial_norms (UT.c:1043-1120). */
+static void orc_qrpmod_ut(int pivoting, int num_stages, int m, int n,
+                          double *A, int lda, int *p, double *t, int pivot_B,
+                          int mB, double *B, int ldb, int pivot_C, int mC,
+                          double *Cm, int ldc, int build_T, double *T,
+                          int ldt) {
+  static const int one = 1;
+  static const double d1 = 1.0;
+  const int mn = OMIN(m, n);
+  if (num_stages < 0) num_stages = mn;
+  double *d = (double *)malloc(sizeof(double) * OMAX(1, n));
+  double *e = (double *)malloc(sizeof(double) * OMAX(1, n));
+  double *w = (double *)malloc(sizeof(double) * OMAX(1, n));
+  if (pivoting == 1)
+    for (int j = 0; j < n; ++j) {
+      d[j] = LP.dnrm2(&m, A + (size_t)j * lda, &one);
+      e[j] = d[j];
+    }
+  const double tol3z = sqrt(LP.dlamch("Epsilon", 7));
+  for (int j = 0; j < num_stages; ++j) {
+    const int n_dB = n - j, m_a21 = m - j - 1, n_A22 = n - j - 1;
+    if (pivoting == 1) {
+      const int q = orc_my_idamax(n_dB, d + j);
+      if (q != 0) {
+        LP.dswap(&m, A + (size_t)j * lda, &one, A + (size_t)(j + q) * lda,
+                 &one);
+        if (pivot_B)
+          LP.dswap(&mB, B + (size_t)j * ldb, &one, B + (size_t)(j + q) * ldb,
+                   &one);
+        if (pivot_C)
+          LP.dswap(&mC, Cm + (size_t)j * ldc, &one, Cm + (size_t)(j + q) * ldc,
+                   &one);
+        const int iv = p[j + q];
+        p[j + q] = p[j];
+        p[j] = iv;
+        d[j + q] = d[j];
+        e[j + q] = e[j];
+      }
+    }
+    double *a11 = A + j + (size_t)j * lda;
+    orc_househ2_ut(m_a21, a11, a11 + 1, t + j);
+    orc_apply_h2_ut(m_a21, n_A22, t + j, a11 + 1, A + j + (size_t)(j + 1) * lda,
+                    lda, A + (j + 1) + (size_t)(j + 1) * lda, lda, w);
+    if (build_T) {
+      T[j + (size_t)j * ldt] = t[j];
+      LP.dcopy(&j, A + j, &lda, T + (size_t)j * ldt, &one);
+      if (m_a21 > 0 && j > 0)
+        LP.dgemv("Transpose", &m_a21, &j, &d1, A + (j + 1), &lda, a11 + 1,
+                 &one, &d1, T + (size_t)j * ldt, &one, 1);
+    }
+    if (pivoting == 1) {
+      for (int c = 0; c < n_A22; ++c) {
+        double *dc = d + j + 1 + c, *ec = e + j + 1 + c;
+        if (*dc != 0.0) {
+          double temp = fabs(A[j + (size_t)(j + 1 + c) * lda]) / *dc;
+          temp = OMAX(0.0, (1.0 + temp) * (1 - temp));
+          const double temp5 = *dc / *ec;
+          const double temp2 = temp * temp5 * temp5;
+          if (temp2 <= tol3z) {
+            if (m_a21 > 0) {
+              *dc = LP.dnrm2(&m_a21, A + (j + 1) + (size_t)(j + 1 + c) * lda,
+                             &one);
+              *ec = *dc;
+            } else {
+              *dc = 0.0;
+              *ec = 0.0;
+            }
+          } else {
+            *dc = *dc * sqrt(temp);
+          }
+        }
+      }
+    }
+  }
+  free(d);
+  free(e);
+  free(w);
+}
+
+/* MyFLA_Apply_Q_UT_lhfc_blk_var2 (UT.c:521-573): [B1; B2] := Q' [B1; B2]
+ * with Q = I - U inv(T)' U'... as written there: W = inv(triu(T))' (U11'B1 +
+ * U21'B2), B2 -= U21 W, B1 -= U11 W.  U11 (b x b unit lower), U21 (mU x b),
+ * B1 (b x nB), B2 (mU x nB). */
+static void orc_apply_q_ut_lhfc(int b, int mU, int nB, const double *U11,
+                                int ldu, const double *U21, const double *T,
+                                int ldt, double *B1, double *B2, int ldb) {
+  static const double d1 = 1.0, dm1 = -1.0;
+  if (nB == 0 || b == 0) return;
+  double *W = (double *)malloc(sizeof(double) * (size_t)b * nB);
+  LP.dlacpy("All", &b, &nB, B1, &ldb, W, &b, 3);
+  LP.dtrmm("Left", "Lower", "Conjugate transpose", "Unit", &b, &nB, &d1, U11,
+           &ldu, W, &b, 4, 5, 19, 4);
+  if (mU > 0)
+    LP.dgemm("Conjugate transpose", "No transpose", &b, &nB, &mU, &d1, U21,
+             &ldu, B2, &ldb, &d1, W, &b, 19, 12);
+  LP.dtrsm("Left", "Upper", "Conjugate transpose", "Non-unit", &b, &nB, &d1, T,
+           &ldt, W, &b, 4, 5, 19, 8);
+  if (mU > 0)
+    LP.dgemm("No transpose", "No transpose", &mU, &nB, &b, &dm1, U21, &ldu, W,
+             &b, &d1, B2, &ldb, 12, 12);
+  LP.dtrmm("Left", "Lower", "No transpose", "Unit", &b, &nB, &dm1, U11, &ldu, W,
+           &b, 4, 5, 12, 4);
+  for (int c = 0; c < nB; ++c)
+    for (int i = 0; i < b; ++i) B1[i + (size_t)c * ldb] += W[i + (size_t)c * b];
+  free(W);
+}
+
+/* MyFLA_Apply_Q_UT_rnfc_blk_var2 (UT.c:576-630): [B1 B2] := [B1 B2] Q with
+ * W = inv(triu(T))' (U11'B1' + U21'B2'), B2 -= W'U21', B1 -= (U11 W)'.
+ * B1 (r x b), B2 (r x mU). */
+static void orc_apply_q_ut_rnfc(int b, int mU, int r, const double *U11,
+                                int ldu, const double *U21, const double *T,
+                                int ldt, double *B1, double *B2, int ldg) {
+  static const double d1 = 1.0, dm1 = -1.0;
+  double *W = (double *)malloc(sizeof(double) * (size_t)b * OMAX(1, r));
+  for (int i = 0; i < r; ++i)
+    for (int c = 0; c < b; ++c) W[c + (size_t)i * b] = B1[i + (size_t)c * ldg];
+  LP.dtrmm("Left", "Lower", "Transpose", "Unit", &b, &r, &d1, U11, &ldu, W, &b,
+           4, 5, 9, 4);
+  if (mU > 0)
+    LP.dgemm("Transpose", "Transpose", &b, &r, &mU, &d1, U21, &ldu, B2, &ldg,
+             &d1, W, &b, 9, 9);
+  LP.dtrsm("Left", "Upper", "Transpose", "Non-unit", &b, &r, &d1, T, &ldt, W, &b,
+           4, 5, 9, 8);
+  if (mU > 0)
+    LP.dgemm("Transpose", "Conjugate transpose", &r, &mU, &b, &dm1, W, &b, U21,
+             &ldu, &d1, B2, &ldg, 9, 19);
+  LP.dtrmm("Left", "Lower", "No transpose", "Unit", &b, &r, &dm1, U11, &ldu, W,
+           &b, 4, 5, 12, 4);
+  for (int i = 0; i < r; ++i)
+    for (int c = 0; c < b; ++c) B1[i + (size_t)c * ldg] += W[c + (size_t)i * b];
+  free(W);
+}
+
+/* MyFLA_Downdate_Y (UT.c:465-518): B = G1 - (G1 U11 + G2 U21) inv(T) U11',
+ * Y2 -= B R12, then [G1 G2] := [G1 G2] Q. */
+static void orc_downdate_y_ut(int b, int mU, int r, int nY2, const double *U11,
+                              int ldu, const double *U21, const double *R12,
+                              int ldr, const double *T, int ldt, double *Y2,
+                              int ldy, double *G1, double *G2, int ldg) {
+  static const double d1 = 1.0, dm1 = -1.0;
+  double *Bm = (double *)malloc(sizeof(double) * (size_t)r * b);
+  LP.dlacpy("All", &r, &b, G1, &ldg, Bm, &r, 3);
+  LP.dtrmm("Right", "Lower", "No transpose", "Unit", &r, &b, &d1, U11, &ldu, Bm,
+           &r, 5, 5, 12, 4);
+  if (mU > 0)
+    LP.dgemm("No transpose", "No transpose", &r, &b, &mU, &d1, G2, &ldg, U21,
+             &ldu, &d1, Bm, &r, 12, 12);
+  LP.dtrsm("Right", "Upper", "No transpose", "Non-unit", &r, &b, &d1, T, &ldt,
+           Bm, &r, 5, 5, 12, 8);
+  LP.dtrmm("Right", "Lower", "Conjugate transpose", "Unit", &r, &b, &dm1, U11,
+           &ldu, Bm, &r, 5, 5, 19, 4);
+  for (int c = 0; c < b; ++c)
+    for (int i = 0; i < r; ++i) Bm[i + (size_t)c * r] += G1[i + (size_t)c * ldg];
+  if (nY2 > 0)
+    LP.dgemm("No transpose", "No transpose", &r, &nY2, &b, &dm1, Bm, &r, R12,
+             &ldr, &d1, Y2, &ldy, 12, 12);
+  orc_apply_q_ut_rnfc(b, mU, r, U11, ldu, U21, T, ldt, G1, G2, ldg);
+  free(Bm);
+}
+
+/* FLA_HQRRP_UT_blk_var2 (UT.c:124-401).  carry: the generator's statics
+ * as the call finds them (NULL: a fresh process). */
+int orc_hqrrp_ut(int m, int n, double *A, int lda, int *p, double *s, int nb,
+                 int pp, int panel_pivoting, int *carry_have,
+                 double *carry_value) {
+  static const double d1 = 1.0, d0 = 0.0;
+  const int mn = OMIN(m, n);
+  if (mn == 0) return 0;
+  const int r = nb + pp;
+  double *G = (double *)malloc(sizeof(double) * (size_t)r * m);
+  double *Y = (double *)malloc(sizeof(double) * (size_t)r * n);
+  double *V = (double *)malloc(sizeof(double) * (size_t)r * n);
+  double *W = (double *)malloc(sizeof(double) * (size_t)nb * n);
+  orc_omega_carry(r, m, G, r, 12u, carry_have, carry_value);
+  LP.dgemm("No transpose", "No transpose", &r, &n, &m, &d1, G, &r, A, &lda, &d0,
+           Y, &r, 12, 12);
+  for (int j = 0; j < mn;) {
+    const int b = OMIN(OMIN(m - j, n - j), nb);
+    const int m22 = m - j - b, n22 = n - j - b;
+    const int last = (OMIN(m22, n22) <= 0);
+    int *pB = p + j;
+    double *sB = s + j;
+    if (!last) {
+      const int nR = n - j;
+      LP.dlacpy("All", &r, &nR, Y + (size_t)j * r, &r, V + (size_t)j * r, &r, 3);
+      orc_qrpmod_ut(1, b, r, nR, V + (size_t)j * r, r, pB, sB, 1, m,
+                    A + (size_t)j * lda, lda, 1, r, Y + (size_t)j * r, r, 0,
+                    NULL, 0);
+    }
+    double *T1 = W + (size_t)j * nb; /* W1's first b rows, ld nb */
+    double *A11 = A + j + (size_t)j * lda;
+    orc_qrpmod_ut(panel_pivoting, -1, m - j, b, A11, lda, pB, sB, 1, j,
+                  A + (size_t)j * lda, lda, 1, r, Y + (size_t)j * r, r, 1, T1,
+                  nb);
+    const int nA12 = n - j - b;
+    if (nA12 > 0) {
+      orc_apply_q_ut_lhfc(b, m - j - b, nA12, A11, lda, A11 + b, T1, nb,
+                          A + j + (size_t)(j + b) * lda,
+                          A + (j + b) + (size_t)(j + b) * lda, lda);
+    }
+    const int nY2 = n - j - b;
+    if (nY2 > 0)
+      orc_downdate_y_ut(b, m - j - b, r, nY2, A11, lda, A11 + b,
+                        A + j + (size_t)(j + b) * lda, lda, T1, nb,
+                        Y + (size_t)(j + b) * r, r, G + (size_t)j * r,
+                        G + (size_t)(j + b) * r, r);
+    j += b;
+  }
+  free(G);
+  free(Y);
+  free(V);
+  free(W);
+  return 0;
 }

@@ -23,6 +23,9 @@ int orc_rand_next(orc_rand_state *s);
 void orc_omega(int r, int m, double *G, int ldg, unsigned int seed);
 void orc_omega_carry(int r, int m, double *G, int ldg, unsigned int seed,
                      int *have, double *cache);
+int orc_hqrrp_ut(int m, int n, double *A, int lda, int *p, double *s, int nb,
+                 int pp, int panel_pivoting, int *carry_have,
+                 double *carry_value);
 
 int orc_qrpmod(int pivoting, int num_stages, int m, int n, double *A, int lda,
                int *p, double *t, int pivot_B, int mB, double *B, int ldb,

@@ -324,3 +324,97 @@ def test_omega_carry_matches_reference_statics(tmp_path):
         assert np.array_equal(a, d["a" + tag])
         assert np.array_equal(j, d["j" + tag])
         assert np.array_equal(t, d["t" + tag])
+
+
+# ------------------------------------------------------- libflame UT variant
+def _libflame_simple_test_matrix():
+    """The 5 x 5 matrix of libflame_sources/simple_test.c:84-116 (integers
+    filled column by column from the diagonal row down and around, a(0,0) =
+    1.2)."""
+    m = n = 5
+    A = np.zeros((m, n))
+    num = 1
+    for j in range(n):
+        for i in list(range(j % m, m)) + list(range(0, j % m)):
+            A[i, j] = num
+            num += 1
+    A[0, 0] = 1.2
+    return A
+
+
+UT_CASES = [("libflame_simple_test", None, 64, 10),
+            ("gaussian", (300, 200), 32, 8),
+            ("gaussian", (200, 300), 32, 8),
+            ("zero_cols", (260, 200), 32, 8),
+            ("graded", (400, 400), 32, 8)]
+
+
+def _ut_case(kind, shape, seed):
+    if kind == "libflame_simple_test":
+        return _libflame_simple_test_matrix()
+    if kind == "zero_cols":
+        A = matgen.gaussian(shape[0], shape[1], seed)
+        A[:, [3, 50, 120]] = 0.0
+        A[:, 7] = A[:, 9]
+        return A
+    return getattr(matgen, kind)(shape[0], shape[1], seed)
+
+
+@pytest.mark.parametrize("kind,shape,nb,pp", UT_CASES)
+def test_ut_variant_restatement(kind, shape, nb, pp):
+    """The UT driver (FLA_HQRRP_UT_blk_var2 restated, libflame's UT
+    Householder included) against what the mathematics fixes: A P = Q R
+    with Q = I - U inv(T) U' (T: diag s, above it U'U), Q orthogonal; and
+    against the WY driver (pinned bitwise against the compiled reference):
+    the same pivots and, for well-determined reflectors, s = 1/tau, the same
+    Householder vectors and R -- except where LAPACK's dlarfg found nothing
+    below the diagonal (tau = 0), where libflame reflects (s = 1/2, that row
+    of R negated).  Planted mistakes (LAPACK's tau taken for s; the sign of a
+    reflected row left alone) fail the identity."""
+    A = _ut_case(kind, shape, 61)
+    m, n = A.shape
+    k = min(m, n)
+    Au, ju, su = oracle.hqrrp_ut(A, nb, pp, 1)
+    Aw, jw, tw = oracle.hqrrp(A, nb, pp, 1)
+    assert np.array_equal(ju, jw)
+
+    def identity(Af, s):
+        Q = oracle.ut_q(Af, s)
+        R = np.triu(Af)[:k, :]
+        res = np.linalg.norm(A[:, ju - 1] - Q[:, :k] @ R) / np.linalg.norm(A)
+        return res, np.abs(Q.T @ Q - np.eye(m)).max()
+
+    EPS = np.finfo(np.float64).eps
+    res, orth = identity(Au, su)
+    assert res <= 50 * max(m, n) * EPS and orth <= 50 * max(m, n) * EPS
+    deg = tw == 0.0
+    assert np.array_equal(su[deg], np.full(deg.sum(), 0.5))
+    if kind == "libflame_simple_test":
+        assert deg[-1]  # the last step has nothing below the diagonal
+    # Elementwise only where the reflectors are well determined: graded
+    # spectra and the numerically zero trailing columns of a rank-deficient
+    # matrix leave the late ones at the mercy of rounding.
+    dw = np.abs(np.diag(Aw)[:k])
+    good = np.arange(k) < (k if kind != "zero_cols" else k - 4)
+    if kind != "graded":
+        g = good & ~deg
+        assert np.abs(su[g] * tw[g] - 1.0).max() <= 1e-12
+        scale = np.abs(np.triu(Aw)).max()
+        sign = np.where(deg, -1.0, 1.0)[:, None]
+        diffR = np.abs(np.triu(Au)[:k] - sign * np.triu(Aw)[:k])
+        assert diffR[good].max() <= 1e-12 * scale
+        diffV = np.abs(np.tril(Au, -1) - np.tril(Aw, -1))[:, :k]
+        assert diffV[:, good].max() <= 1e-11
+    assert dw.max() > 0
+    # planted: LAPACK's tau as s
+    if (~deg).any():
+        bad = su.copy()
+        bad[~deg] = tw[~deg]
+        assert identity(Au, bad)[0] > 1e-6
+    # planted: a reflected row left with LAPACK's sign (where that row of R
+    # is not negligible)
+    q = int(np.nonzero(deg)[0][0]) if deg.any() else -1
+    if q >= 0 and np.abs(Au[q, q:]).max() > 1e-6 * np.abs(Au).max():
+        Ab = Au.copy()
+        Ab[q, q:] = -Ab[q, q:]
+        assert identity(Ab, su)[0] > 1e-6
