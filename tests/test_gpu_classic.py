@@ -4,10 +4,11 @@ branches, which the reference sends to LAPACK (min(m,n) < 250 -> dgeqpf,
 LAPACK (OpenBLAS dgeqp3 / dgeqpf; its dgeqp4 is bitwise equal to the compiled
 reference front end):
 
-* jpvt equal, or first diverging at step i at a near-tie: the two candidate
-  columns' partial norms at step i, read off the oracle's R (||R(i:k+1, k)||
-  is the partial norm after i steps of the column it placed at k), differ by
-  less than 1e-12 |R_00|;
+* jpvt equal, or -- only for the cases listed in TIES -- first diverging at
+  step i at a near-tie: the two candidate columns' partial norms at step i,
+  read off the oracle's R (||R(i:k+1, k)|| is the partial norm after i steps
+  of the column it placed at k), differ by less than 1e-12 |R_00|; every
+  divergence is printed;
 * ||AP - QR||/||A||, ||Q'Q - I|| <= 50 max(m,n) eps;
 * |diag R| within 1e-10 |R_00| of the oracle's up to the divergence;
 * the reference example driver's 7x5 case (fixed columns, dgeqpf branch)
@@ -55,6 +56,8 @@ def _compare(A, Ag, jg, tg, Ar, jr, tr):
         assert kpos > i
         top = min(kpos + 1, m)
         gap = abs(Ar[i, i]) - np.linalg.norm(Ar[i:top, kpos])
+        print(f"{m}x{n}: pivots diverge at {i}, oracle gap "
+              f"{gap / scale:.3e} of |R_00|", flush=True)
         assert gap < 1e-12 * scale, (
             f"pivots diverge at {i}: oracle gap {gap / scale:.3e} of |R_00|")
         upto = i
@@ -92,6 +95,12 @@ CASES = [
 ]
 
 
+# Cases whose pivots diverge from LAPACK's at a near-tie (the defective
+# matrix's last columns are rounding noise: gap 6e-17 of |R_00| at step
+# 297); every other case must match exactly.
+TIES = {(400, 300, "defective")}
+
+
 @pytest.mark.parametrize("m,n,family,seed", CASES)
 def test_geqp3_device_matches_lapack(handle, m, n, family, seed):
     print(f"geqp3 {family} {m}x{n}", flush=True)
@@ -99,7 +108,9 @@ def test_geqp3_device_matches_lapack(handle, m, n, family, seed):
     Ag, jg, tg = _run_dev(handle, A)
     Ar, jr, tr, info = oracle.dgeqp3(A)
     assert info == 0
-    _compare(A, Ag, jg, tg, Ar, jr, tr)
+    div = _compare(A, Ag, jg, tg, Ar, jr, tr)
+    if (m, n, family) not in TIES:
+        assert div is None, f"pivots diverge at {div}"
 
 
 @pytest.mark.parametrize("m,n,fixed", [
@@ -122,7 +133,7 @@ def test_dgeqp4_small_branches(handle, m, n, fixed):
     assert info == 0 and info_r == 0 and w0 == 3 * n + 1
     nf = min(len(set(fixed)), m)
     assert sorted(jg[:nf].tolist()) == sorted(jr[:nf].tolist())
-    _compare(A, Ag, jg, tg, Ar, jr, tr)
+    assert _compare(A, Ag, jg, tg, Ar, jr, tr) is None
 
 
 def test_dgeqp4_small_matches_device_entry(handle):
