@@ -98,3 +98,30 @@ def test_ut_device_entry_equals_host_entry(handle):
     k = min(m, n)
     assert np.array_equal(np.triu(Ah)[:k], sign * np.triu(Aw)[:k])
     assert np.array_equal(np.tril(Ah, -1), np.tril(Aw, -1))
+
+
+def test_ut_device_entry_truncated(handle):
+    """With hqrrp_options.max_rank the rewrite covers exactly the factored
+    columns: s and R's reflected rows for k < kmax, the rest untouched
+    (equal to the WY device entry's output there)."""
+    import hqrrp_amd
+
+    m, n, kmax = 1300, 1200, 300
+    A = matgen.gaussian(m, n, 116)
+
+    def run(ut):
+        At = hqrrp_amd.to_colmajor(A)
+        jp = torch.arange(1, n + 1, dtype=torch.int32, device="cuda")
+        s = torch.zeros(min(m, n), dtype=torch.float64, device="cuda")
+        f = hqrrp_amd.hqrrp_ut_device if ut else hqrrp_amd.hqrrp_device
+        f(handle, At, jp, s, m, n, max_rank=kmax)
+        handle.synchronize()
+        torch.cuda.synchronize()
+        return hqrrp_amd.from_colmajor(At), jp.cpu().numpy(), s.cpu().numpy()
+
+    Au, ju, su = run(True)
+    Aw, jw, tw = run(False)
+    assert np.array_equal(ju, jw)
+    assert np.array_equal(su[:kmax], 1.0 / tw[:kmax])  # no reflection here
+    assert np.array_equal(su[kmax:], tw[kmax:])
+    assert np.array_equal(Au, Aw)
