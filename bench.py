@@ -18,11 +18,15 @@ realised shapes beside it.
 Also reported:
   phase_ms      -- per-phase kernel time of one extra, untimed step in which
                    every phase is bracketed by HIP events;
-  roofline      -- the phase with the largest time in that step (the
-                   dominant kernel): during the timed steps only its launches
-                   are bracketed by HIP events (recorded on the stream they
-                   run on), and its algorithmic flops (or bytes) per launch
-                   are divided by its average launch duration;
+  roofline      -- the bandwidth- or compute-bound phase with the largest
+                   time in that step (the dominant kernel): during the timed
+                   steps only its launches are bracketed by HIP events
+                   (recorded on the stream they run on), and its algorithmic
+                   flops (or bytes) per launch are divided by its average
+                   launch duration;
+  chain         -- the pivot chain's latency-bound kernels (sketch and panel
+                   QRCP, one persistent launch per block): time per launch
+                   and per dependent pivot step;
   cpu_baseline  -- the reference's dgeqp4 compiled from its source against
                    OpenBLAS (oracle/_ref; the oracle's restatement where it
                    was not built) on the whole of the same matrix, timed on
@@ -58,9 +62,9 @@ MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64):
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {
     "sketch_gemm": "mfma",
-    "sketch_qrcp": "hbm",
+    "sketch_qrcp": "latency",
     "gather": "hbm",
-    "panel": "hbm",
+    "panel": "latency",
     "panel_post": "hbm",
     "trail_vtc": "mfma",
     "trail_tw": "mfma",
@@ -609,7 +613,12 @@ def main():
         step()
         h.synchronize()
         prof_all = h.profile_read()
-        dominant = max(prof_all, key=lambda k: prof_all[k][0])
+        # The roofline is the bandwidth- or compute-bound phase with the most
+        # time; the pivot chain's persistent kernels (sketch and panel QRCP)
+        # are bound by the latency of their dependent pivot steps and are
+        # reported per step in "chain".
+        dominant = max((k for k in prof_all if PHASE_BOUND[k] != "latency"),
+                       key=lambda k: prof_all[k][0])
         h.profile(True, phases=[dominant])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -630,6 +639,7 @@ def main():
     roof = None
     phase_ms = {}
     gemm_roof = {}
+    chain = {}
     if prof_all:
         # The trailing-update kernels carry ~97% of the flops; report their
         # rooflines (from the breakdown step) beside the dominant phase's.
@@ -642,6 +652,15 @@ def main():
                     "hbm_gbs": round(by / (t_ms / 1e3) / 1e9, 1),
                     "frac_hbm": round(by / (t_ms / 1e3) / PEAK_HBM, 4)}
         phase_ms = {k: round(v[0], 3) for k, v in prof_all.items()}
+        # Pivot chain: one launch per block, nb dependent pivot steps each.
+        for name in ("sketch_qrcp", "panel"):
+            t_ms, launches, fl, by = prof_all[name]
+            if launches:
+                chain[name] = {
+                    "ms": round(t_ms, 3), "launches": launches,
+                    "us_per_launch": round(1e3 * t_ms / launches, 2),
+                    "us_per_pivot_step": round(1e3 * t_ms / launches / nb, 3),
+                    "share_of_step": round(t_ms / ms, 4)}
     if prof and dominant:
         name = dominant
         t_ms, launches, fl, by = prof[name]
@@ -715,7 +734,7 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "cpu_dgeqp3": cpu3,
         "vs_cpu_baseline": (round(value / cpu["value"], 2) if cpu else None),
         "quality": quality, "phase_ms": phase_ms,
-        "gemm_roofline": gemm_roof,
+        "gemm_roofline": gemm_roof, "chain": chain,
     }
     print(json.dumps(line), flush=True)
 

@@ -772,7 +772,14 @@ int trail_apply(hqrrp_handle_t h, const TrailArgs& a) {
   // C (and C2) are read and written once; V and W are small.
   const double mn = (double)a.mrows * (a.nc + (a.C2 ? a.nc2 : 0));
   PhaseScope ps(h, PH_TRAIL_UPD, 2.0 * a.b * mn, 16.0 * mn, 1);
-  HQ_TRY(launch_trail_update(a, h->cur));
+  // The CUs this stream may use (the look-ahead update stream holds all
+  // but the chain's); the persistent update kernel needs 160 KB of LDS.
+  TrailArgs u = a;
+  u.ncu = h->lds_max >= 160 * 1024
+              ? (h->cur == h->s_upd && h->s_upd ? h->cu_count - h->chain_cus
+                                                : h->cu_count)
+              : 0;
+  HQ_TRY(launch_trail_update(u, h->cur));
   return HQRRP_OK;
 }
 

@@ -16,6 +16,9 @@
 //
 // v_mfma_f64_16x16x4f64 fragments: A lane l = A[l%16][l/16], B lane l =
 // B[l/16][l%16]; accumulator element r of lane l = C[l/16 + 4r][l%16].
+#include <algorithm>
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -66,8 +69,8 @@ __device__ __forceinline__ void update_quadrant(const TrailArgs& a, long m0,
                                                 long nv, long n0, long nc,
                                                 const double* Cb, long ldc,
                                                 double* Ob, long ldo,
-                                                bool masked, int b) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                                bool masked, int b, int wave) {
+  const int lane = threadIdx.x & 63;
   const int wm = wave, wn = 0;  // 64-row quadrant of the 256 x 64 tile
   const int l15 = lane & 15, l4 = lane >> 4;
   // acc[i][j] element r: C'(n, m), n = n0+64wn+16i+l/16+4r, m = m0+64wm+16j+l%16,
@@ -182,12 +185,13 @@ __device__ __forceinline__ void update_quadrant(const TrailArgs& a, long m0,
   }
 }
 
-__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
+// The 256 x 64 tile at row m0 and virtual column nv, by four wavefronts
+// (wave = 0..3, one 64-row quadrant each).
+__device__ __forceinline__ void update_tile(const TrailArgs& a, long m0,
+                                            long nv, int wave) {
   const int b = a.b;
-  const long m0 = (long)blockIdx.x * kUR;
   const bool masked = a.vmask && m0 < b;  // only the top tile holds R / the diagonal
   // The tile's column range: C's, C2's (updated in place) or the gap.
-  const long nv = (long)blockIdx.y * kUC;  // first virtual column (W' row)
   long n0, nc, ldc, ldo;
   double *Cb, *Ob;
   if (nv < a.nc) {
@@ -208,9 +212,335 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
 
   const bool fast = m0 + kUR <= a.mrows && n0 + kUC <= nc && b == 64 && !masked;
   if (fast)
-    update_quadrant<true>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, false, 64);
+    update_quadrant<true>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, false, 64, wave);
   else
-    update_quadrant<false>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, masked, b);
+    update_quadrant<false>(a, m0, nv, n0, nc, Cb, ldc, Ob, ldo, masked, b,
+                           wave);
+}
+
+__global__ __launch_bounds__(HQ_NT, 2) void trail_update_kernel(TrailArgs a) {
+  update_tile(a, (long)blockIdx.x * kUR, (long)blockIdx.y * kUC,
+              threadIdx.x >> 6);
+}
+
+// ---------------------------------------------------------------------------
+// trail_band_kernel: the same update (every element of C and C2 receives the
+// same MFMA products in the same order as in trail_update_kernel, so the
+// results are bitwise equal) as a persistent launch of one 512-thread
+// workgroup per CU, for launches large enough to give every workgroup
+// several tiles.
+//
+// The tiles of full columns (256 rows x 64 virtual columns, the last band
+// possibly short) are numbered band-major (a band = 256 rows) and dealt to
+// the workgroups in contiguous runs, so that a workgroup keeps one band's V
+// for many tiles:
+//   * -V of the band (256 rows x 64 reflectors) sits in LDS, staged once per
+//     band; W' of the tile (64 columns x 64 reflectors) sits in LDS, the next
+//     tile's loaded into registers while the current tile's MFMAs run and
+//     written between two workgroup barriers at the tile's end;
+//   * the eight wavefronts (two per SIMD) each own a 32-row slice of the
+//     tile and hold two accumulator sets: while the matrix cores work on one
+//     tile in one set, the previous tile's results are stored out of the
+//     other set and the next tile's C is loaded into it (four stores and
+//     four loads per K step, the stores ahead of the loads), so the
+//     wavefront never has more than a few dozen memory operations in flight
+//     and never waits for C;
+//   * C is addressed as a buffer resource based at the tile's origin (its
+//     size ends after row 255 of the tile's last column); lanes whose rows lie
+//     beyond the matrix get a lane offset past that size, so their loads
+//     return zero and their stores are dropped;
+//   * the operand LDS layouts give every lane 16 contiguous bytes per
+//     operand (-V: [wave][k][row % 16][row / 16 % 2]; W': [k][col % 16]
+//     [col / 16]), whole wavefronts reading 1-2 KB runs.
+// Tiles with fewer than 64 columns (C's and C2's last ones) are left to
+// trail_edge_kernel (the per-tile code of trail_update_kernel).  V beyond b reflectors or below mrows is staged as zero, so
+// every tile runs all 16 K steps (exact zeros added, as in
+// trail_update_kernel); the unit-lower mask of the top rows is applied while
+// staging.
+namespace {
+
+constexpr int kBT = 512;        // threads of a band workgroup
+constexpr int kBW = kBT / 64;   // wavefronts (two per SIMD)
+constexpr int kBR = kUR / kBW;  // rows of a wavefront's slice (32)
+typedef unsigned int hq_u2 __attribute__((ext_vector_type(2)));
+
+// Wavefront-uniform copies (the compiler then keeps them, and the buffer
+// resources built from them, in scalar registers).
+__device__ __forceinline__ long band_u(long v) {
+  const unsigned long long u = (unsigned long long)v;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return (long)(((unsigned long long)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* band_up(T* p) {
+  return reinterpret_cast<T*>(band_u(reinterpret_cast<long>(p)));
+}
+
+// Tile counts per band: full C tiles, full C2 tiles, partial ones.
+struct BandGeom {
+  long nfc, nf2, full, npart, nbands;
+};
+
+__device__ __forceinline__ BandGeom band_geom(const TrailArgs& a) {
+  BandGeom g;
+  g.nfc = a.nc / kUC;
+  g.nf2 = a.C2 ? a.nc2 / kUC : 0;
+  g.full = g.nfc + g.nf2;
+  g.npart = (a.nc % kUC ? 1 : 0) + (a.C2 && a.nc2 % kUC ? 1 : 0);
+  g.nbands = hq_cdiv(a.mrows, kUR);
+  return g;
+}
+
+// Full-column tile t (band-major) as a scalar description.
+struct BandTile {
+  const double* in;  // origin of the tile in the input
+  double* out;       // ... and in the output
+  long ldi, ldo;
+  long nv;    // first virtual column (row of W')
+  long band;
+  int nrows;  // valid rows (<= 256)
+};
+
+__device__ __forceinline__ BandTile band_tile(const TrailArgs& a,
+                                              const BandGeom& g, long t) {
+  BandTile bt;
+  bt.band = t / g.full;
+  const long k = t - bt.band * g.full;
+  const long m0 = bt.band * kUR;
+  bt.nrows = (int)((long)a.mrows - m0 < kUR ? (long)a.mrows - m0 : kUR);
+  if (k < g.nfc) {
+    const long n0 = k * kUC;
+    bt.ldi = a.ldc;
+    bt.in = a.C + m0 + n0 * a.ldc;
+    bt.ldo = a.Cout ? a.ldco : a.ldc;
+    bt.out = (a.Cout ? a.Cout : a.C) + m0 + n0 * bt.ldo;
+    bt.nv = n0;
+  } else {
+    const long n0 = (k - g.nfc) * kUC;
+    bt.ldi = bt.ldo = a.ldc2;
+    bt.in = bt.out = a.C2 + m0 + n0 * a.ldc2;
+    bt.nv = a.nca + n0;
+  }
+  bt.in = band_up(bt.in);
+  bt.out = band_up(bt.out);
+  bt.ldi = band_u(bt.ldi);
+  bt.ldo = band_u(bt.ldo);
+  bt.nv = band_u(bt.nv);
+  bt.band = band_u(bt.band);
+  bt.nrows = __builtin_amdgcn_readfirstlane(bt.nrows);
+  return bt;
+}
+
+// Buffer resource over a full-column tile (needs ld >= 256, so its size,
+// which ends after row 255 of column 63, lies below the next column).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t band_rsrc(const double* p,
+                                                           long ld) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0,
+                                           (int)((63 * ld + kUR) * 8),
+                                           0x00020000);
+}
+
+// Byte offset of entry e's column group (16 i + 4 r) within a tile.
+__device__ __forceinline__ int band_soff(int e, long ld) {
+  return (int)((16 * (e >> 3) + 4 * ((e >> 1) & 3)) * ld * 8);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBT, 1) void trail_band_kernel(TrailArgs a) {
+  __shared__ __attribute__((aligned(16))) double vl[kBW * 64 * kBR];  // 128 KB
+  __shared__ __attribute__((aligned(16))) double wl[64 * 64];          // 32 KB
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l15 = lane & 15, l4 = lane >> 4;
+  const BandGeom geo = band_geom(a);
+  const long total = geo.nbands * geo.full;
+  const long G = gridDim.x, g = blockIdx.x;
+  const long t0 = total * g / G, t1 = total * (g + 1) / G;
+  const int b = a.b;
+
+  if (t0 < t1) {
+    // -V(band rows, k) into LDS: thread t stages row m = t % 256 for the
+    // reflectors of parity t / 256.  Rows beyond mrows and reflectors beyond
+    // b are zero; the top rows carry the unit-lower mask.
+    auto stage_v = [&](long band) {
+      const int m = t & (kUR - 1), kh = t >> 8;
+      const long row = band * kUR + m;
+      const bool rok = row < a.mrows;
+      const double* vr = a.V + (rok ? row : 0);
+      const int base = (m / kBR) * (64 * kBR) + (m & 15) * 2 + ((m >> 4) & 1);
+#pragma unroll 1  // few registers: this runs with both accumulator sets live
+      for (int c = 0; c < 4; ++c) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int k = 2 * (8 * c + q) + kh;
+          v[q] = vr[(long)(k < b ? k : 0) * a.ldv];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int k = 2 * (8 * c + q) + kh;
+          const double x = a.vmask ? vmask(v[q], row, k) : v[q];
+          vl[base + k * kBR] = (rok && k < b) ? -x : 0.0;
+        }
+      }
+    };
+    // W' of a tile: thread t holds columns 16 i + (p % 16) of reflector
+    // p / 16 for p = t and t + 512; reflectors beyond b are zero.
+    double ws[2][4];
+    auto load_w = [&](const BandTile& bt) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int p = t + kBT * u, k = p >> 4, nn = p & 15;
+        const double* wr = a.W + bt.nv + nn + (long)(k < b ? k : 0) * a.ldw;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double w = wr[16 * i];
+          ws[u][i] = k < b ? w : 0.0;
+        }
+      }
+    };
+    auto store_w = [&]() {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int p = t + kBT * u;
+        hq_d2* d = reinterpret_cast<hq_d2*>(&wl[p * 4]);
+        d[0] = hq_d2{ws[u][0], ws[u][1]};
+        d[1] = hq_d2{ws[u][2], ws[u][3]};
+      }
+    };
+
+    // Accumulator set s, entry e (0..31): acc[s][i][j][r] = C'(16 i + 4 r +
+    // l4, 32 wave + 16 j + l15) of its tile, i = e / 8, j = e % 2,
+    // r = e / 2 % 4.
+    hq_d4 acc[2][4][2];
+#define HQ_ENT(s, e) acc[s][(e) >> 3][(e) & 1][((e) >> 1) & 3]
+    // Lane byte offsets of the slice's rows j = 0, 1 (past the buffer for
+    // rows beyond the tile's valid ones).
+    auto lane_off = [&](const BandTile& bt, long ld, int (&vo)[2]) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int row = wave * kBR + 16 * j + l15;
+        vo[j] = row < bt.nrows ? (int)((row + l4 * ld) * 8) : 0x7ffffff0;
+      }
+    };
+    auto ld_entry = [&](__amdgpu_buffer_rsrc_t r, const int (&vo)[2], long ld,
+                        int e) -> double {
+      return __builtin_bit_cast(
+          double, __builtin_amdgcn_raw_buffer_load_b64(r, vo[e & 1],
+                                                       band_soff(e, ld), 2));
+    };
+    auto st_entry = [&](double v, __amdgpu_buffer_rsrc_t r, const int (&vo)[2],
+                        long ld, int e) {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(hq_u2, v), r,
+                                            vo[e & 1], band_soff(e, ld), 2);
+    };
+
+    long tt = t0;
+    BandTile cur = band_tile(a, geo, tt);
+    long band = cur.band;
+    stage_v(band);
+    {
+      int vo[2];
+      lane_off(cur, cur.ldi, vo);
+      const auto r = band_rsrc(cur.in, cur.ldi);
+#pragma unroll
+      for (int e = 0; e < 32; ++e) HQ_ENT(0, e) = ld_entry(r, vo, cur.ldi, e);
+    }
+    load_w(cur);
+    store_w();
+    // Nothing of the prologue stays in flight into the tile loop.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    bool prev = false;  // the other set holds the previous tile's results
+    BandTile pre = cur;
+    const double* wb = &wl[4 * lane];
+    const double* vb = &vl[wave * (64 * kBR) + 2 * lane];
+
+    auto step = [&](auto set) {
+      constexpr int S = decltype(set)::value, O = 1 - S;
+      const bool nxt = tt + 1 < t1;
+      // Without a next tile the loads below re-read this tile (harmless:
+      // the other set is not used again).
+      const BandTile nx = nxt ? band_tile(a, geo, tt + 1) : cur;
+      const bool chg = nx.band != band;
+      int von[2], vop[2];
+      lane_off(nx, nx.ldi, von);
+      lane_off(pre, pre.ldo, vop);
+      const auto rn = band_rsrc(nx.in, nx.ldi);
+      const auto rp = band_rsrc(pre.out, pre.ldo);
+      load_w(nx);
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const hq_d2 a01 = *reinterpret_cast<const hq_d2*>(wb + 256 * s);
+        const hq_d2 a23 = *reinterpret_cast<const hq_d2*>(wb + 256 * s + 2);
+        const hq_d2 b01 = *reinterpret_cast<const hq_d2*>(vb + 2 * 64 * s);
+        const double af[4] = {a01[0], a01[1], a23[0], a23[1]};
+        const double bf[2] = {b01[0], b01[1]};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[S][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                af[i], bf[j], acc[S][i][j], 0, 0, 0);
+        if (s < 8 && prev) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int e = 4 * s + q;
+            st_entry(HQ_ENT(O, e), rp, vop, pre.ldo, e);
+          }
+        }
+        if (s >= 4 && s < 12) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int e = 4 * (s - 4) + q;
+            HQ_ENT(O, e) = ld_entry(rn, von, nx.ldi, e);
+          }
+        }
+      }
+      if (!nxt || chg) {
+        // The end of a band (or of the run): this tile's results go out now.
+        int vo[2];
+        lane_off(cur, cur.ldo, vo);
+        const auto r = band_rsrc(cur.out, cur.ldo);
+#pragma unroll
+        for (int e = 0; e < 32; ++e) st_entry(HQ_ENT(S, e), r, vo, cur.ldo, e);
+        prev = false;
+      } else {
+        prev = true;
+      }
+      pre = cur;
+      cur = nx;
+      __syncthreads();  // every wavefront is done with this tile's W' and V
+      if (nxt) {
+        if (chg) {
+          band = nx.band;
+          stage_v(band);
+        }
+        store_w();
+        __syncthreads();
+      }
+      ++tt;
+    };
+    while (tt < t1) {
+      step(std::integral_constant<int, 0>{});
+      if (tt >= t1) break;
+      step(std::integral_constant<int, 1>{});
+    }
+#undef HQ_ENT
+  }
+
+}
+
+// The tiles trail_band_kernel leaves: the partial-column tiles (C's last and
+// C2's last, at most two per band), one per workgroup.
+__global__ __launch_bounds__(HQ_NT, 2) void trail_edge_kernel(TrailArgs a) {
+  const BandGeom geo = band_geom(a);
+  const long e = blockIdx.x;
+  const long band = e / geo.npart, w = e - band * geo.npart;
+  const bool cpart = (a.nc % kUC) && w == 0;
+  const long nv = cpart ? geo.nfc * kUC : a.nca + geo.nf2 * kUC;
+  update_tile(a, band * kUR, nv, threadIdx.x >> 6);
 }
 
 // part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
@@ -804,6 +1134,25 @@ hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s) {
+  // The persistent band kernel when every one of a.ncu workgroups (one per
+  // CU the stream may use) gets at least 16 tiles (below that its per-band
+  // staging costs more than it saves); its buffer resources need leading
+  // dimensions of at least 256 and tile extents below 2^30 bytes.
+  const long ntiles = hq_cdiv(a.mrows, kUR) *
+                      (a.nc / kUC + (a.C2 ? a.nc2 / kUC : 0));
+  const long ldmax = std::max(std::max(a.ldc, a.Cout ? a.ldco : a.ldc),
+                              a.C2 ? a.ldc2 : a.ldc);
+  const long ldmin = std::min(std::min(a.ldc, a.Cout ? a.ldco : a.ldc),
+                              a.C2 ? a.ldc2 : a.ldc);
+  if (a.ncu > 0 && ntiles >= 16L * a.ncu && ldmin >= kUR &&
+      ldmax * 8 * kUC < (1L << 30)) {
+    const long nedge = hq_cdiv(a.mrows, kUR) *
+                       ((a.nc % kUC ? 1 : 0) + (a.C2 && a.nc2 % kUC ? 1 : 0));
+    if (nedge > 0)
+      hipLaunchKernelGGL(trail_edge_kernel, dim3(nedge), dim3(HQ_NT), 0, s, a);
+    hipLaunchKernelGGL(trail_band_kernel, dim3(a.ncu), dim3(kBT), 0, s, a);
+    return hipGetLastError();
+  }
   dim3 grid(hq_cdiv(a.mrows, kUR), hq_cdiv(a.C2 ? a.ncv : a.nc, kUC));
   hipLaunchKernelGGL(trail_update_kernel, grid, dim3(HQ_NT), 0, s, a);
   return hipGetLastError();

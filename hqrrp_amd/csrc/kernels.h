@@ -86,6 +86,9 @@ struct TrailArgs {
   double* C2;
   long ldc2;
   int nc2, nca, ncv;
+  // trail_update: CUs the launch stream may use (0: the per-tile kernel
+  // only); the persistent band kernel takes one workgroup per CU.
+  int ncu;
 };
 
 // k_downdate.hip: FP64 sketch downdate of one block (r <= 80, b <= 64).
