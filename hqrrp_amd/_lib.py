@@ -58,6 +58,7 @@ SIGNATURES = {
     "hqrrp_destroy": (C.c_int, [_vp]),
     "hqrrp_set_stream": (C.c_int, [_vp, _vp]),
     "hqrrp_reset_stream": (C.c_int, [_vp]),
+    "hqrrp_set_panel_alg": (C.c_int, [_vp, C.c_int]),
     "hqrrp_synchronize": (C.c_int, [_vp]),
     "hqrrp_phase_count": (C.c_int, []),
     "hqrrp_phase_name": (C.c_char_p, [C.c_int]),

@@ -143,6 +143,12 @@ class Handle:
                                          C.c_void_p(stream.cuda_stream)),
                   "hqrrp_set_stream")
 
+    def set_panel_alg(self, alg: int) -> None:
+        """0: Cholesky-QR panel with the per-step kernel as its fallback;
+        1 (default): the per-step panel kernel for every panel."""
+        check(lib().hqrrp_set_panel_alg(self._h, int(alg)),
+              "hqrrp_set_panel_alg")
+
     def synchronize(self) -> None:
         check(lib().hqrrp_synchronize(self._h), "hqrrp_synchronize")
 

@@ -178,28 +178,58 @@ __device__ __forceinline__ void hq_argmax_dpp(double& key, int& pos) {
       __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo)),
       po);
 }
+// Maximum of v over the wavefront, in every lane: within each row of 16
+// lanes a prefix maximum by DPP row shifts (1, 2, 4, 8; lanes shifted in from
+// outside the row read 0), then the row maxima combined by row broadcasts of
+// lanes 15 and 31, so lane 63 holds the maximum.  All 64 lanes active.
+__device__ __forceinline__ unsigned hq_wave_max_u32(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xE, true));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xC, true));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, true));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, true));
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Order-preserving map of a non-NaN double to an unsigned 64-bit key (both
+// zeros to the same key): sign bit set -> all bits inverted, else the sign
+// bit set.
+__device__ __forceinline__ unsigned long long hq_order_key(double key) {
+  const unsigned long long b =
+      (unsigned long long)__double_as_longlong(key == 0.0 ? 0.0 : key);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
 // Argmax over the 64 lanes of a wavefront with the rule of hq_argmax_take
 // (idamax's: largest key, first position among equals); every lane receives
-// the result.  Within each row of 16 lanes: xor 1, xor 2 (quad
-// permutations), then the half-row and row mirrors, each pairing every lane
-// with one holding the other half of its group; the four row results are
-// then combined from scalar reads of lanes 0, 16, 32, 48.  The rule is a
-// lexicographic maximum, so the pairing order cannot change the result.
-// All 64 lanes must be active.
+// the result.  Keys are compared through hq_order_key: the maximum of the
+// upper 32 bits, then -- only when several lanes share it -- of the lower 32
+// bits among those lanes, then -- only for an exact tie -- the smallest
+// position among the tied lanes.  Keys must not be NaN (hq_pivot_key maps
+// NaN norms to a finite key).  All 64 lanes must be active.
 __device__ __forceinline__ void hq_wave_argmax64(double& key, int& pos) {
-  hq_argmax_dpp<0xB1>(key, pos);   // quad_perm [1,0,3,2]
-  hq_argmax_dpp<0x4E>(key, pos);   // quad_perm [2,3,0,1]
-  hq_argmax_dpp<0x141>(key, pos);  // row_half_mirror
-  hq_argmax_dpp<0x140>(key, pos);  // row_mirror
-  double k0 = hq_readlane(key, 0), k1 = hq_readlane(key, 16);
-  double k2 = hq_readlane(key, 32), k3 = hq_readlane(key, 48);
-  int p0 = hq_readlane(pos, 0), p1 = hq_readlane(pos, 16);
-  int p2 = hq_readlane(pos, 32), p3 = hq_readlane(pos, 48);
-  hq_argmax_take(k0, p0, k1, p1);
-  hq_argmax_take(k2, p2, k3, p3);
-  hq_argmax_take(k0, p0, k2, p2);
-  key = k0;
-  pos = p0;
+  const unsigned long long k = hq_order_key(key);
+  const unsigned hi = (unsigned)(k >> 32), lo = (unsigned)k;
+  const unsigned mhi = hq_wave_max_u32(hi);
+  unsigned long long cand = __ballot(hi == mhi);
+  if (__popcll(cand) > 1) {
+    const unsigned mlo = hq_wave_max_u32(hi == mhi ? lo : 0u);
+    cand = __ballot(hi == mhi && lo == mlo);
+    if (__popcll(cand) > 1) {
+      const bool mine = (cand >> (threadIdx.x & 63)) & 1ull;
+      int pp = mine ? pos : 0x7fffffff;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) pp = min(pp, __shfl_xor(pp, off, 64));
+      const int lane = __builtin_ctzll(__ballot(mine && pos == pp));
+      key = hq_readlane(key, lane);
+      pos = pp;
+      return;
+    }
+  }
+  const int lane = __builtin_ctzll(cand);
+  key = hq_readlane(key, lane);
+  pos = hq_readlane(pos, lane);
 }
 
 // Pivot key of a candidate's partial norm: a NaN norm (NaN or Inf entries

@@ -250,6 +250,11 @@ struct hqrrp_handle_s {
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
   DevBuf Pn;  // look-ahead: the next panel
+  // Cholesky-QR panel (k_cqr.hip): R1 (64 x 64) and the fallback flag.
+  // panel_alg 0 selects it (with the per-step kernel as its fallback), 1
+  // keeps every panel on the per-step kernel.
+  DevBuf cqr;
+  int panel_alg = 1;
   DevBuf gen, genT, genM;  // generic block step: sketch copy / norms, T
                            // products, trailing products
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
@@ -642,6 +647,77 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
   return HQRRP_OK;
 }
 
+// The panel from two Gram passes (k_cqr.hip): Cholesky QR2, the reference's
+// QRCP of the b x b factor R, Householder reconstruction; T is written
+// directly.  A panel the kernels flag (nearly dependent columns, a column
+// with nothing below its diagonal) is factored by the per-step kernel and T
+// by larft in the same stream order, both launches gated on the flag, so
+// the host never waits.  sel: as in run_panel (the selected columns are
+// formed and stored into dP by the first pass).
+int run_panel_cqr(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
+                  int pivoting, double* dtau, int* dperm, double* dT, long ldt,
+                  const PanelArgs* sel) {
+  const int nwc = hq_cdiv(mrows, cqr_rows_per_wg());
+  HQ_RET(ensure(h->cqr, sizeof(double) * (4096 + 64)));
+  CqrArgs c{};
+  c.R1 = h->cqr.as<double>();
+  c.flag = reinterpret_cast<int*>(c.R1 + 4096);
+  c.P = dP;
+  c.ldp = ldp;
+  c.mrows = mrows;
+  c.b = b;
+  c.pivoting = pivoting;
+  c.nw = nwc;
+  c.tau = dtau;
+  c.perm = dperm;
+  c.T = dT;
+  c.ldt = ldt;
+  c.err = h->err();
+  if (sel) {
+    c.sel_moves = sel->sel_moves;
+    c.sel_b2 = sel->sel_b2;
+    c.sel_r0 = sel->sel_r0;
+    c.sel_b = sel->sel_b;
+    c.sel_C = sel->sel_C;
+    c.sel_ldc = sel->sel_ldc;
+    c.sel_V = sel->sel_V;
+    c.sel_ldv = sel->sel_ldv;
+    c.sel_W = sel->sel_W;
+    c.sel_ldw = sel->sel_ldw;
+  }
+  {
+    const size_t need = cqr_xchg_doubles(c.nw);
+    HQ_RET(xchg_begin(h, h->xp, need, &c.xcur, &c.xprev, &c.xprev_len));
+    PhaseScope ps(h, PH_PANEL, 2.0 * mrows * b * b - 2.0 / 3.0 * b * b * b,
+                  16.0 * mrows * b, 1);
+    HQ_TRY(launch_cqr_panel(c, h->cur));
+    xchg_end(h->xp, need);
+  }
+  // Fallback, gated on the flag: the per-step kernel on the panel as stored.
+  PanelArgs a{};
+  HQ_RET(panel_geometry(h, mrows, &a.nw));
+  a.P = dP;
+  a.ldp = ldp;
+  a.mrows = mrows;
+  a.b = b;
+  a.pivoting = pivoting;
+  a.tau = dtau;
+  a.perm = dperm;
+  a.run_if = c.flag;
+  HQ_RET(ensure(h->gram, sizeof(double) * 64 * 64));
+  const size_t xneed = panel_xchg_doubles(a.nw, b);
+  HQ_RET(xchg_begin(h, h->xp, xneed, &a.xcur, &a.xprev, &a.xprev_len));
+  a.gram = h->gram.as<double>();
+  a.err = h->err();
+  {
+    PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 2);
+    HQ_TRY(launch_panel_qrcp(a, h->cur));
+    xchg_end(h->xp, xneed);
+    HQ_TRY(launch_larft(b, a.gram, b, dtau, dT, ldt, h->cur, c.flag));
+  }
+  return HQRRP_OK;
+}
+
 // dT == nullptr: T is left to the caller (larft from h->gram).  sel: the
 // look-ahead selection and update of the panel's columns, done by the panel
 // kernel as it loads them (its sel_* fields; null: the panel is read from
@@ -657,6 +733,9 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
     return run_panel_generic(h, mrows, b, dP, ldp, pivoting, dtau, dperm, dT,
                              ldt);
   }
+  if (h->panel_alg == 0 && dT)
+    return run_panel_cqr(h, mrows, b, dP, ldp, pivoting, dtau, dperm, dT, ldt,
+                         sel);
   PanelArgs a{};
   if (sel) {
     a.sel_moves = sel->sel_moves;
@@ -1291,10 +1370,13 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       sel.sel_W = ta.W;
       sel.sel_ldw = ta.ldw;
       sel.sel_b = b;
+      // The Cholesky-QR panel writes T itself; the per-step kernel leaves
+      // it to larft, which then runs beside the next block's V'C.
+      const bool direct_t = h->panel_alg == 0;
       HQ_RET(run_panel(h, mnext, b2, Pn, mnext, o.panel_pivoting, dtau + jn,
-                       h->perm.as<int>(), nullptr, nb, &sel));
+                       h->perm.as<int>(), direct_t ? T : nullptr, nb, &sel));
       HQ_TRY(hipEventRecord(h->ev_chain, h->s_chain));
-      {
+      if (!direct_t) {
         PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
         HQ_TRY(launch_larft(b2, h->gram.as<double>(), b2, dtau + jn, T, nb,
                             h->cur));
@@ -1686,7 +1768,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
                     &h->T,     &h->perm,  &h->tmpA,  &h->tmpY,   &h->tmpJ,
                     &h->split, &h->Mx,    &h->W2,    &h->Pp,     &h->Qm,
                     &h->hA,    &h->hJ,    &h->hT,     &h->hW,    &h->vn,
-                    &h->Pn,    &h->Xb,     &h->adjAP, &h->gen,
+                    &h->Pn,    &h->Xb,     &h->adjAP, &h->gen,  &h->cqr,
                     &h->genT,  &h->genM,
                     &h->adjTau, &h->adjK, &h->adjX};
   for (DevBuf* b : bufs)
@@ -1699,6 +1781,13 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
     if (st) (void)hipStreamDestroy(st);
   if (h->own) (void)hipStreamDestroy(h->own);
   delete h;
+  return HQRRP_OK;
+}
+
+extern "C" int hqrrp_set_panel_alg(hqrrp_handle_t h, int alg) {
+  if (!h || alg < 0 || alg > 1) return HQRRP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->panel_alg = alg;
   return HQRRP_OK;
 }
 

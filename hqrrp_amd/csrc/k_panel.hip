@@ -61,6 +61,10 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
 
   const int t = threadIdx.x, w = blockIdx.x;
   const int b = a.b, nw = a.nw;
+  if (a.run_if && *a.run_if == 0) {
+    hq_xreset(a.xprev, a.xprev_len, w, nw, kPT);
+    return;
+  }
   const long row0 = (long)w * kPR;
   const int cc = t & 63, g = t >> 6;
   const bool col_ok = cc < b;
@@ -393,7 +397,9 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
 __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
                                                       long ldg,
                                                       const double* tau,
-                                                      double* T, long ldt) {
+                                                      double* T, long ldt,
+                                                      const int* run_if) {
+  if (run_if && *run_if == 0) return;
   __shared__ double sT[kMaxB][kMaxB + 1];
   __shared__ double sG[kMaxB][kMaxB + 1];
   __shared__ double sX[kMaxB][kMaxB + 1];
@@ -521,10 +527,11 @@ size_t panel_xchg_doubles(int nw, int b) {
 }
 
 hipError_t launch_larft(int b, const double* gram, long ldg, const double* tau,
-                        double* T, long ldt, hipStream_t stream) {
+                        double* T, long ldt, hipStream_t stream,
+                        const int* run_if) {
   if (b < 1 || b > kMaxB) return hipErrorInvalidValue;
   hipLaunchKernelGGL(larft_kernel, dim3(1), dim3(HQ_NT), 0, stream, b, gram,
-                     ldg, tau, T, ldt);
+                     ldg, tau, T, ldt, run_if);
   return hipGetLastError();
 }
 

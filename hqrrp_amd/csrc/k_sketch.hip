@@ -112,9 +112,11 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     //    thread combines the 8 wavefront winners identically and the
     //    winning column's lanes publish the slot.
     {
-      double key = (pos >= 0) ? hq_pivot_key((double)d) : -1.0;
-      int kp = (pos >= 0) ? pos : INT_MAX;
-      hq_wave_argmax64(key, kp);  // lanes l, l+32 hold the same column
+      // Lanes l and l+32 hold the same column: the upper half stands aside
+      // (a duplicate would only make the argmax resolve a tie).
+      double key = (pos >= 0 && half == 0) ? hq_pivot_key((double)d) : -1.0;
+      int kp = (pos >= 0 && half == 0) ? pos : INT_MAX;
+      hq_wave_argmax64(key, kp);
       if (lane == 0) {
         s_key[wave] = key;
         s_pos[wave] = kp;
@@ -122,11 +124,11 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     {
-      // Every wavefront combines the 8 wavefront winners (lane l holds
-      // winner l % 8; identical result in every lane); bq: the winner's
+      // Every wavefront combines the 8 wavefront winners (lane l < 8 holds
+      // winner l; identical result in every lane); bq: the winner's
       // wavefront.
-      const int wq = s_pos[lane & 7];
-      double bk = s_key[lane & 7];
+      const int wq = (lane < 8) ? s_pos[lane] : INT_MAX;
+      double bk = (lane < 8) ? s_key[lane] : -1.0;
       int bp = wq;
       hq_wave_argmax64(bk, bp);
       const int bq = __builtin_ctzll(__ballot(wq == bp) & 0xFFull);

@@ -45,7 +45,45 @@ struct PanelArgs {
   long sel_ldv;
   const double* sel_W;
   long sel_ldw;
+  // Run only when null or *run_if != 0 (the fallback of the Cholesky-QR
+  // panel); otherwise the launch only restores its exchange sentinel.
+  const int* run_if;
 };
+
+// k_cqr.hip: panel QRCP from two Gram passes (pivoted Cholesky QR with the
+// reference's pivot rule, a second Cholesky QR pass) and Householder
+// reconstruction, b <= 64, one persistent launch; the per-step panel kernel
+// takes the panels it flags.
+struct CqrArgs {
+  double* P;  // panel: mrows x b, ld ldp (factored in place)
+  long ldp;
+  int mrows, b, pivoting;
+  int nw;          // workgroups: cqr_rows_per_wg() rows each, co-resident
+  double* tau;     // b
+  int* perm;       // b: original column at each position (0-based)
+  double* T;       // b x b, ld ldt
+  long ldt;
+  double* R1;      // workspace: 64 x 64
+  int* flag;       // set to 1 when the per-step kernel must take the panel
+  // Exchange buffers as in SketchArgs; cqr_xchg_doubles(nw) used.
+  double* xcur;
+  double* xprev;
+  long xprev_len;
+  int* err;
+  // Look-ahead column selection as in PanelArgs (sel_C set); the formed
+  // panel is stored into P.
+  const int* sel_moves;
+  int sel_b2, sel_r0, sel_b;
+  const double* sel_C;
+  long sel_ldc;
+  const double* sel_V;
+  long sel_ldv;
+  const double* sel_W;
+  long sel_ldw;
+};
+int cqr_rows_per_wg();
+size_t cqr_xchg_doubles(int nw);
+hipError_t launch_cqr_panel(const CqrArgs& a, hipStream_t s);
 
 struct GemmArgs {
   int M, N, K;
@@ -203,9 +241,10 @@ hipError_t launch_panel_qrcp(const PanelArgs& a, hipStream_t s);
 int panel_rows_per_wg();
 size_t panel_xchg_doubles(int nw, int b);
 // T (b x b, ld ldt) from the reflector inner products gram (strictly upper
-// part, ld ldg) and tau; b <= 64.
+// part, ld ldg) and tau; b <= 64.  With run_if, only when *run_if != 0.
 hipError_t launch_larft(int b, const double* gram, long ldg, const double* tau,
-                        double* T, long ldt, hipStream_t s);
+                        double* T, long ldt, hipStream_t s,
+                        const int* run_if = nullptr);
 // Rows [0, nrows) of the b panel columns follow the panel's pivots (and
 // jpvt); with P, the same launch also copies the factored panel (prows x b,
 // ld ldp) into rows nrows.. of those columns.
