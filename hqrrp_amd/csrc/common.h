@@ -288,6 +288,25 @@ __device__ __forceinline__ hq_reflector_t<T> hq_larfg(T alpha, T xnorm2) {
   h.scale = T(1) / (alpha - beta);
   return h;
 }
+// The same with the norm sqrt(alpha^2 + ||x||^2) taken directly (one square
+// root instead of dlapy2's): for the kernels of a factorisation, which runs
+// on a matrix prescaled to magnitudes within 2^+-256 (2^+-60 with the float
+// sketch), where these squares can neither overflow nor underflow.
+template <class T>
+__device__ __forceinline__ hq_reflector_t<T> hq_larfg_direct(T alpha, T xnorm2) {
+  hq_reflector_t<T> h;
+  if (xnorm2 == T(0)) {
+    h.beta = alpha;
+    h.tau = T(0);
+    h.scale = T(1);
+    return h;
+  }
+  T beta = -copysign(sqrt(__builtin_fma(alpha, alpha, xnorm2)), alpha);
+  h.beta = beta;
+  h.tau = (beta - alpha) / beta;
+  h.scale = T(1) / (alpha - beta);
+  return h;
+}
 __device__ __forceinline__ hq_reflector hq_dlarfg(double alpha, double xnorm2) {
   return hq_larfg<double>(alpha, xnorm2);
 }

@@ -285,7 +285,8 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     // 2. Reflector, dots and gram entries (every thread, identical bits):
     //    the pivot's row-k entry and G_p are read from the pivot's lane.
     const double gsum = full(cc), rin = s_rowin[cc];
-    const hq_reflector h = hq_dlarfg(hq_readlane(rin, pc), hq_readlane(gsum, pc));
+    const hq_reflector h =
+        hq_larfg_direct<double>(hq_readlane(rin, pc), hq_readlane(gsum, pc));
     if (w == 0 && t == 0) a.tau[k] = h.tau;
     const double xk = col_ok ? rin : 0.0;
     const double dot = xk + h.scale * (col_ok ? gsum : 0.0);
@@ -323,16 +324,18 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
 
     // 4. LAWN-176 downdate with row k of R (= the owner's updated x[r]).
     bool need = false;
+    // The reference's t = |r| / dn, (1 + t)(1 - t), test t (dn / en)^2 <=
+    // tol3z and dn sqrt((1 + t)(1 - t)), without divisions: (dn - |r|)(dn +
+    // |r|) is dn^2 (1 + t)(1 - t), compared with tol3z en^2.
     if (a.pivoting && !done && dn != 0.0) {
       const double rk = (h.tau != 0.0) ? xk + f : xk;
-      double tt = fabs(rk) / dn;
-      const double u = (1.0 + tt) * (1.0 - tt);
-      tt = (0.0 > u) ? 0.0 : u;
-      const double ratio = dn / en;
-      if (tt * ratio * ratio <= HQ_TOL3Z)
+      const double ar = fabs(rk);
+      double u2 = (dn - ar) * (dn + ar);
+      u2 = (0.0 > u2) ? 0.0 : u2;
+      if (u2 <= HQ_TOL3Z * (en * en))
         need = true;
       else
-        dn = dn * sqrt(tt);
+        dn = sqrt(u2);
     }
     if (__ballot(need)) {  // identical in every wavefront of every workgroup
       // Every row still held is below k (the row-k wavefront has shifted).

@@ -92,11 +92,13 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
   T d = sqrt(ss), e = d;
   // Squared norm of this column's rows below the first active one (the
   // ||x||^2 of its reflector, or LAWN 176's recompute): lane pair sum.
+  // Four interleaved partial sums keep the dependent chain short.
   auto below2 = [&]() -> T {
-    T s1 = 0;
+    T s4[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int q = 1; q < H; ++q) s1 += x[q] * x[q];
-    s1 += half ? x[0] * x[0] : T(0);
+    for (int q = 1; q < H; ++q) s4[q & 3] += x[q] * x[q];
+    s4[0] += half ? x[0] * x[0] : T(0);
+    const T s1 = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     return s1 + hq_xor32(s1);
   };
   int pos = own ? c : -1;  // current position; -1 once chosen as a pivot
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       double cv[2];
       cv[0] = f0 ? hq_fetch(src + lane) : 0.0;
       cv[1] = f1 ? hq_fetch(src + lane + 64) : 0.0;
-      const hq_reflector_t<T> h = hq_larfg<T>((T)ba, (T)bs);
+      const hq_reflector_t<T> h = hq_larfg_direct<T>((T)ba, (T)bs);
       if (ok && (hq_pending(cv[0]) || hq_pending(cv[1])))
         ok = hq_await(cv, [&](int i) -> const double* {
           return (i == 0 ? f0 : f1) ? src + lane + 64 * i : nullptr;
@@ -235,9 +237,10 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     if (pos >= 0) {
       const T tau = s_tau;
       if (tau != T(0)) {
-        T wd = 0;
+        T w4[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int q = 0; q < H; ++q) wd += s_v[ibase + q] * x[q];
+        for (int q = 0; q < H; ++q) w4[q & 3] += s_v[ibase + q] * x[q];
+        T wd = (w4[0] + w4[1]) + (w4[2] + w4[3]);
         wd += hq_xor32(wd);
         const T f = -tau * wd;
 #pragma unroll
@@ -248,15 +251,16 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
       const T x0 = x[0];
       const T up = hq_xor32(x0);  // partner's first row
       const T wt = half ? up : x0;  // row k
+      // The reference's t = |w| / d, (1 + t)(1 - t), test t (d / e)^2 <=
+      // tol3z and d sqrt((1 + t)(1 - t)), without divisions: (d - |w|)(d +
+      // |w|) is d^2 (1 + t)(1 - t), compared with tol3z e^2.
       bool need = false;
-      T tt = 0;
       if (d != T(0)) {
-        const T t1 = fabs(wt) / d;
-        const T u = (T(1) + t1) * (T(1) - t1);
-        tt = (T(0) > u) ? T(0) : u;
-        const T ratio = d / e;
-        need = tt * ratio * ratio <= hq_tol3z<T>();
-        if (!need) d = d * sqrt(tt);
+        const T aw = fabs(wt);
+        T u2 = (d - aw) * (d + aw);
+        u2 = (T(0) > u2) ? T(0) : u2;
+        need = u2 <= hq_tol3z<T>() * (e * e);
+        if (!need) d = sqrt(u2);
       }
       if (__ballot(need)) {
         const T b2 = below2();
