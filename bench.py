@@ -503,6 +503,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--lookahead", type=int, default=1,
                     help="overlap the next panel with the trailing update")
+    ap.add_argument("--panel", choices=("default", "cqr", "step"),
+                    default="default",
+                    help="panel factorisation: the library's default, the "
+                         "two-Gram-pass kernel (cqr) or the per-step kernel")
     ap.add_argument("--input", choices=("graded", "gaussian", "lowrank"),
                     default="graded",
                     help="graded: the benchmark matrix; gaussian: same shape, "
@@ -592,6 +596,8 @@ def main():
     tau = torch.empty(min(m, n), dtype=torch.float64, device=dev)
     h = hqrrp_amd.Handle(0)
     h.use_stream(torch.cuda.current_stream())
+    if args.panel != "default":
+        h.set_panel_alg(0 if args.panel == "cqr" else 1)
 
     def step():
         A.copy_(A0)
@@ -730,7 +736,7 @@ def main():
                        f"HQRRP {args.sketch} sketch {m}x{n} {args.input}"),
                    "m": m, "n": n, "nb": nb, "pp": pp, "panel_pivoting": 1,
                    "sketch": args.sketch, "omega": "reference stream seed 12",
-                   "lookahead": args.lookahead},
+                   "lookahead": args.lookahead, "panel": args.panel},
         "roofline": roof, "cpu_baseline": cpu, "cpu_dgeqp3": cpu3,
         "vs_cpu_baseline": (round(value / cpu["value"], 2) if cpu else None),
         "quality": quality, "phase_ms": phase_ms,

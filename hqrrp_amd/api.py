@@ -144,8 +144,8 @@ class Handle:
                   "hqrrp_set_stream")
 
     def set_panel_alg(self, alg: int) -> None:
-        """0: Cholesky-QR panel with the per-step kernel as its fallback;
-        1 (default): the per-step panel kernel for every panel."""
+        """0 (default): Cholesky-QR panel with the per-step kernel as its
+        fallback; 1: the per-step panel kernel for every panel."""
         check(lib().hqrrp_set_panel_alg(self._h, int(alg)),
               "hqrrp_set_panel_alg")
 

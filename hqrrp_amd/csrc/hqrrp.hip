@@ -254,7 +254,7 @@ struct hqrrp_handle_s {
   // panel_alg 0 selects it (with the per-step kernel as its fallback), 1
   // keeps every panel on the per-step kernel.
   DevBuf cqr;
-  int panel_alg = 1;
+  int panel_alg = 0;
   DevBuf gen, genT, genM;  // generic block step: sketch copy / norms, T
                            // products, trailing products
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)

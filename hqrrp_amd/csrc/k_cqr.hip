@@ -59,7 +59,6 @@ constexpr int kMat = 64 * LD;
 constexpr int kSmem = 4 * kMat + 32 * 33;  // >= 8 * 64 * kXl
 static_assert(kSmem >= 8 * 64 * kXl, "transposition regions");
 constexpr double kChol1Thr = 1e-6;   // pivot / squared column norm, pass 1
-constexpr double kChol2Thr = 1e-10;  // pivot / diagonal, pass 2 (G2 ~ I)
 
 __device__ __forceinline__ int tile_i(int t) {
   return t < 4 ? 0 : t < 7 ? 1 : t < 9 ? 2 : 3;
@@ -408,242 +407,290 @@ __device__ __forceinline__ void tri_inv_upper(double* X, UF u, DF dinv, double* 
   __syncthreads();
 }
 
-// Right-looking elimination on a 64 x 64 LDS matrix held in registers by
-// wavefronts 0..3 (wavefront w: columns 16w..16w+15, lane i: row i), one
-// workgroup barrier per step: column k of the current Schur complement is
-// double-buffered in s_col by its owner, row k is read from lane k with
-// readlane.  The 16 column updates of a step are independent and branch-
-// free (rows and columns outside the trailing block get a zero factor), so
-// they overlap in the pipeline; rows keep their unscaled values and are
-// scaled once at the end.  Wavefronts 4..7 only pass the barriers.
-//
-// chol4: upper Cholesky factor of the symmetric matrix in A (both triangles;
-// leading b x b block, the identity beyond), in place, strictly lower part
-// zeroed.  Every pivot must exceed thr times its original diagonal entry
-// (ruling out zero, negative, NaN and Inf pivots); returns false otherwise.
-// s_v: 128 doubles of scratch.
-__device__ __forceinline__ bool chol4(double* A, int b, double thr, double* s_col, double* s_g,
-                      double* s_v, int* s_fail) {
-  const int t = threadIdx.x, i = t & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  double x[16];
-  if (wv < 4) {
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) x[jj] = A[i + LD * (16 * wv + jj)];
+// Upper Cholesky factor of a matrix close to the identity, G = I + E (A:
+// leading b x b block, the identity beyond; // caller: max |E| <= kNearI), as R = I + U by the fixed-point iteration U =
+// up(E - U'U), whose strictly upper part is that of E - U'U and whose
+// diagonal is half its diagonal (from R'R = I + U + U' + U'U = I + E).  The
+// error contracts by about ||E|| <= 64 kNearI per iteration; the iteration
+// stops when an update changes nothing (at most kCholIters).  Every product is a 64 x 64 tile product on
+// the matrix cores.  A receives R (zero below the diagonal); X is scratch
+// (U'U), s_flag one int.  Returns false if it did not converge.
+constexpr int kCholIters = 12;
+constexpr double kNearI = 1e-4;
+__device__ __forceinline__ bool chol_near_identity(double* A, double* U,
+                                                   double* X, int b,
+                                                   int* s_flag) {
+  const int t = threadIdx.x;
+  // E in A (upper part used); U0 = up(E).
+  for (int idx = t; idx < 64 * 64; idx += kT) {
+    const int i = idx & 63, j = idx >> 6;
+    const double e = (i < b && j < b) ? A[i + LD * j] - (i == j ? 1.0 : 0.0) : 0.0;
+    U[i + LD * j] = (i < j) ? e : (i == j ? 0.5 * e : 0.0);
   }
-  if (t < 64) {
-    s_g[t] = A[t + LD * t];
-    s_col[t] = A[t];
-  }
-  if (t == 0) *s_fail = 0;
   __syncthreads();
-  for (int k = 0; k < b; ++k) {
-    const double* ck = s_col + 64 * (k & 1);
-    const double akk = ck[k];
-    if (!(akk > thr * s_g[k])) {
-      if (t == 0) *s_fail = 1;
-      break;
-    }
-    const double rkk = sqrt(akk), rinv = 1.0 / rkk;
-    if (t == 0) {
-      s_v[k] = rkk;
-      s_v[64 + k] = rinv;
-    }
-    if (wv < 4) {
-      // R(k, i) R(k, j) = A(i, k) A(k, j) / A(k, k).
-      const double rki = (i > k) ? ck[i] * rinv : 0.0;
-      double akj[16];
+  bool conv = false;
+  for (int it = 0; it < kCholIters && !conv; ++it) {
+    if (t == 0) *s_flag = 0;
+    // X = U'U (U upper: column tiles need rows up to their own).
+    {
+      const int wave = t >> 6, l = t & 63;
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) akj[jj] = hq_readlane(x[jj], k);
+      for (int u = 0; u < 2; ++u) {
+        const int tt = wave + 8 * u;
+        const int r0 = 16 * (tt & 3), c0 = 16 * (tt >> 2);
+        hq_d4 acc = {0.0, 0.0, 0.0, 0.0};
+        const int kend = min(r0, c0) + 16;  // U(k, r) = 0 for k > r
+        for (int k = 0; k < kend; k += 4)
+          acc = mfma4(U[(k + (l >> 4)) + LD * (r0 + (l & 15))],
+                      U[(k + (l >> 4)) + LD * (c0 + (l & 15))], acc);
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const double f = (16 * wv + jj > k) ? akj[jj] * rinv : 0.0;
-        x[jj] = __builtin_fma(-rki, f, x[jj]);
-      }
-      const int kn = k + 1;
-      if (kn < b && wv == (kn >> 4)) {
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj)
-          if (jj == (kn & 15)) s_col[64 * (kn & 1) + i] = x[jj];
+        for (int e = 0; e < 4; ++e) X[(r0 + (l >> 4) + 4 * e) + LD * (c0 + (l & 15))] = acc[e];
       }
     }
     __syncthreads();
-  }
-  __syncthreads();
-  const bool ok = *s_fail == 0;
-  if (wv < 4 && ok) {
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
-      const int j = 16 * wv + jj;
-      double v;
-      if (i >= b || j >= b)
-        v = (i == j) ? 1.0 : 0.0;
-      else
-        v = (i > j) ? 0.0 : (i == j ? s_v[i] : x[jj] * s_v[64 + i]);
-      A[i + LD * j] = v;
+    bool changed = false;
+    for (int idx = t; idx < 64 * 64; idx += kT) {
+      const int i = idx & 63, j = idx >> 6;
+      if (i <= j && i < b && j < b) {
+        const double e = A[i + LD * j] - (i == j ? 1.0 : 0.0);
+        const double f = e - X[i + LD * j];
+        const double un = (i == j) ? 0.5 * f : f;
+        changed |= un != U[i + LD * j];
+        U[i + LD * j] = un;
+      }
     }
+    if (changed) *s_flag = 1;
+    __syncthreads();
+    conv = *s_flag == 0;
+    __syncthreads();
+  }
+  for (int idx = t; idx < 64 * 64; idx += kT) {
+    const int i = idx & 63, j = idx >> 6;
+    A[i + LD * j] = (i == j ? 1.0 : 0.0) + ((i <= j) ? U[i + LD * j] : 0.0);
   }
   __syncthreads();
-  return ok;
+  return conv;
 }
 
-// lu4: LU without pivoting of W + diag(S) (W in A, leading b x b block,
-// zero beyond), S_k the sign of the k-th Schur pivot w (sign 0 = +), so that
+// LU without pivoting of W + diag(S) (W in A, leading b x b block, zero
+// beyond), S_k the sign of the k-th Schur pivot w (sign 0 = +), so that
 // U(k, k) = S_k + w has |U(k, k)| >= 1; U on and above the diagonal of A,
-// L's multipliers below (the identity beyond b), S in s_S.  |w| = 1 means
-// the column's part below the diagonal vanishes (LAPACK then takes tau =
-// 0): returns false.  s_v: 128 doubles of scratch.
-__device__ __forceinline__ bool lu4(double* A, int b, double* s_col, double* s_S, double* s_v,
-                    int* s_fail) {
-  const int t = threadIdx.x, i = t & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  double x[16];
-  if (wv < 4) {
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) x[jj] = A[i + LD * (16 * wv + jj)];
-  }
-  if (t < 64) s_col[t] = A[t];
+// L's multipliers below (the identity beyond b), S_k, U(k, k) and 1 /
+// U(k, k) in s_S[k], s_v[k], s_v[64 + k].  |w| = 1 means the column's part
+// below the diagonal vanishes (LAPACK then takes tau = 0): returns false.
+// By blocks of 16 columns: for each block,
+// wavefront 0 eliminates the block's columns in registers (lane i: row i, 16
+// steps without barriers, row k read from lane k), solves for the block row
+// to the right with the block's unit lower triangle (lane j: column j), and
+// all wavefronts apply the block's rank-16 update to the trailing matrix on
+// the matrix cores.  The arithmetic of every entry is that of the unblocked
+// elimination regrouped (W's Schur complements, the same multipliers).
+__device__ __forceinline__ bool lu_blocked(double* A, int b, double* s_S,
+                                           double* s_v, int* s_fail) {
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
   if (t == 0) *s_fail = 0;
   __syncthreads();
-  for (int k = 0; k < b; ++k) {
-    const double* ck = s_col + 64 * (k & 1);
-    const double wkk = ck[k];
-    if (!(fabs(wkk) < 1.0 - 1e-12)) {
-      if (t == 0) *s_fail = 1;
-      break;
-    }
-    const double sk = (wkk >= 0.0) ? 1.0 : -1.0;
-    const double ukk = sk + wkk, uinv = 1.0 / ukk;
-    if (t == 0) {
-      s_S[k] = sk;
-      s_v[k] = ukk;
-      s_v[64 + k] = uinv;
-    }
-    if (wv < 4) {
-      const double lik = (i > k) ? ck[i] * uinv : 0.0;
-      double ukj[16];
+  for (int c0 = 0; c0 < 64; c0 += 16) {
+    if (c0 >= b) break;
+    // Block panel: rows c0..63 of columns c0..c0+15 (wavefront 0).
+    if (wave == 0) {
+      const int i = l;
+      double x[16];
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) ukj[jj] = hq_readlane(x[jj], k);
+      for (int jj = 0; jj < 16; ++jj) x[jj] = A[i + LD * (c0 + jj)];
+      bool fail = false;
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const double f = (16 * wv + jj > k) ? ukj[jj] : 0.0;
-        x[jj] = __builtin_fma(-lik, f, x[jj]);
+      for (int kk = 0; kk < 16; ++kk) {
+        const int k = c0 + kk;
+        if (k < b && !fail) {
+          const double wkk = hq_readlane(x[kk], k);
+          if (!(fabs(wkk) < 1.0 - 1e-12)) {
+            fail = true;
+          } else {
+            const double sk = (wkk >= 0.0) ? 1.0 : -1.0;
+            const double ukk = sk + wkk, uinv = 1.0 / ukk;
+            if (l == 0) {
+              s_S[k] = sk;
+              s_v[k] = ukk;
+              s_v[64 + k] = uinv;
+            }
+            const double lik = (i > k) ? x[kk] * uinv : 0.0;
+#pragma unroll
+            for (int jj = kk + 1; jj < 16; ++jj)
+              x[jj] = __builtin_fma(-lik, hq_readlane(x[jj], k), x[jj]);
+            if (i > k) x[kk] = lik;
+            if (i == k) x[kk] = ukk;
+          }
+        }
       }
-      const int kn = k + 1;
-      if (kn < b && wv == (kn >> 4)) {
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj)
-          if (jj == (kn & 15)) s_col[64 * (kn & 1) + i] = x[jj];
-      }
+      for (int jj = 0; jj < 16; ++jj)
+        if (i >= c0) A[i + LD * (c0 + jj)] = x[jj];
+      if (l == 0 && fail) *s_fail = 1;
+    }
+    __syncthreads();
+    if (*s_fail) break;
+    const int c1 = c0 + 16;
+    if (c1 >= 64) break;
+    // Block row: U12 = L11^-1 A12 (lane j: column j >= c1; wavefront 0).
+    if (wave == 0 && l >= c1) {
+      const int j = l;
+      double y[16];
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii) y[ii] = A[(c0 + ii) + LD * j];
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii)
+#pragma unroll
+        for (int i2 = ii + 1; i2 < 16; ++i2)
+          y[i2] = __builtin_fma(-A[(c0 + i2) + LD * (c0 + ii)], y[ii], y[i2]);
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii) A[(c0 + ii) + LD * j] = y[ii];
+    }
+    __syncthreads();
+    // Trailing update A22 -= L21 U12 (rows and columns c1..63, K = 16).
+    const int nt = (64 - c1) / 16;
+    for (int tt = wave; tt < nt * nt; tt += kT / 64) {
+      const int r0 = c1 + 16 * (tt % nt), q0 = c1 + 16 * (tt / nt);
+      hq_d4 acc;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = A[(r0 + (l >> 4) + 4 * e) + LD * (q0 + (l & 15))];
+#pragma unroll
+      for (int k = 0; k < 16; k += 4)
+        acc = mfma4(-A[(r0 + (l & 15)) + LD * (c0 + k + (l >> 4))],
+                    A[(c0 + k + (l >> 4)) + LD * (q0 + (l & 15))], acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) A[(r0 + (l >> 4) + 4 * e) + LD * (q0 + (l & 15))] = acc[e];
     }
     __syncthreads();
   }
   __syncthreads();
   const bool ok = *s_fail == 0;
-  if (wv < 4 && ok) {
-    // Column j's entries below the diagonal stay at their step-j Schur
-    // values: the multiplier is that times 1 / U(j, j), as used in step j.
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
-      const int j = 16 * wv + jj;
-      double v;
-      if (i >= b || j >= b)
-        v = (i == j) ? 1.0 : 0.0;
-      else
-        v = (i < j) ? x[jj] : (i == j ? s_v[i] : x[jj] * s_v[64 + j]);
-      A[i + LD * j] = v;
+  if (ok) {
+    // The identity beyond b (W was zero there: S = 1, U = 1, L = 0).
+    for (int idx = t; idx < 64 * 64; idx += kT) {
+      const int i = idx & 63, j = idx >> 6;
+      if (i >= b || j >= b) A[i + LD * j] = (i == j) ? 1.0 : 0.0;
     }
   }
   __syncthreads();
   return ok;
 }
 
-// Step 1 on wavefront 0 of workgroup 0: pivoted Cholesky of G (LDS, full
-// symmetric, leading b x b block) with the reference's pivot rule, left-
-// looking and without barriers.  Lane c keeps column c's Schur diagonal,
-// its LAWN 176 partial norm (vn1, vn2), position and state in registers;
-// row k of R (physical columns) is stored to Rk (LD, zero on entry), from
-// which row k+1 reads the pivot's column and its own.  On success
-// s_perm[pos] = column.
-// Returns the outcome in wavefront 0 (the caller broadcasts it).
-__device__ __forceinline__ bool chol_pivoted_wave(const double* G, double* Rk, int b,
-                                  int mrows, int pivoting, int* s_perm) {
-  const int c = threadIdx.x & 63;
-  const double gc = G[c + LD * c];
+// Step 1 on workgroup 0: pivoted Cholesky of G (in S: LDS, full symmetric,
+// leading b x b block) with the reference's pivot rule, by blocks of 16
+// steps.  Within a block wavefront 0 works left-looking and without
+// barriers: lane c keeps column c's Schur diagonal, its LAWN 176 partial
+// norm (vn1, vn2), position and state in registers, and row k of R
+// (physical columns, stored to Rk, LD, zero on entry) is S(p, c) minus the
+// block's earlier rows' contribution, over R(k, k).  After each block all
+// wavefronts subtract the block's rank-16 contribution from S on the matrix
+// cores.  On success s_perm[pos] = column.  Returns the outcome in every
+// thread.
+__device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
+                                             int mrows, int pivoting,
+                                             int* s_perm, int* s_fail) {
+  const int t = threadIdx.x, wave = t >> 6, c = t & 63;
+  const double gc = S[c + LD * c];
   double d = gc;  // Schur diagonal: squared norm of the part below row k
   double vn1 = (c < b) ? sqrt(gc) : 0.0, vn2 = vn1;
   int pos = c;
   bool done = c >= b;
   double rprev = 0.0;
-  bool ok = true;
+  if (t == 0) *s_fail = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < b; k0 += 16) {
+    const int k1 = min(k0 + 16, b);
+    if (wave == 0) {
+      bool ok = true;
 #pragma nounroll
-  for (int k = 0; k < b; ++k) {
-    // LAWN 176 downdate with row k-1 of R (NoFLA_HQRRP_WY_blk_var4.c:
-    // 941-986), in the same quantities without divisions: with t = |r| /
-    // vn1, (1 + t)(1 - t) vn1^2 = (vn1 - |r|)(vn1 + |r|) and the recompute
-    // test t (vn1 / vn2)^2 <= tol3z reads (vn1 - |r|)(vn1 + |r|) <= tol3z
-    // vn2^2; the recompute takes the Schur diagonal.
-    if (k > 0 && pivoting && !done && vn1 != 0.0) {
-      const double ar = fabs(rprev);
-      double u2 = (vn1 - ar) * (vn1 + ar);
-      u2 = (0.0 > u2) ? 0.0 : u2;
-      if (u2 <= HQ_TOL3Z * (vn2 * vn2)) {
-        vn1 = (mrows - (k - 1) - 1 > 0) ? sqrt(d > 0.0 ? d : 0.0) : 0.0;
-        vn2 = vn1;
-      } else {
-        vn1 = sqrt(u2);
+      for (int k = k0; k < k1; ++k) {
+        // LAWN 176 downdate with row k-1 of R (NoFLA_HQRRP_WY_blk_var4.c:
+        // 941-986), in the same quantities without divisions: with t = |r|
+        // / vn1, (1 + t)(1 - t) vn1^2 = (vn1 - |r|)(vn1 + |r|) and the
+        // recompute test t (vn1 / vn2)^2 <= tol3z reads (vn1 - |r|)(vn1 +
+        // |r|) <= tol3z vn2^2; the recompute takes the Schur diagonal.
+        if (k > 0 && pivoting && !done && vn1 != 0.0) {
+          const double ar = fabs(rprev);
+          double u2 = (vn1 - ar) * (vn1 + ar);
+          u2 = (0.0 > u2) ? 0.0 : u2;
+          if (u2 <= HQ_TOL3Z * (vn2 * vn2)) {
+            vn1 = (mrows - (k - 1) - 1 > 0) ? sqrt(d > 0.0 ? d : 0.0) : 0.0;
+            vn2 = vn1;
+          } else {
+            vn1 = sqrt(u2);
+          }
+        }
+        // idamax over positions k..b-1 (largest norm, first position).
+        int pk = k;
+        if (pivoting) {
+          double key = done ? -1.0 : hq_pivot_key(vn1);
+          int q = done ? INT_MAX : pos;
+          hq_wave_argmax64(key, q);
+          pk = q;
+        }
+        const int p = __builtin_ctzll(__ballot(!done && pos == pk));
+        const bool pend = !done && c != p;
+        if (c == p) {
+          pos = k;
+          done = true;
+        } else if (pend && pos == k) {
+          pos = pk;
+        }
+        const double dp = hq_readlane(d, p), gp = hq_readlane(gc, p);
+        if (!(dp > kChol1Thr * gp)) {
+          ok = false;
+          break;
+        }
+        const double rkk = sqrt(dp), rinv = 1.0 / rkk;
+        // The block's earlier rows: R(k0:k, p)' R(k0:k, c); rows k.. of Rk
+        // are still zero, so the four-way groups need no masks.
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = k0; i < k; i += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            s4[u] = __builtin_fma(Rk[i + u + LD * p], Rk[i + u + LD * c], s4[u]);
+        }
+        const double dot = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        const double r = (c == p) ? rkk : (pend ? (S[p + LD * c] - dot) * rinv : 0.0);
+        if (pend) d = __builtin_fma(-r, r, d);
+        rprev = r;
+        Rk[k + LD * c] = r;  // row k is not read before the next step
+        wave_sync();
+      }
+      if (!ok && c == 0) *s_fail = 1;
+    }
+    __syncthreads();
+    if (*s_fail) break;
+    if (k1 < b) {
+      // S -= R(k0:k0+16, :)' R(k0:k0+16, :) (16 tiles of 16 x 16, K = 16;
+      // rows of Rk beyond k1 are zero).
+      const int l = c;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int tt = wave + 8 * u;
+        const int r0 = 16 * (tt & 3), q0 = 16 * (tt >> 2);
+        hq_d4 acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] = S[(r0 + (l >> 4) + 4 * e) + LD * (q0 + (l & 15))];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4)
+          acc = mfma4(-Rk[(k0 + kk + (l >> 4)) + LD * (r0 + (l & 15))],
+                      Rk[(k0 + kk + (l >> 4)) + LD * (q0 + (l & 15))], acc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) S[(r0 + (l >> 4) + 4 * e) + LD * (q0 + (l & 15))] = acc[e];
       }
     }
-    // idamax over positions k..b-1 (largest norm, first position).
-    int pk = k;
-    if (pivoting) {
-      double key = done ? -1.0 : hq_pivot_key(vn1);
-      int q = done ? INT_MAX : pos;
-      hq_wave_argmax64(key, q);
-      pk = q;
-    }
-    const int p = __builtin_ctzll(__ballot(!done && pos == pk));
-    const bool pend = !done && c != p;
-    if (c == p) {
-      pos = k;
-      done = true;
-    } else if (pend && pos == k) {
-      pos = pk;
-    }
-    const double dp = hq_readlane(d, p), gp = hq_readlane(gc, p);
-    if (!(dp > kChol1Thr * gp)) {
-      ok = false;
-      break;
-    }
-    const double rkk = sqrt(dp), rinv = 1.0 / rkk;
-    // R(0:k, p)' R(0:k, c), eight independent partial sums.
-    double s8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // Rows k.. of Rk are still zero, so the groups of eight need no masks.
-    for (int i = 0; i < k; i += 8) {
-      double rp[8], rc[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        rp[u] = Rk[i + u + LD * p];
-        rc[u] = Rk[i + u + LD * c];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) s8[u] = __builtin_fma(rp[u], rc[u], s8[u]);
-    }
-    const double dot = ((s8[0] + s8[1]) + (s8[2] + s8[3])) +
-                       ((s8[4] + s8[5]) + (s8[6] + s8[7]));
-    const double r = (c == p) ? rkk : (pend ? (G[p + LD * c] - dot) * rinv : 0.0);
-    if (pend) d = __builtin_fma(-r, r, d);
-    rprev = r;
-    Rk[k + LD * c] = r;  // row k is not read before the next step
-    wave_sync();
+    __syncthreads();
   }
-  if (ok) s_perm[c < b ? pos : c] = c;
+  const bool ok = *s_fail == 0;
+  if (ok && wave == 0) s_perm[c < b ? pos : c] = c;
+  __syncthreads();
   return ok;
 }
 
 __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
   __shared__ double smem[kSmem];
-  __shared__ double s_row[128], s_g[64], s_S[64], s_v2[128];
+  __shared__ double s_S[64], s_v2[128];
   __shared__ int s_src[64], s_perm[64];
   __shared__ int s_abort, s_flag, s_fail;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -711,12 +758,7 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
     }
     for (int idx = t; idx < kMat; idx += kT) mC[idx] = 0.0;
     __syncthreads();
-    if (wave == 0) {
-      const bool okw = chol_pivoted_wave(mA, mC, b, a.mrows, a.pivoting, s_perm);
-      if (t == 0) s_fail = okw ? 0 : 1;
-    }
-    __syncthreads();
-    const bool ok = s_fail == 0;
+    const bool ok = chol_pivoted(mA, mC, b, a.mrows, a.pivoting, s_perm, &s_fail);
     if (ok) {
       // R1 in pivoted order (mB and the workspace): R1(i, j) = Rk(i, perm j).
       for (int idx = t; idx < 64 * 64; idx += kT) {
@@ -764,10 +806,10 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
     __syncthreads();
     for (int idx = t; idx < b * b; idx += kT) {
       const int i = idx % b, j = idx / b;
-      if (i <= j && !(fabs(mA[i + LD * j] - (i == j ? 1.0 : 0.0)) <= 0.5)) s_fail = 1;
+      if (i <= j && !(fabs(mA[i + LD * j] - (i == j ? 1.0 : 0.0)) <= kNearI)) s_fail = 1;
     }
     __syncthreads();
-    bool ok = s_fail == 0 && chol4(mA, b, kChol2Thr, s_row, s_g, s_v2, &s_fail);
+    bool ok = s_fail == 0 && chol_near_identity(mA, mB, mC, b, &s_fail);
     if (ok) {
       // mB = R2^-1; mD = R = R2 R1 (R1 from pass 1); mC = Q1's first 64
       // rows (this workgroup's wavefronts 0, 1 hold them).
@@ -796,7 +838,7 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
         mA[i + LD * j] = (i < b && j < b) ? v : 0.0;
       });
       __syncthreads();
-      ok = lu4(mA, b, s_row, s_S, s_v2, &s_fail);
+      ok = lu_blocked(mA, b, s_S, s_v2, &s_fail);
     }
     if (ok) {
       // The identity beyond b; mC = U^-1; N2 = -R2^-1 U^-1 (published).

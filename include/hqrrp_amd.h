@@ -151,11 +151,11 @@ int hqrrp_synchronize(hqrrp_handle_t h);
 
 /* Panel factorisation used by every block of a handle's factorisations (no
  * counterpart in the reference, whose panel is NoFLA_QRPmod_WY_unb_var4,
- * NoFLA_HQRRP_WY_blk_var4.c:520-524, 774-886): 0 = two Gram passes
- * (pivoted Cholesky QR with the reference's pivot rule, a second Cholesky QR
- * pass) and Householder reconstruction, falling back on the device to the
- * per-step kernel for panels it cannot factor to working accuracy; 1
- * (default) = the per-step kernel (one grid-wide reduction per pivot) for
+ * NoFLA_HQRRP_WY_blk_var4.c:520-524, 774-886): 0 (default) = two Gram
+ * passes (pivoted Cholesky QR with the reference's pivot rule, a second
+ * Cholesky QR pass) and Householder reconstruction, falling back on the
+ * device to the per-step kernel for panels it cannot factor to working
+ * accuracy; 1 = the per-step kernel (one grid-wide reduction per pivot) for
  * every panel.  Both give the reference's pivots up to near-ties and values
  * equal to rounding. */
 int hqrrp_set_panel_alg(hqrrp_handle_t h, int alg);
