@@ -803,23 +803,37 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   a.ldc = ldc;
   a.T = T;
   a.ldt = ldt;
-  // V'C: 256-column tiles (one workgroup per CU) when they can fill the
-  // CUs with K slices of >= 256 rows, else 64-column tiles (two per CU).
-  // Split K so that the grid fills but does not exceed those slots (a
-  // second, partial round of workgroups would double the time).
-  const long maxs = std::max(1L, (long)mrows / 256);
-  const long wtiles = hq_cdiv(nc, trail_vtc_cols());
-  a.wide = wtiles * maxs >= h->cu_count;
-  const long tiles = a.wide ? wtiles : hq_cdiv(nc, 64);
-  const long slots = a.wide ? h->cu_count : 2L * h->cu_count;
-  const int kbk = trail_kbk();
-  long S = std::max(1L, slots / tiles);
-  S = std::min(S, maxs);
-  long kc = (mrows + S - 1) / S;
-  kc = (kc + kbk - 1) / kbk * kbk;
-  S = (mrows + kc - 1) / kc;
+  // V'C: the stream-K partition over all wavefronts of one workgroup per
+  // CU when every wavefront gets >= 256 rows of a 64-column group; else
+  // 64-column tiles (two workgroups per CU) with K split uniformly so that
+  // the grid fills but does not exceed those slots (a second, partial round
+  // of workgroups would double the time).
+  const long nch = hq_cdiv(mrows, kVtcChunk);
+  const long ngroups = hq_cdiv(a.nc, 64) + hq_cdiv(a.nc2, 64);
+  const long waves = 4L * h->cu_count;
+  long S;
+  if (nch * ngroups >= 16 * waves) {
+    a.sk = 1;
+    a.sk_waves = (int)waves;
+    a.sk_nch = nch;
+    a.sk_q = nch * ngroups / waves;
+    a.sk_r = nch * ngroups % waves;
+    const VtcPart vp = vtc_part(a);
+    S = 1;
+    for (long g = 0; g < ngroups; ++g) S = std::max(S, (long)vp.count(g));
+  } else {
+    const long maxs = std::max(1L, (long)mrows / 256);
+    const long tiles = hq_cdiv(nc, 64);
+    const long slots = 2L * h->cu_count;
+    const int kbk = trail_kbk();
+    S = std::max(1L, slots / tiles);
+    S = std::min(S, maxs);
+    long kc = (mrows + S - 1) / S;
+    kc = (kc + kbk - 1) / kbk * kbk;
+    S = (mrows + kc - 1) / kc;
+    a.kchunk = kc;
+  }
   a.splits = (int)S;
-  a.kchunk = kc;
   HQ_RET(ensure(h->split, sizeof(double) * (size_t)S * b * nc));
   HQ_RET(ensure(h->W2, sizeof(double) * (size_t)b * nc));
   a.part = h->split.as<double>();
@@ -1060,6 +1074,8 @@ int block_tail(hqrrp_handle_t h, const TrailArgs& ta, int j, int jn,
   t.b = ta.b;
   t.nc = ta.nc;
   t.S = ta.splits;
+  t.sk = ta.sk;
+  t.vp = vtc_part(ta);
   t.nc2 = ta.nc2;
   t.nca = ta.nca;
   t.ncv = ta.ncv;
@@ -1220,8 +1236,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
                              m, nullptr, 0, dA, lda, &ta));
         ta.vmask = 0;
         HQ_TRY(launch_trail_vtc(ta, h->cur));
-        HQ_TRY(launch_reduce_splitk(pb, n, ta.splits, ta.part, 0.0,
-                                    static_cast<double*>(Y) + p0, r, h->cur));
+        HQ_TRY(launch_trail_sum(ta, static_cast<double*>(Y) + p0, r,
+                                h->cur));
       }
     }
   }

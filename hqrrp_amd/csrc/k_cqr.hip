@@ -727,8 +727,9 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
       if (dst < a.sel_b2) s_src[dst] = a.sel_moves[2 + 2 * t];
     }
     __syncthreads();
+    static_assert(kSmem >= 64 * kSelLd, "W' staging fits in smem");
     select_tile(xa, s_src, a.sel_b2, a.sel_C, a.sel_ldc, a.sel_r0, a.mrows,
-                a.sel_V, a.sel_ldv, a.sel_W, a.sel_ldw, a.sel_b, mw);
+                a.sel_V, a.sel_ldv, a.sel_W, a.sel_ldw, a.sel_b, mw, smem);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll

@@ -98,8 +98,10 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
     }
     __syncthreads();
     hq_d4 acc[4][2];
+    static_assert(kMaxB * 129 >= 64 * kSelLd, "W' staging fits in stage");
     select_tile(acc, s_src, a.sel_b2, a.sel_C, a.sel_ldc, a.sel_r0, a.mrows,
-                a.sel_V, a.sel_ldv, a.sel_W, a.sel_ldw, a.sel_b, glo);
+                a.sel_V, a.sel_ldv, a.sel_W, a.sel_ldw, a.sel_b, glo,
+                &stage[0][0]);
     const int l15 = cc & 15, l4 = cc >> 4;
 #pragma unroll
     for (int rd = 0; rd < kPR / 128; ++rd) {

@@ -543,44 +543,60 @@ __global__ __launch_bounds__(HQ_NT, 2) void trail_edge_kernel(TrailArgs a) {
   update_tile(a, band * kUR, nv, threadIdx.x >> 6);
 }
 
-// part[s](b x nc) = V(Ks, :)' * C(Ks, :) over the K slice Ks of split s,
-// computed transposed (C' V) so the accumulator stores whole columns of the
-// partial.  A workgroup owns 256 columns of C; wavefront w streams its own
-// 64 columns (against all 64 reflectors: 4 x 4 MFMA blocks, accumulators in
-// AGPRs) with no LDS and no barrier: both operands go from memory straight
-// into MFMA operand registers.  K is permuted inside each 16-row chunk: in
-// MFMA substep e (0..3) lane l supplies row 4*(l/16) + e of the chunk for
-// both operands, so every lane reads 4 consecutive doubles (32 bytes) of one
-// column of C and of V per chunk, and 16 lanes cover a column's 128-byte
-// line.  A four-chunk register ring keeps the next three chunks' loads in
-// flight while the matrix cores work on the current one.  VEC: the 32-byte groups are
-// 32-byte aligned (vector loads); otherwise the same values are loaded one
-// double at a time, so both forms compute identical bits.
-constexpr int kVN = 256;  // columns of C per workgroup
-constexpr int kCK = 16;   // K rows per chunk
+// The partials of M = V'[C C2] on the stream-K partition (VtcPart): each
+// wavefront walks its run of (64-column group, 16-row chunk) units against
+// all 64 reflectors (4 x 4 MFMA blocks, accumulators in AGPRs), computed
+// transposed (C' V) so the accumulator stores whole columns of a partial,
+// and writes its accumulators to the group's partial slice whenever its run
+// leaves a group.  No LDS and no barrier: both operands go from memory
+// straight into MFMA operand registers.  K is permuted inside each 16-row
+// chunk: in MFMA substep e (0..3) lane l supplies row 4*(l/16) + e of the
+// chunk for both operands, so every lane reads 4 consecutive doubles (32
+// bytes) of one column of C and of V per chunk, and 16 lanes cover a
+// column's 128-byte line.  A four-chunk register ring keeps the next three
+// chunks' loads in flight (across group boundaries) while the matrix cores
+// work on the current one.  Every wavefront runs q or q + 1 chunks, so all
+// CUs finish together whatever the column count.  VEC: the 32-byte groups
+// are 32-byte aligned (vector loads); otherwise the same values are loaded
+// one double at a time, so both forms compute identical bits.
+constexpr int kCK = kVtcChunk;  // K rows per chunk
 constexpr int kRing = 4;  // chunks of the register ring (kRing - 1 in flight)
 template <bool VEC, int RING = kRing>
 __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lane = threadIdx.x & 63;
   const int l15 = lane & 15, g4 = lane >> 4;
-  const long n0 = (long)blockIdx.x * kVN + wave * 64;
-  const int s = blockIdx.y;
-  const long kbeg = (long)s * a.kchunk;
-  const long kend = min((long)a.mrows, kbeg + a.kchunk);
+  const VtcPart vp = vtc_part(a);
+  const long w = (long)blockIdx.x * (HQ_NT / 64) +
+                 __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long u0 = vp.start(w);
+  const long nu = vp.start(w + 1) - u0;
+  if (nu <= 0) return;
+  const long nch = vp.nch, kend = a.mrows;
   const int b = a.b;
 
-  const double* cp[4];
-  const double* vp[4];
-  bool cok[4], vok[4];
+  const double* vptr[4];
+  bool vok[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const double* col = vcol(a, n0 + 16 * i + l15);
-    cok[i] = col != nullptr;
-    cp[i] = (cok[i] ? col : a.C) + 4 * g4;
     const int p = 16 * i + l15;
     vok[i] = p < b;
-    vp[i] = a.V + (long)(vok[i] ? p : 0) * a.ldv + 4 * g4;
+    vptr[i] = a.V + (long)(vok[i] ? p : 0) * a.ldv + 4 * g4;
   }
+  // Load cursor: group lg, chunk lc of the next unit to load; cp/cok are
+  // that group's columns.
+  long lg = u0 / nch, lc = u0 - lg * nch, ld_left = nu;
+  const double* cp[4];
+  bool cok[4];
+  auto set_cols = [&](long g) {
+    const long v0 = vp.vcol0(g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double* col = vcol(a, v0 + 16 * i + l15);
+      cok[i] = col != nullptr;
+      cp[i] = (cok[i] ? col : a.C) + 4 * g4;
+    }
+  };
+  set_cols(lg);
 
   hq_d4 acc[4][4];  // [n block][p block]
 #pragma unroll
@@ -588,15 +604,9 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = hq_d4{0.0, 0.0, 0.0, 0.0};
 
-  // The K slice is walked starting at a per-column-tile rotation, so that
-  // concurrently running workgroups read different rows (spreading the
-  // power-of-two-strided columns over HBM channels).
-  const long nsteps = (kend - kbeg + kCK - 1) / kCK;
-  const long rot = nsteps > 0 ? ((long)blockIdx.x * 7) % nsteps : 0;
-
   hq_d4 cb[RING][4], vb[RING][4];
-  auto load = [&](hq_d4 (&cr)[4], hq_d4 (&vr)[4], long it) {
-    if (it >= nsteps) {
+  auto load = [&](hq_d4 (&cr)[4], hq_d4 (&vr)[4]) {
+    if (ld_left <= 0) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         cr[i] = hq_d4{0.0, 0.0, 0.0, 0.0};
@@ -604,7 +614,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
       }
       return;
     }
-    const long k0 = kbeg + ((it + rot) % nsteps) * kCK;
+    const long k0 = lc * kCK;
     const long r0 = k0 + 4 * g4;  // this lane's first row
     if (VEC && k0 + kCK <= kend) {
 #pragma unroll
@@ -612,7 +622,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
         cr[i] = cok[i] ? __builtin_nontemporal_load(
                              reinterpret_cast<const hq_d4*>(cp[i] + k0))
                        : hq_d4{0.0, 0.0, 0.0, 0.0};
-        vr[i] = vok[i] ? *reinterpret_cast<const hq_d4*>(vp[i] + k0)
+        vr[i] = vok[i] ? *reinterpret_cast<const hq_d4*>(vptr[i] + k0)
                        : hq_d4{0.0, 0.0, 0.0, 0.0};
       }
     } else {
@@ -622,7 +632,7 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
         for (int e = 0; e < 4; ++e) {
           const bool rok = r0 + e < kend;
           cr[i][e] = (cok[i] && rok) ? cp[i][k0 + e] : 0.0;
-          vr[i][e] = (vok[i] && rok) ? vp[i][k0 + e] : 0.0;
+          vr[i][e] = (vok[i] && rok) ? vptr[i][k0 + e] : 0.0;
         }
     }
     if (a.vmask && k0 < b) {  // the top b x b block of V is unit lower
@@ -630,6 +640,12 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) vr[i][e] = vmask(vr[i][e], r0 + e, 16 * i + l15);
+    }
+    --ld_left;
+    if (++lc == nch) {
+      lc = 0;
+      ++lg;
+      if (ld_left > 0) set_cols(lg);
     }
   };
   auto compute = [&](const hq_d4 (&cr)[4], const hq_d4 (&vr)[4]) {
@@ -642,33 +658,47 @@ __global__ __launch_bounds__(HQ_NT, 1) void trail_vtc_kernel(TrailArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[i][e], vr[j][e],
                                                             acc[i][j], 0, 0, 0);
   };
-  // RING-chunk ring, the step count padded to a multiple of RING (padding
-  // chunks are zeros): constant ring indices, no early exit.  Before chunk q
-  // is multiplied, chunk q + RING - 1 is loaded into the slot chunk q - 1
-  // left.
+  // Group cg's partial: acc[i][j] element r of lane l is partial(p, n) with
+  // p = 16j + l%16 and n = vcol0(cg) + 16i + l/16 + 4r.
+  long cg = u0 / nch;
+  long cnext = (cg + 1) * nch - u0;  // first unit (relative) of group cg + 1
+  auto flush = [&]() {
+    double* out = a.part + (w - vp.first(cg)) * b * a.ncv;
+    const long v0 = vp.vcol0(cg);
 #pragma unroll
-  for (int q = 0; q + 1 < RING; ++q) load(cb[q], vb[q], q);
-  for (long it = 0; it < nsteps; it += RING) {
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int p = j * 16 + l15;
+          const long gn = v0 + i * 16 + g4 + 4 * r;
+          if (p < b && gn < a.ncv) out[p + gn * b] = acc[i][j][r];
+        }
+  };
+  // Before unit u is multiplied, unit u + RING - 1 is loaded into the slot
+  // unit u - 1 left; ring indices are compile-time constants.
+#pragma unroll
+  for (int q = 0; q + 1 < RING; ++q) load(cb[q], vb[q]);
+  for (long it = 0; it < nu; it += RING) {
 #pragma unroll
     for (int q = 0; q < RING; ++q) {
-      load(cb[(q + RING - 1) % RING], vb[(q + RING - 1) % RING],
-           it + q + RING - 1);
-      compute(cb[q], vb[q]);
+      load(cb[(q + RING - 1) % RING], vb[(q + RING - 1) % RING]);
+      if (it + q < nu) {
+        if (it + q == cnext) {
+          flush();
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = hq_d4{0.0, 0.0, 0.0, 0.0};
+          ++cg;
+          cnext += nch;
+        }
+        compute(cb[q], vb[q]);
+      }
     }
   }
-  // acc[i][j] element r of lane l: partial(p, n) with p = 16j + l%16 and
-  // n = n0 + 16i + l/16 + 4r.
-  double* out = a.part + (long)s * b * a.ncv;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int p = j * 16 + l15;
-        const long gn = n0 + i * 16 + g4 + 4 * r;
-        if (p < b && gn < a.ncv) out[p + gn * b] = acc[i][j][r];
-      }
+  flush();
 }
 
 // The same product on 64-column tiles (4 wavefronts of 32 x 32, two
@@ -768,12 +798,13 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
   const int t = threadIdx.x, b = a.b;
   const long n0 = (long)blockIdx.x * kTile;
   const long slice = (long)b * a.ncv;
+  const int S = a.sk ? vtc_part(a).count(n0 / 64) : a.splits;
   for (int idx = t; idx < b * kTile; idx += HQ_NT) {
     int p = idx % b, n = idx / b;
     long gn = n0 + n;
     double s = 0.0;
     if (gn < a.nc)
-      for (int q = 0; q < a.splits; ++q) s += a.part[q * slice + p + gn * b];
+      for (int q = 0; q < S; ++q) s += a.part[q * slice + p + gn * b];
     Ms[p][n] = s;
   }
   for (int idx = t; idx < b * b; idx += HQ_NT) {
@@ -793,6 +824,21 @@ __global__ __launch_bounds__(HQ_NT) void trail_tw_kernel(TrailArgs a) {
     }
     a.W[gn + (long)q * a.ldw] = s;  // W' (nc x b), ld ldw
   }
+}
+
+// Y(p, n) = sum over the partial slices of column n, in slice order (the
+// product V'C alone, for callers without T).
+__global__ __launch_bounds__(HQ_NT) void trail_sum_kernel(TrailArgs a,
+                                                          double* Y, long ldy) {
+  const long idx = (long)blockIdx.x * HQ_NT + threadIdx.x;
+  const int b = a.b;
+  if (idx >= (long)b * a.nc) return;
+  const long p = idx % b, n = idx / b;
+  const int S = a.sk ? vtc_part(a).count(n / 64) : a.splits;
+  const long slice = (long)b * a.ncv;
+  double s = 0.0;
+  for (int q = 0; q < S; ++q) s += a.part[q * slice + idx];
+  Y[p + n * ldy] = s;
 }
 
 }  // namespace
@@ -839,6 +885,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
     return;
   }
   const long slice = (long)b * a.ncv;
+  const int S = a.sk ? a.vp.count(a.vp.group(v0)) : a.S;  // partial slices
   const int cw = wave * 16 + l15;  // this lane's column of the tile
   const bool cok = c0 + cw < nc;
   const long gc = c0 + cw;
@@ -868,7 +915,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         top[q][e] = Cb[q * 16 + l4 + 4 * e + gc * ldc];
-    const int S4 = a.S & ~3;
+    const int S4 = S & ~3;
     for (int q0 = 0; q0 < S4; q0 += 4) {
       double x[4][kPer];
 #pragma unroll
@@ -882,7 +929,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
 #pragma unroll
         for (int u = 0; u < kPer; ++u) msum[u] += x[k][u];
     }
-    for (int q = S4; q < a.S; ++q) {
+    for (int q = S4; q < S; ++q) {
       const double* src = a.part + q * slice + off0;
       double x[kPer];
 #pragma unroll
@@ -913,11 +960,11 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
     // The slices four at a time: 4 x kPer independent loads in flight, then
     // added in slice order.
     const bool pok = (t & 63) < b;
-    for (int q0 = 0; q0 < a.S; q0 += 4) {
+    for (int q0 = 0; q0 < S; q0 += 4) {
       double x[4][kPer];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const bool sok = q0 + k < a.S;
+        const bool sok = q0 + k < S;
         const double* src = a.part + (q0 + k) * slice + off0;
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
@@ -1106,14 +1153,15 @@ hipError_t launch_trail_y2(const TailArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
-  if (a.wide) {
-    dim3 grid(hq_cdiv(a.ncv, kVN), a.splits);
-    // 32-byte groups of rows k0 + 4g (k0 a multiple of 16 from the slice
-    // start, which is a multiple of trail_kbk()) are 32-byte aligned when
-    // both bases and leading dimensions are.
+  if (a.sk) {
+    if (a.sk_waves < 1 || a.sk_waves % (HQ_NT / 64) || a.b > 64)
+      return hipErrorInvalidValue;
+    dim3 grid(a.sk_waves / (HQ_NT / 64));
+    // 32-byte groups of rows 16c + 4g are 32-byte aligned when both bases
+    // and leading dimensions are.
     const bool vec = (reinterpret_cast<uintptr_t>(a.C) % 32 == 0) &&
                      (reinterpret_cast<uintptr_t>(a.V) % 32 == 0) &&
-                     a.ldc % 4 == 0 && a.ldv % 4 == 0 && a.kchunk % 4 == 0 &&
+                     a.ldc % 4 == 0 && a.ldv % 4 == 0 &&
                      (!a.C2 || (reinterpret_cast<uintptr_t>(a.C2) % 32 == 0 &&
                                 a.ldc2 % 4 == 0));
     if (vec)
@@ -1124,6 +1172,14 @@ hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s) {
     dim3 grid(hq_cdiv(a.ncv, kTile), a.splits);
     hipLaunchKernelGGL(trail_vtc64_kernel, grid, dim3(HQ_NT), 0, s, a);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_trail_sum(const TrailArgs& a, double* Y, long ldy,
+                            hipStream_t s) {
+  if (a.C2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(trail_sum_kernel, dim3(hq_cdiv((long)a.b * a.nc, HQ_NT)),
+                     dim3(HQ_NT), 0, s, a, Y, ldy);
   return hipGetLastError();
 }
 
@@ -1160,4 +1216,3 @@ hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s) {
 
 int trail_kbk() { return kBK64; }  // multiple of kBK too
 
-int trail_vtc_cols() { return kVN; }

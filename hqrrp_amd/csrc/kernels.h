@@ -102,7 +102,6 @@ struct GemmArgs {
 struct TrailArgs {
   int mrows, nc, b, splits;
   int vmask;        // 1: V's top b x b block is unit lower (masked)
-  int wide;         // trail_vtc: 256-column tiles (else 64)
   int tnotrans;     // trail_tw: W = T M (apply Q) instead of T' M (apply Q')
   const double* V;  // mrows x b reflectors (unit lower trapezoidal), ld ldv
   long ldv;
@@ -127,7 +126,47 @@ struct TrailArgs {
   // trail_update: CUs the launch stream may use (0: the per-tile kernel
   // only); the persistent band kernel takes one workgroup per CU.
   int ncu;
+  // V'C partition (sk = 1; else `splits` uniform K slices of kchunk rows):
+  // see VtcPart.
+  int sk, sk_waves;
+  long sk_nch, sk_q, sk_r;
 };
+
+// The stream-K partition of V'[C C2] (trail_vtc with sk = 1).  A unit is
+// (group g, chunk c): 64 virtual columns by kVtcChunk rows, numbered u = g *
+// nch + c over C's ceil(nc / 64) groups, then C2's.  The units are dealt in
+// consecutive runs to the wavefronts: wavefront w takes [start(w),
+// start(w + 1)), q units each and one more for the first r.  A wavefront
+// writes one partial per group it touches, into slice w - first(g) of the
+// partials, so group g's sum is over count(g) slices, added in slice order.
+constexpr int kVtcChunk = 16;
+struct VtcPart {
+  long nch, q, r;
+  int nc, nca;
+  __host__ __device__ long start(long w) const {
+    return w * q + (w < r ? w : r);
+  }
+  __host__ __device__ long wave_of(long u) const {
+    const long big = r * (q + 1);
+    return u < big ? u / (q + 1) : r + (u - big) / q;
+  }
+  __host__ __device__ long first(long g) const { return wave_of(g * nch); }
+  __host__ __device__ int count(long g) const {
+    return (int)(wave_of((g + 1) * nch - 1) - wave_of(g * nch) + 1);
+  }
+  __host__ __device__ int ngc() const { return (nc + 63) / 64; }
+  // First virtual column of group g; the group of virtual column v (a
+  // multiple of 64 in C's or C2's range).
+  __host__ __device__ long vcol0(long g) const {
+    return g < ngc() ? 64 * g : nca + 64 * (g - ngc());
+  }
+  __host__ __device__ long group(long v) const {
+    return v < nca ? v / 64 : ngc() + (v - nca) / 64;
+  }
+};
+__host__ __device__ inline VtcPart vtc_part(const TrailArgs& a) {
+  return VtcPart{a.sk_nch, a.sk_q, a.sk_r, a.nc, a.nca};
+}
 
 // k_downdate.hip: FP64 sketch downdate of one block (r <= 80, b <= 64).
 struct DowndateArgs {
@@ -203,6 +242,8 @@ struct TailArgs {
   int b, nc, S;
   int nc2, nca, ncv;   // C2's columns, as in TrailArgs
   const double* part;  // S slices of b x ncv (trail_vtc's partials)
+  int sk;              // the partials follow the stream-K partition vp
+  VtcPart vp;
   const double* T;     // b x b upper triangular, ld ldt
   long ldt;
   const double* V;  // the block's reflectors (top b x b read unit lower)
@@ -225,9 +266,11 @@ hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s);
 hipError_t launch_trail_y2(const TailArgs& a, hipStream_t s);
 hipError_t launch_trail_vtc(const TrailArgs& a, hipStream_t s);
 hipError_t launch_trail_tw(const TrailArgs& a, hipStream_t s);
+// Y(p, n) = sum of the partials of V'C (b x nc, no C2), ld ldy.
+hipError_t launch_trail_sum(const TrailArgs& a, double* Y, long ldy,
+                            hipStream_t s);
 hipError_t launch_trail_update(const TrailArgs& a, hipStream_t s);
 int trail_kbk();
-int trail_vtc_cols();  // columns of C per wide trail_vtc workgroup
 
 // k_sketch.hip
 size_t sketch_xchg_doubles(int nw, int rmax, int stages);

@@ -12,20 +12,33 @@
 // k0 + l/16), so the result is bitwise what the full trailing update writes
 // for those columns.  acc[i][j] element e: P(m = mw + 16j + l%16,
 // q = 16i + l/16 + 4e).  Rows m >= rows and positions q >= b2 read as 0.
+//
+// The selected rows of W' are gathered once per workgroup into `ws` (LDS,
+// at least 64 x kSelLd doubles): every element of W'(src[q], k) lies in a
+// cache line of its own, so wavefronts gathering them each for themselves
+// multiply that traffic by the wavefront count.  Every thread of the
+// workgroup calls select_tile; it ends with a barrier, after which `ws` may
+// be reused.
 #pragma once
 #include "common.h"
+
+constexpr int kSelLd = 65;  // LDS leading dimension of the gathered W' rows
 
 __device__ __forceinline__ void select_tile(hq_d4 (&acc)[4][2],
                                             const int* src, int b2,
                                             const double* C, long ldc, int r0,
                                             int rows, const double* V,
                                             long ldv, const double* Wt,
-                                            long ldw, int b, long mw) {
+                                            long ldw, int b, long mw,
+                                            double* ws) {
   const int lane = threadIdx.x & 63;
   const int l15 = lane & 15, l4 = lane >> 4;
-  int wsrc[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) wsrc[i] = (16 * i + l15 < b2) ? src[16 * i + l15] : -1;
+  // ws[q * kSelLd + k] = W'(src[q], k), zero for q >= b2 or k >= b.
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
+    const int q = idx & 63, k = idx >> 6;
+    ws[q * kSelLd + k] = (q < b2 && k < b) ? Wt[src[q] + (long)k * ldw] : 0.0;
+  }
+  __syncthreads();
   if (b == 64 && b2 == 64 && mw + 32 <= rows) {
     // Every operand exists: C and the first operand steps are loaded
     // without predicates, in flight together, and a four-step register ring
@@ -42,7 +55,7 @@ __device__ __forceinline__ void select_tile(hq_d4 (&acc)[4][2],
     auto load = [&](double (&af)[4], double (&bf)[2], int st) {
       const int k = 4 * st + l4;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = Wt[wsrc[i] + (long)k * ldw];
+      for (int i = 0; i < 4; ++i) af[i] = ws[(16 * i + l15) * kSelLd + k];
 #pragma unroll
       for (int j = 0; j < 2; ++j) bf[j] = -V[mw + 16 * j + l15 + (long)k * ldv];
     };
@@ -69,6 +82,7 @@ __device__ __forceinline__ void select_tile(hq_d4 (&acc)[4][2],
       if (st + 6 < 16) load(ra[2], rb[2], st + 6);
       mma(ra[3], rb[3]);
     }
+    __syncthreads();
     return;
   }
 #pragma unroll
@@ -88,8 +102,7 @@ __device__ __forceinline__ void select_tile(hq_d4 (&acc)[4][2],
     const int k = kk + l4;
     double af[4], bf[2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      af[i] = (k < b && wsrc[i] >= 0) ? Wt[wsrc[i] + (long)k * ldw] : 0.0;
+    for (int i = 0; i < 4; ++i) af[i] = ws[(16 * i + l15) * kSelLd + k];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const long m = mw + 16 * j + l15;
@@ -102,4 +115,5 @@ __device__ __forceinline__ void select_tile(hq_d4 (&acc)[4][2],
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
                                                           acc[i][j], 0, 0, 0);
   }
+  __syncthreads();
 }

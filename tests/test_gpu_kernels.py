@@ -134,7 +134,11 @@ def test_panel_matches_oracle(handle, mrows, b, pivoting, seed):
 
 # ------------------------------------------------------------ dlarfb
 @pytest.mark.parametrize("mrows,ncols,b", [(1000, 700, 64), (300, 5, 16),
-                                           (64, 100, 64), (4000, 3000, 64)])
+                                           (64, 100, 64), (4000, 3000, 64),
+                                           # V'C on the stream-K partition:
+                                           # ragged groups, then a partial
+                                           # last chunk with odd ld and b < 64
+                                           (8192, 4100, 64), (8201, 4096, 40)])
 def test_apply_qt_left(handle, mrows, ncols, b):
     print(f"apply_qt mrows={mrows} ncols={ncols} b={b}", flush=True)
     rng = np.random.default_rng(mrows + ncols)
