@@ -54,10 +54,11 @@ PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 # HBM bytes per launch of a phase's kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and --pmc WRITE_SIZE on this workload (summary and commands in
 # profiles/hqrrp_16384_hbm_counters.md), each counter scaled by what it reads
-# for a plain copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
+# for a plain copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0):
+# the KiB summed over the phase's kernels of all 255 blocks, per block.
 # Keyed by (phase, m, n, nb); other shapes report null.
 MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64):
-                    round((551954.8 * 2.0 + 702442.8) * 1024)}
+                    round((106355800.5625 * 2.0 + 179232166.25) * 1024 / 255)}
 
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {

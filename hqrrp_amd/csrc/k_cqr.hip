@@ -602,8 +602,15 @@ __device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
     const int k1 = min(k0 + 16, b);
     if (wave == 0) {
       bool ok = true;
-#pragma nounroll
-      for (int k = k0; k < k1; ++k) {
+      // This lane's rows k0.. of R (registers; the block's steps unrolled
+      // so that the index is static).
+      double rc[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) rc[i] = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) {
+        const int k = k0 + kk;
+        if (k >= k1 || !ok) break;
         // LAWN 176 downdate with row k-1 of R (NoFLA_HQRRP_WY_blk_var4.c:
         // 941-986), in the same quantities without divisions: with t = |r|
         // / vn1, (1 + t)(1 - t) vn1^2 = (vn1 - |r|)(vn1 + |r|) and the
@@ -636,24 +643,29 @@ __device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
         } else if (pend && pos == k) {
           pos = pk;
         }
+        // The block's earlier rows: R(k0:k, p)' R(k0:k, c), over all 16 rows
+        // of the block so that every load is in flight at once: rows k.. of
+        // Rk are still zero and add exact zeros.
+        const double spc = S[p + LD * c];
+        double rp[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) rp[i] = Rk[k0 + i + LD * p];
         const double dp = hq_readlane(d, p), gp = hq_readlane(gc, p);
         if (!(dp > kChol1Thr * gp)) {
           ok = false;
           break;
         }
         const double rkk = sqrt(dp), rinv = 1.0 / rkk;
-        // The block's earlier rows: R(k0:k, p)' R(k0:k, c); rows k.. of Rk
-        // are still zero, so the four-way groups need no masks.
         double s4[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = k0; i < k; i += 4) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            s4[u] = __builtin_fma(Rk[i + u + LD * p], Rk[i + u + LD * c], s4[u]);
-        }
+        for (int i = 0; i < 16; i += 4)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) s4[u] = __builtin_fma(rp[i + u], rc[i + u], s4[u]);
         const double dot = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-        const double r = (c == p) ? rkk : (pend ? (S[p + LD * c] - dot) * rinv : 0.0);
+        const double r = (c == p) ? rkk : (pend ? (spc - dot) * rinv : 0.0);
         if (pend) d = __builtin_fma(-r, r, d);
         rprev = r;
+        rc[kk] = r;
         Rk[k + LD * c] = r;  // row k is not read before the next step
         wave_sync();
       }
