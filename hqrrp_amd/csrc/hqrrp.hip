@@ -1396,8 +1396,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
         PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
         HQ_TRY(launch_larft(b2, h->gram.as<double>(), b2, dtau + jn, T, nb,
                             h->cur));
+        HQ_TRY(hipEventRecord(h->ev_t, h->s_chain));
       }
-      HQ_TRY(hipEventRecord(h->ev_t, h->s_chain));
       return HQRRP_OK;
     };
     if (rc == HQRRP_OK) rc = chain();
@@ -1405,7 +1405,9 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     HQ_RET(rc);
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_upd, 0));
     HQ_TRY(hipStreamWaitEvent(main_s, h->ev_chain, 0));
-    t_pending = true;
+    // The Cholesky-QR panel (and its gated fallback) wrote T before
+    // ev_chain; the per-step panel's T follows on s_chain (ev_t).
+    t_pending = h->panel_alg != 0;
     // Join: block jn's factored panel over its gathered columns and the
     // rows above it permuted, in one launch.
     {
@@ -1756,11 +1758,15 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
     HQ_TRY(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
     HQ_TRY(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
     if (h->s_chain) {
-      HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
-      HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, hipEventDisableTiming));
-      HQ_TRY(hipEventCreateWithFlags(&h->ev_chain, hipEventDisableTiming));
-      HQ_TRY(hipEventCreateWithFlags(&h->ev_sketch, hipEventDisableTiming));
-      HQ_TRY(hipEventCreateWithFlags(&h->ev_t, hipEventDisableTiming));
+      // The look-ahead's events only order device work between this
+      // device's queues: a device-scope release is enough (no system-scope
+      // cache writeback and invalidation at each fork and join).
+      const unsigned f = hipEventDisableTiming | hipEventDisableSystemFence;
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, f));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, f));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_chain, f));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_sketch, f));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_t, f));
     }
   }
   int rc = ensure(h->words, 256);
