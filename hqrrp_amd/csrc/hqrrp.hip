@@ -626,7 +626,7 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
                int stages, int* dpiv, int* dmoves, int f32 = 0) {
   int rmax = sketch_rmax(r);
   if (rmax < 0 || stages > sketch_max_stages() ||
-      hq_cdiv(ncols, sketch_cols_per_wg()) > h->sketch_max_wg)
+      hq_cdiv(ncols, sketch_cols_per_wg(ncols)) > h->sketch_max_wg)
     return run_sketch_generic(h, r, ncols, dY, ldy, stages, dpiv, dmoves, f32);
   SketchArgs a{};
   a.Y = dY;
@@ -635,7 +635,8 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
   a.r = r;
   a.ncols = ncols;
   a.stages = stages;
-  a.nw = hq_cdiv(ncols, sketch_cols_per_wg());
+  a.cols = sketch_cols_per_wg(ncols);
+  a.nw = hq_cdiv(ncols, a.cols);
   if (a.nw > h->sketch_max_wg) return HQRRP_ERR_UNSUPPORTED;
   const size_t need = sketch_xchg_doubles(a.nw, rmax, stages);
   HQ_RET(xchg_begin(h, h->xs, need, &a.xcur, &a.xprev, &a.xprev_len));
@@ -1293,7 +1294,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
         use_la && jn < kmax &&
         panel_geometry(h, m - jn, &nw_panel) == HQRRP_OK &&
         nw_panel <= h->chain_cus &&
-        hq_cdiv(n - jn, sketch_cols_per_wg()) <= h->chain_cus;
+        hq_cdiv(n - jn, sketch_cols_per_wg(n - jn)) <= h->chain_cus;
     if (!la && xc != 0) {
       // The in-place update below needs the trailing matrix back in A.
       PhaseScope ps(h, PH_PANEL_POST, 0.0, 16.0 * (double)mrows * nc, 1);

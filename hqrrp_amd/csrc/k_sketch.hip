@@ -7,7 +7,9 @@
 // Design (MI355X): one persistent launch per block of b pivots.  Two lanes of
 // a wavefront own one sketch column and keep it in registers for all b
 // steps; a 512-thread workgroup holds 256 columns, so n = 16384 needs 64
-// workgroups.  A step is:
+// workgroups; up to 8192 columns, 256-thread workgroups of 128 columns (one
+// wavefront per SIMD instead of two: a step's reflector application, which
+// is bound by the SIMDs' FP64 issue, takes half as long).  A step is:
 //   1. per-workgroup argmax of the partial norms (first position on ties),
 //      the winning column published to this step's exchange slot;
 //   2. every workgroup reads every slot header as soon as it is written (no
@@ -33,8 +35,10 @@ constexpr int kSlotHead = 8;  // slot: [key, pos, alpha, ||x||^2, pad, rows]
 // Slots start 1 KB apart, so no cache line holds parts of two workgroups'
 // slots (lines written by several workgroups slow the polling).
 constexpr long slot_stride(long used) { return (used + 127) / 128 * 128; }
-constexpr int kThreads = 512;  // 8 wavefronts, 32 columns each
-constexpr int kCols = 256;     // sketch columns per workgroup
+// A workgroup of NT threads (NT / 64 wavefronts, 32 columns each) holds
+// NT / 2 columns: 512 threads (256 columns), or 256 (128 columns) when every
+// column fits in kNarrowWg workgroups.
+constexpr int kNarrowWg = 64;
 constexpr int kMaxWg = 256;    // workgroups of one launch at most
 constexpr int kMaxStages = 80;
 
@@ -56,14 +60,15 @@ constexpr int kMaxStages = 80;
 //
 // T is the sketch's arithmetic: double, or float for the mixed-precision
 // sketch (values travel through the exchange as doubles, exactly).
-template <class T, int RMAX>
-__global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
+template <class T, int RMAX, int NT>
+__global__ __launch_bounds__(NT) void sketch_qrcp_kernel(SketchArgs a) {
+  constexpr int NW = NT / 64;  // wavefronts
   constexpr int H = RMAX / 2;
   const int t = threadIdx.x, w = blockIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int half = lane >> 5;
   const int r = a.r;
-  const int c = w * kCols + wave * 32 + (lane & 31);
+  const int c = w * (NT / 2) + wave * 32 + (lane & 31);
   const bool own = c < a.ncols;
   const int ibase = half * H;  // first active row (relative) of this lane
   const long slot = kSlotHead + RMAX;
@@ -126,14 +131,15 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     __syncthreads();
     {
-      // Every wavefront combines the 8 wavefront winners (lane l < 8 holds
-      // winner l; identical result in every lane); bq: the winner's
+      // Every wavefront combines the NW wavefront winners (lane l < NW
+      // holds winner l; identical result in every lane); bq: the winner's
       // wavefront.
-      const int wq = (lane < 8) ? s_pos[lane] : INT_MAX;
-      double bk = (lane < 8) ? s_key[lane] : -1.0;
+      const int wq = (lane < NW) ? s_pos[lane] : INT_MAX;
+      double bk = (lane < NW) ? s_key[lane] : -1.0;
       int bp = wq;
       hq_wave_argmax64(bk, bp);
-      const int bq = __builtin_ctzll(__ballot(wq == bp) & 0xFFull);
+      const int bq =
+          __builtin_ctzll(__ballot(wq == bp) & ((1ull << NW) - 1ull));
       if (bp == INT_MAX) {  // no candidate here
         if (t == 0) {
           hq_publish(mine + 0, -1.0);
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void sketch_qrcp_kernel(SketchArgs a) {
     }
     if (lane == 0) a.moves[0] = base;
   }
-  hq_xreset(a.xprev, a.xprev_len, w, a.nw, kThreads);
+  hq_xreset(a.xprev, a.xprev_len, w, a.nw, NT);
 }
 
 }  // namespace
@@ -345,7 +351,9 @@ size_t sketch_xchg_doubles(int nw, int rmax, int stages) {
   return (size_t)stages * nw * slot_stride(kSlotHead + rmax);
 }
 
-int sketch_cols_per_wg() { return kCols; }
+int sketch_cols_per_wg(int ncols) {
+  return ncols <= kNarrowWg * 128 ? 128 : 256;
+}
 
 int sketch_max_stages() { return kMaxStages; }
 
@@ -355,28 +363,35 @@ int sketch_rmax(int r) {
   return -1;
 }
 
-hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax,
-                              hipStream_t stream) {
-  if (a.nw < 1 || a.nw > kMaxWg || a.stages < 1 || a.stages > kMaxStages)
-    return hipErrorInvalidValue;
-  dim3 grid(a.nw), block(kThreads);
+template <int NT>
+hipError_t launch_sketch_nt(const SketchArgs& a, int rmax, hipStream_t stream) {
+  dim3 grid(a.nw), block(NT);
   if (a.f32) {
     if (rmax == 32)
-      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 32>), grid, block, 0,
+      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 32, NT>), grid, block, 0,
                          stream, a);
     else if (rmax == 80)
-      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 80>), grid, block, 0,
+      hipLaunchKernelGGL((sketch_qrcp_kernel<float, 80, NT>), grid, block, 0,
                          stream, a);
     else
       return hipErrorInvalidValue;
   } else if (rmax == 32) {
-    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 32>), grid, block, 0,
+    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 32, NT>), grid, block, 0,
                        stream, a);
   } else if (rmax == 80) {
-    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 80>), grid, block, 0,
+    hipLaunchKernelGGL((sketch_qrcp_kernel<double, 80, NT>), grid, block, 0,
                        stream, a);
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax,
+                              hipStream_t stream) {
+  if (a.nw < 1 || a.nw > kMaxWg || a.stages < 1 || a.stages > kMaxStages ||
+      (a.cols != 128 && a.cols != 256) || (long)a.nw * a.cols < a.ncols)
+    return hipErrorInvalidValue;
+  return a.cols == 128 ? launch_sketch_nt<256>(a, rmax, stream)
+                       : launch_sketch_nt<512>(a, rmax, stream);
 }

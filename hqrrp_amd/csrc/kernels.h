@@ -9,6 +9,7 @@ struct SketchArgs {
   long ldy;
   int f32;        // Y holds floats (the pivoting then runs in float)
   int r, ncols, stages, nw;
+  int cols;  // columns per workgroup: sketch_cols_per_wg(ncols)
   int* piv;    // stages: position swapped with position k at step k
   int* moves;  // [0] = count, then (dst, src) position pairs
   // Exchange buffers (common.h): xcur holds only the sentinel on entry
@@ -275,7 +276,7 @@ int trail_kbk();
 // k_sketch.hip
 size_t sketch_xchg_doubles(int nw, int rmax, int stages);
 int sketch_rmax(int r);
-int sketch_cols_per_wg();
+int sketch_cols_per_wg(int ncols);  // 128 or 256
 int sketch_max_stages();
 hipError_t launch_sketch_qrcp(const SketchArgs& a, int rmax, hipStream_t s);
 
