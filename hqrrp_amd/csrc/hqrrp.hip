@@ -255,6 +255,9 @@ struct hqrrp_handle_s {
   // keeps every panel on the per-step kernel.
   DevBuf cqr;
   int panel_alg = 0;
+  // The last launch on xp was the gated per-step kernel after a
+  // Cholesky-QR panel, whose flag (in cqr) says whether it wrote xp.
+  bool xp_after_cqr = false;
   DevBuf gen, genT, genM;  // generic block step: sketch copy / norms, T
                            // products, trailing products
   DevBuf Xb;             // look-ahead: second trailing-matrix buffer (m x n)
@@ -689,10 +692,12 @@ int run_panel_cqr(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
   {
     const size_t need = cqr_xchg_doubles(c.nw);
     HQ_RET(xchg_begin(h, h->xp, need, &c.xcur, &c.xprev, &c.xprev_len));
+    c.xprev_if = h->xp_after_cqr ? c.flag : nullptr;
     PhaseScope ps(h, PH_PANEL, 2.0 * mrows * b * b - 2.0 / 3.0 * b * b * b,
                   16.0 * mrows * b, 1);
     HQ_TRY(launch_cqr_panel(c, h->cur));
-    xchg_end(h->xp, need);
+    // The kernel restores its partials' sentinels itself.
+    xchg_end(h->xp, cqr_xchg_dirty());
   }
   // Fallback, gated on the flag: the per-step kernel on the panel as stored.
   PanelArgs a{};
@@ -714,6 +719,7 @@ int run_panel_cqr(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
     PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 2);
     HQ_TRY(launch_panel_qrcp(a, h->cur));
     xchg_end(h->xp, xneed);
+    h->xp_after_cqr = true;
     HQ_TRY(launch_larft(b, a.gram, b, dtau, dT, ldt, h->cur, c.flag));
   }
   return HQRRP_OK;
@@ -770,6 +776,7 @@ int run_panel(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
     HQ_TRY(launch_panel_qrcp(a, h->cur));
   }
   xchg_end(h->xp, need);
+  h->xp_after_cqr = false;
   if (!dT) return HQRRP_OK;
   PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
   HQ_TRY(launch_larft(b, a.gram, b, dtau, dT, ldt, h->cur));

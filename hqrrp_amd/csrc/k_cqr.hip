@@ -710,12 +710,16 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
   const int w = blockIdx.x, nw = a.nw, b = a.b;
   const int ng = hq_cdiv(nw, kGroup);
   const long mw = (long)w * kRows + 32 * wave;
-  double* xA1 = a.xcur;
+  // Exchange layout: the two published matrices first (left for the next
+  // launch to restore), then the two passes' partials, which every
+  // workgroup restores as soon as it knows they have been consumed.
+  const long npart = (long)(nw + ng) * kPart;
+  double* xC2 = a.xcur;
+  double* xC1 = xC2 + 4097;
+  double* xA1 = xC1 + 4097;
   double* xB1 = xA1 + (long)nw * kPart;
-  double* xC1 = xB1 + (long)ng * kPart;
-  double* xA2 = xC1 + 4097;
+  double* xA2 = xA1 + npart;
   double* xB2 = xA2 + (long)nw * kPart;
-  double* xC2 = xB2 + (long)ng * kPart;
   // LDS: pass phases use [0, 8 * 64 * kXl) (the M staging at [0, kMat)
   // first); workgroup 0's 64 x 64 work uses four matrices and tmp.
   double* mA = smem;
@@ -726,7 +730,7 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
   if (t == 0) s_abort = 0;
   // The previous launch's exchange region (the other buffer) gets its
   // sentinel back now, overlapping the loads below.
-  hq_xreset(a.xprev, a.xprev_len, w, nw, kT);
+  if (!a.xprev_if || *a.xprev_if != 0) hq_xreset(a.xprev, a.xprev_len, w, nw, kT);
 
   // ---- the panel's rows in registers (formed from the trailing matrix
   // under look-ahead, and stored: the fallback reads them)
@@ -797,6 +801,9 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
     __syncthreads();
   }
   int f = await_matrix(xC1, smem, a.err, &s_flag);
+  // Pass 1's partials are consumed once workgroup 0 has published (leaders
+  // read xA1 before publishing xB1, workgroup 0 read xB1 before xC1).
+  if (f >= 0) hq_xreset(xA1, npart, w, nw, kT);
   if (f != 0) return;
 
   // ---- pass 2: Q1 = P M1, G2 = Q1'Q1; workgroup 0: R2, the reconstruction.
@@ -901,6 +908,7 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
     __syncthreads();
   }
   f = await_matrix(xC2, smem, a.err, &s_flag);
+  if (f >= 0) hq_xreset(xA2, npart, w, nw, kT);
   if (f != 0) return;
 
   // ---- V = Q1 N2 below the panel's first b rows (V' = N2' Q1', so that 16
@@ -929,6 +937,8 @@ __global__ __launch_bounds__(kT) void cqr_panel_kernel(CqrArgs a) {
 }  // namespace
 
 int cqr_rows_per_wg() { return kRows; }
+
+size_t cqr_xchg_dirty() { return 2 * 4097; }
 
 size_t cqr_xchg_doubles(int nw) {
   const int ng = hq_cdiv(nw, kGroup);

@@ -66,10 +66,15 @@ struct CqrArgs {
   long ldt;
   double* R1;      // workspace: 64 x 64
   int* flag;       // set to 1 when the per-step kernel must take the panel
-  // Exchange buffers as in SketchArgs; cqr_xchg_doubles(nw) used.
+  // Exchange buffers as in SketchArgs: cqr_xchg_doubles(nw) used, of which
+  // the kernel restores the sentinel itself over all but the first
+  // cqr_xchg_dirty().  xprev_if: when set, xprev is restored only if
+  // *xprev_if != 0 (the previous launch was the gated per-step kernel after
+  // a Cholesky-QR panel, which wrote its buffer only if that flag was set).
   double* xcur;
   double* xprev;
   long xprev_len;
+  const int* xprev_if;
   int* err;
   // Look-ahead column selection as in PanelArgs (sel_C set); the formed
   // panel is stored into P.
@@ -84,6 +89,7 @@ struct CqrArgs {
 };
 int cqr_rows_per_wg();
 size_t cqr_xchg_doubles(int nw);
+size_t cqr_xchg_dirty();
 hipError_t launch_cqr_panel(const CqrArgs& a, hipStream_t s);
 
 struct GemmArgs {
