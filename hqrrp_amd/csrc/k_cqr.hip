@@ -593,6 +593,9 @@ __device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
   const double gc = S[c + LD * c];
   double d = gc;  // Schur diagonal: squared norm of the part below row k
   double vn1 = (c < b) ? sqrt(gc) : 0.0, vn2 = vn1;
+  // The pivot key: vn1^2 as the downdate forms it (the same order as vn1;
+  // the square root then stays off the argmax's path).
+  double q2 = (c < b) ? gc : 0.0;
   int pos = c;
   bool done = c >= b;
   double rprev = 0.0;
@@ -621,16 +624,18 @@ __device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
           double u2 = (vn1 - ar) * (vn1 + ar);
           u2 = (0.0 > u2) ? 0.0 : u2;
           if (u2 <= HQ_TOL3Z * (vn2 * vn2)) {
-            vn1 = (mrows - (k - 1) - 1 > 0) ? sqrt(d > 0.0 ? d : 0.0) : 0.0;
+            q2 = (mrows - (k - 1) - 1 > 0) ? (d > 0.0 ? d : 0.0) : 0.0;
+            vn1 = sqrt(q2);
             vn2 = vn1;
           } else {
+            q2 = u2;
             vn1 = sqrt(u2);
           }
         }
         // idamax over positions k..b-1 (largest norm, first position).
         int pk = k;
         if (pivoting) {
-          double key = done ? -1.0 : hq_pivot_key(vn1);
+          double key = done ? -1.0 : hq_pivot_key(q2);
           int q = done ? INT_MAX : pos;
           hq_wave_argmax64(key, q);
           pk = q;
@@ -655,7 +660,12 @@ __device__ __forceinline__ bool chol_pivoted(double* S, double* Rk, int b,
           ok = false;
           break;
         }
-        const double rkk = sqrt(dp), rinv = 1.0 / rkk;
+        // 1 / sqrt(dp): the hardware estimate and two Newton steps.
+        double rinv = __builtin_amdgcn_rsq(dp);
+#pragma unroll
+        for (int it = 0; it < 2; ++it)
+          rinv = rinv * __builtin_fma(-0.5 * dp * rinv, rinv, 1.5);
+        const double rkk = dp * rinv;
         double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < 16; i += 4)
