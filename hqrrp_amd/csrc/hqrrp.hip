@@ -233,6 +233,7 @@ struct hqrrp_handle_s {
   hipStream_t s_chain = nullptr, s_upd = nullptr;
   int chain_cus = 0;
   hipEvent_t ev_fork = nullptr, ev_upd = nullptr, ev_chain = nullptr;
+  hipEvent_t ev_perm = nullptr;  // look-ahead: the panel copied into A
   hipEvent_t ev_sketch = nullptr, ev_t = nullptr;  // look-ahead: moves, T
   // Joins the caller's stream to the handle's own stream when the caller's
   // is the null stream (whose implicit synchronisation with every blocking
@@ -249,7 +250,7 @@ struct hqrrp_handle_s {
   // Sketch precision of the running factorisation: Y and G hold floats
   // (sk32) and the sketch GEMM rounds its operands to bf16 (skbf).
   int sk32 = 0, skbf = 0;
-  DevBuf Pn;  // look-ahead: the next panel
+  DevBuf Pn;  // look-ahead: the next panel (two, used in turn)
   // Cholesky-QR panel (k_cqr.hip): R1 (64 x 64) and the fallback flag.
   // panel_alg 0 selects it (with the per-step kernel as its fallback), 1
   // keeps every panel on the per-step kernel.
@@ -1205,7 +1206,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
   const bool use_la = o.lookahead != 0 && h->s_chain && h->s_upd &&
                       kmax > nb && n > nb && !gen_trail;
   if (use_la) {
-    HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)nb * m));
+    HQ_RET(ensure(h->Pn, sizeof(double) * (size_t)2 * nb * m));
     HQ_RET(ensure(h->Xb, sizeof(double) * (size_t)m * n));
   }
   void* Y = h->Y.p;  // FP64 or FP32 (h->sk32)
@@ -1277,11 +1278,19 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     const bool fin = (j + nb >= m) || (j + nb >= n);
     return fin ? w : std::min(w, kmax - j);
   };
+  // A look-ahead panel's reflectors are read from its buffer in Pn (ld its
+  // row count): its copy into A runs on s_upd beside the next V'C.
+  double* Vp = nullptr;
+  long ldvp = 0;
+  int pn_par = 0;
+  bool perm_pending = false;  // a panel copy on s_upd not yet joined (ev_perm)
   for (int j = 0; j < kmax; j += nb) {
     const int b = block_width(j);
     const bool last = (j + nb >= m) || (j + nb >= n);
     const int mrows = m - j, nc = n - j - b;
     double* Ajj = dA + j + (long)j * lda;
+    double* Vj = have_panel ? Vp : Ajj;  // the block's reflectors
+    const long ldvj = have_panel ? ldvp : lda;
     if (!have_panel) {
       if (!last) {
         HQ_RET(block_sketch(h, r, j, n, b, Y));
@@ -1319,7 +1328,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       continue;
     }
     TrailArgs ta;
-    HQ_RET(trail_prepare(h, mrows, nc, b, Ajj, lda, T, nb, C, ldx[xc], &ta,
+    HQ_RET(trail_prepare(h, mrows, nc, b, Vj, ldvj, T, nb, C, ldx[xc], &ta,
                          gcols ? Gt + j : nullptr, m, gcols ? r : 0));
     HQ_RET(trail_vtc_only(h, ta));
     if (t_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_t, 0));
@@ -1332,7 +1341,7 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
 
     // jn < kmax implies !last and b == nb.
     const int xn = xc ^ 1;
-    double* Pn = h->Pn.as<double>();
+    double* Pn = h->Pn.as<double>() + (size_t)pn_par * nb * m;
     const int mnext = m - jn;
     const int b2 = block_width(jn);
     const bool last_next = (jn + nb >= m) || (jn + nb >= n);
@@ -1389,8 +1398,8 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
       sel.sel_C = C;
       sel.sel_ldc = ldx[xc];
       sel.sel_r0 = b;
-      sel.sel_V = Ajj + b;
-      sel.sel_ldv = lda;
+      sel.sel_V = Vj + b;
+      sel.sel_ldv = ldvj;
       sel.sel_W = ta.W;
       sel.sel_ldw = ta.ldw;
       sel.sel_b = b;
@@ -1416,18 +1425,33 @@ int factor_loop(hqrrp_handle_t h, int m, int n, int kmax, double* dA,
     // The Cholesky-QR panel (and its gated fallback) wrote T before
     // ev_chain; the per-step panel's T follows on s_chain (ev_t).
     t_pending = h->panel_alg != 0;
-    // Join: block jn's factored panel over its gathered columns and the
-    // rows above it permuted, in one launch.
-    {
+    // Block jn's factored panel over its gathered columns and the rows
+    // above it permuted, in one launch on s_upd (after the gathers, which
+    // move the rows above, and the panel): beside block jn's V'C, which
+    // reads the reflectors from Pn.
+    h->cur = h->s_upd;
+    rc = hipStreamWaitEvent(h->s_upd, h->ev_chain, 0) == hipSuccess
+             ? HQRRP_OK
+             : HQRRP_ERR_HIP;
+    if (rc == HQRRP_OK) {
       PhaseScope ps(h, PH_PANEL_POST, 0.0,
                     16.0 * (double)mnext * b2 + 16.0 * jn * b2, 1);
-      HQ_TRY(launch_panel_permute(dA + (long)jn * lda, lda, jn, b2,
-                                  h->perm.as<int>(), djpvt + jn, h->cur, Pn,
-                                  mnext, mnext));
+      if (launch_panel_permute(dA + (long)jn * lda, lda, jn, b2,
+                               h->perm.as<int>(), djpvt + jn, h->cur, Pn,
+                               mnext, mnext) != hipSuccess ||
+          hipEventRecord(h->ev_perm, h->s_upd) != hipSuccess)
+        rc = HQRRP_ERR_HIP;
     }
+    h->cur = main_s;
+    HQ_RET(rc);
+    perm_pending = true;
+    Vp = Pn;
+    ldvp = mnext;
+    pn_par ^= 1;
     xc = xn;
     have_panel = true;
   }
+  if (perm_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_perm, 0));
   // A last T formed on s_chain but never consumed (no trailing columns)
   // still reads h->gram: order the caller's next work after it.
   if (t_pending) HQ_TRY(hipStreamWaitEvent(main_s, h->ev_t, 0));
@@ -1771,6 +1795,7 @@ extern "C" int hqrrp_create(hqrrp_handle_t* out, int device) {
       // cache writeback and invalidation at each fork and join).
       const unsigned f = hipEventDisableTiming | hipEventDisableSystemFence;
       HQ_TRY(hipEventCreateWithFlags(&h->ev_fork, f));
+      HQ_TRY(hipEventCreateWithFlags(&h->ev_perm, f));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_upd, f));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_chain, f));
       HQ_TRY(hipEventCreateWithFlags(&h->ev_sketch, f));
@@ -1804,7 +1829,7 @@ extern "C" int hqrrp_destroy(hqrrp_handle_t h) {
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {h->ev_fork, h->ev_upd, h->ev_chain, h->ev_sketch,
+  for (hipEvent_t e : {h->ev_fork, h->ev_perm, h->ev_upd, h->ev_chain, h->ev_sketch,
                        h->ev_t, h->ev_in, h->ev_out})
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t st : {h->s_chain, h->s_upd})
