@@ -853,11 +853,14 @@ __global__ __launch_bounds__(HQ_NT) void trail_sum_kernel(TrailArgs a,
 // reference's B = G1 - (G1 U11 + G2 U21) T U11' (NoFLA_HQRRP_WY_blk_var4.c:
 // 646-686).  A second launch (trail_y2_kernel) then applies
 // Y2 -= B R12 (691-693).  Each product is 64 x 64 x 64 on the matrix cores
-// with both operands in LDS.
+// with both operands in LDS.  Eight wavefronts per workgroup, so that twice
+// as many partial loads are in flight as with four (the slices are still
+// added in slice order); each product's 16 x 16 tile is one wavefront's.
 namespace {
 constexpr int kTP = 65;  // LDS pitch of the 64-wide tiles
 constexpr int kTB = 81;  // LDS pitch of B' (r <= 80 rows)
-__global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
+constexpr int kTT = 512;  // threads of a tail workgroup
+__global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
   __shared__ double Ts[64][kTP];   // T(p, q)
   __shared__ double Ms[64][kTP];   // M(p, c)
   __shared__ double V1s[64][kTP];  // V1(i, p)
@@ -886,7 +889,10 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   }
   const long slice = (long)b * a.ncv;
   const int S = a.sk ? a.vp.count(a.vp.group(v0)) : a.S;  // partial slices
-  const int cw = wave * 16 + l15;  // this lane's column of the tile
+  // Wavefront w: columns 16 (w % 4) .. + 15 of the tile and row tiles
+  // qb, qb + 1 (qb = 2 (w / 4)) of both products.
+  const int cw = (wave & 3) * 16 + l15;  // this lane's column of the tile
+  const int qb = 2 * (wave >> 2);
   const bool cok = c0 + cw < nc;
   const long gc = c0 + cw;
   // Every load up front (T, V1, the top rows for R12, then the partial
@@ -896,8 +902,8 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   // interior tile with all 64 reflectors (fast) issues every load without a
   // predicate.
   const bool fast = b == 64 && c0 + 64 <= nc;
-  constexpr int kPer = 64 * 64 / HQ_NT;
-  hq_d4 top[4];
+  constexpr int kPer = 64 * 64 / kTT;
+  hq_d4 top[2];
   double msum[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) msum[u] = 0.0;
@@ -906,15 +912,15 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
     double tv[kPer], vv[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
       tv[u] = a.T[p + (long)c * a.ldt];
       vv[u] = a.V[p + (long)c * a.ldv];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        top[q][e] = Cb[q * 16 + l4 + 4 * e + gc * ldc];
+        top[q][e] = Cb[(qb + q) * 16 + l4 + 4 * e + gc * ldc];
     const int S4 = S & ~3;
     for (int q0 = 0; q0 < S4; q0 += 4) {
       double x[4][kPer];
@@ -922,7 +928,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
       for (int k = 0; k < 4; ++k) {
         const double* src = a.part + (q0 + k) * slice + off0;
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) x[k][u] = src[(long)u * (HQ_NT / 64) * b];
+        for (int u = 0; u < kPer; ++u) x[k][u] = src[(long)u * (kTT / 64) * b];
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k)
@@ -933,28 +939,28 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
       const double* src = a.part + q * slice + off0;
       double x[kPer];
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) x[u] = src[(long)u * (HQ_NT / 64) * b];
+      for (int u = 0; u < kPer; ++u) x[u] = src[(long)u * (kTT / 64) * b];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) msum[u] += x[u];
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
       Ts[p][c] = tv[u];
       V1s[p][c] = vv[u];
     }
   } else {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int idx = t + u * HQ_NT, p = idx & 63, c = idx >> 6;
+      const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
       Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
       V1s[p][c] = (p < b && c < b) ? a.V[p + (long)c * a.ldv] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = q * 16 + l4 + 4 * e;
+        const int i = (qb + q) * 16 + l4 + 4 * e;
         top[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
       }
     // The slices four at a time: 4 x kPer independent loads in flight, then
@@ -968,9 +974,9 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
         const double* src = a.part + (q0 + k) * slice + off0;
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-          const int c = (t >> 6) + u * (HQ_NT / 64);
+          const int c = (t >> 6) + u * (kTT / 64);
           x[k][u] = (sok && pok && c0 + c < nc)
-                        ? src[(long)u * (HQ_NT / 64) * b]
+                        ? src[(long)u * (kTT / 64) * b]
                         : 0.0;
         }
       }
@@ -982,29 +988,30 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int idx = t + u * HQ_NT;
+    const int idx = t + u * kTT;
     Ms[idx & 63][idx >> 6] = msum[u];
   }
   __syncthreads();
   // W = T' M: A = T' (q x p) from T(p, q), B = M.
   {
-    hq_d4 acc[4];
+    hq_d4 acc[2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < 2; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 4) {
       const double bf = Ms[kk + l4][cw];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 2; ++q) {
+        const int qr = (qb + q) * 16 + l15;
         acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            (kk + l4 <= q * 16 + l15) ? Ts[kk + l4][q * 16 + l15] : 0.0, bf,
-            acc[q], 0, 0, 0);
+            (kk + l4 <= qr) ? Ts[kk + l4][qr] : 0.0, bf, acc[q], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int qq = q * 16 + l4 + 4 * e;
+        const int qq = (qb + q) * 16 + l4 + 4 * e;
         Ws[qq][cw] = acc[q][e];
         if (cok && qq < b) a.W[v0 + cw + (long)qq * a.ldw] = acc[q][e];
       }
@@ -1013,23 +1020,24 @@ __global__ __launch_bounds__(HQ_NT) void trail_tail_kernel(TailArgs a) {
   // R12 (or B') = top rows - V1 W: A = -V1 (i x p), B = W; the top rows
   // straight into the accumulators.
   {
-    hq_d4 acc[4];
+    hq_d4 acc[2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = top[q];
+    for (int q = 0; q < 2; ++q) acc[q] = top[q];
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 4) {
       const double bf = Ws[kk + l4][cw];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 2; ++q) {
+        const int qr = (qb + q) * 16 + l15;
         acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            -vmask(V1s[q * 16 + l15][kk + l4], q * 16 + l15, kk + l4), bf,
-            acc[q], 0, 0, 0);
+            -vmask(V1s[qr][kk + l4], qr, kk + l4), bf, acc[q], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int i = q * 16 + l4 + 4 * e;
+        const int i = (qb + q) * 16 + l4 + 4 * e;
         if (cok && i < b) Rb[i + gc * ldr] = acc[q][e];
       }
   }
@@ -1139,7 +1147,7 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
 hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s) {
   if (a.b < 1 || a.b > 64 || a.nc < 1 || a.S < 1 || (a.C2 && a.nc2 > 80))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.ncv, 64)), dim3(HQ_NT),
+  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.ncv, 64)), dim3(kTT),
                      0, s, a);
   return hipGetLastError();
 }
