@@ -655,9 +655,9 @@ int run_sketch(hqrrp_handle_t h, int r, int ncols, const void* dY, long ldy,
 // The panel from two Gram passes (k_cqr.hip): Cholesky QR2, the reference's
 // QRCP of the b x b factor R, Householder reconstruction; T is written
 // directly.  A panel the kernels flag (nearly dependent columns, a column
-// with nothing below its diagonal) is factored by the per-step kernel and T
-// by larft in the same stream order, both launches gated on the flag, so
-// the host never waits.  sel: as in run_panel (the selected columns are
+// with nothing below its diagonal) is factored by the per-step kernel,
+// whose workgroup 0 then forms T as larft would: one launch, gated on the
+// flag, so the host never waits.  sel: as in run_panel (the selected columns are
 // formed and stored into dP by the first pass).
 int run_panel_cqr(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
                   int pivoting, double* dtau, int* dperm, double* dT, long ldt,
@@ -716,12 +716,13 @@ int run_panel_cqr(hqrrp_handle_t h, int mrows, int b, double* dP, long ldp,
   HQ_RET(xchg_begin(h, h->xp, xneed, &a.xcur, &a.xprev, &a.xprev_len));
   a.gram = h->gram.as<double>();
   a.err = h->err();
+  a.T = dT;
+  a.ldt = ldt;
   {
-    PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 2);
+    PhaseScope ps(h, PH_PANEL_POST, 0.0, 0.0, 1);
     HQ_TRY(launch_panel_qrcp(a, h->cur));
     xchg_end(h->xp, xneed);
     h->xp_after_cqr = true;
-    HQ_TRY(launch_larft(b, a.gram, b, dtau, dT, ldt, h->cur, c.flag));
   }
   return HQRRP_OK;
 }

@@ -44,6 +44,88 @@ constexpr int kXS = 2 * kMaxB;    // exchange slot of one workgroup: [0,64)
                                   // (owner only)
 constexpr int kChunk = 8;         // workgroups' partials awaited together
 
+// dlarft('Forward','Columnwise') from the reflector inner products
+// G(l,q) = v_l'v_q (l < q, gram[l + q ldg]): T is the compact-WY factor of
+// H_0 H_1 ... with T(q,q) = tau_q and, for adjacent column blocks B1, B2,
+//   T(B1,B2) = -T(B1,B1) G(B1,B2) T(B2,B2),
+// built by doubling the block size (1, 2, 4, ..., 32): six levels of small
+// triangular products instead of LAPACK's b-step recurrence (same matrix in
+// exact arithmetic; a zero tau gives a zero column, as LAPACK's).  Run by
+// every thread of one workgroup of NT threads; P: kMaxB x (kMaxB + 1)
+// doubles of LDS holding T's upper triangle (with the diagonal) and G's
+// strict upper part transposed below the diagonal.  A level's X(B1, B2) =
+// G(B1, B2) T(B2, B2) goes to T(B1, B2)'s place, and T(B1, B2) is formed in
+// registers and written after a barrier.  Every entry is the same sum in
+// the same order whatever NT is.
+template <int NT>
+__device__ __forceinline__ void larft_lds(int b, const double* gram, long ldg,
+                                          const double* tau, double* T,
+                                          long ldt, double (*P)[kMaxB + 1]) {
+  constexpr int kE = (kMaxB * kMaxB / 4 + NT - 1) / NT;  // a level's blocks
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kMaxB * kMaxB / NT; ++u) {
+    const int idx = t + u * NT, i = idx & (kMaxB - 1), j = idx >> 6;
+    const bool in = i < b && j < b;
+    P[i][j] = !in ? 0.0 : (i == j ? tau[i] : (i < j ? 0.0 : gram[j + i * ldg]));
+  }
+  __syncthreads();
+  // Level s: for the pairs of column blocks B1 = [c0, c0+s), B2 = [c0+s,
+  // c0+2s), element (r, cc) of the s x s block (row c0+r, column c0+s+cc).
+  // s is a compile-time power of two after unrolling, so the index split is
+  // shifts and masks.
+#pragma unroll
+  for (int lv = 0; lv < 6; ++lv) {
+    const int s2 = 1 << lv;
+    if (s2 >= b) break;
+    const int per = s2 * s2, npairs = (b + 2 * s2 - 1) / (2 * s2);
+    // X(B1, B2) = G(B1, B2) T(B2, B2).
+    for (int e = t; e < npairs * per; e += NT) {
+      const int pr = e >> (2 * lv), rem = e & (per - 1);
+      const int r = rem >> lv, cc = rem & (s2 - 1);
+      const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
+      if (col >= b) continue;
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < s2; ++u)  // u <= cc: T(B2, B2) is upper
+        acc += (u <= cc) ? P[c1 + u][c0 + r] * P[c1 + u][col] : 0.0;
+      P[c0 + r][col] = acc;
+    }
+    __syncthreads();
+    // T(B1, B2) = -T(B1, B1) X(B1, B2).
+    double nv[kE];
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      const int e = t + q * NT;
+      const int pr = e >> (2 * lv), rem = e & (per - 1);
+      const int r = rem >> lv, cc = rem & (s2 - 1);
+      const int c0 = pr * 2 * s2, col = c0 + s2 + cc;
+      double acc = 0.0;
+      if (e < npairs * per && col < b) {
+#pragma unroll
+        for (int u = 0; u < s2; ++u)  // u >= r: T(B1, B1) is upper
+          acc += (u >= r) ? P[c0 + r][c0 + u] * P[c0 + u][col] : 0.0;
+      }
+      nv[q] = -acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      const int e = t + q * NT;
+      const int pr = e >> (2 * lv), rem = e & (per - 1);
+      const int r = rem >> lv, cc = rem & (s2 - 1);
+      const int c0 = pr * 2 * s2, col = c0 + s2 + cc;
+      if (e < npairs * per && col < b) P[c0 + r][col] = nv[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kMaxB * kMaxB / NT; ++u) {
+    const int idx = t + u * NT, i = idx & (kMaxB - 1), j = idx >> 6;
+    if (i < b && j < b) T[i + (long)j * ldt] = (i <= j) ? P[i][j] : 0.0;
+  }
+}
+
 // Thread (c, g) = (t % 64, t / 64) holds column c of the 32 consecutive rows
 // row0 + 32g + i, i < 32, in registers x[i]; rows beyond mrows are zero.
 // Relative to the current step k, a wavefront's rows are either all above
@@ -390,75 +472,23 @@ __global__ __launch_bounds__(kPT) void panel_qrcp_kernel(PanelArgs a) {
   }
   if (w == 0 && g == 0 && col_ok) a.perm[pos] = cc;
   hq_xreset(a.xprev, a.xprev_len, w, nw, kPT);
+  // T from this workgroup's gram and tau (written above; the barrier orders
+  // them before they are read back), in `stage`, free again.
+  if (a.T && w == 0)
+    larft_lds<kPT>(b, a.gram, b, a.tau, a.T, a.ldt,
+                   reinterpret_cast<double (*)[kMaxB + 1]>(&stage[0][0]));
 }
 
-// dlarft('Forward','Columnwise') from the reflector inner products
-// G(l,q) = v_l'v_q (l < q): T is the compact-WY factor of H_0 H_1 ... with
-// T(q,q) = tau_q and, for adjacent column blocks B1, B2,
-//   T(B1,B2) = -T(B1,B1) G(B1,B2) T(B2,B2),
-// built by doubling the block size (1, 2, 4, ..., 32): six levels of small
-// triangular products instead of LAPACK's b-step recurrence (same matrix in
-// exact arithmetic; a zero tau gives a zero column, as LAPACK's).
+// dlarft('Forward','Columnwise'): T of the b reflectors, formed by one
+// workgroup (larft_lds); see larft_lds for the method.
 __global__ __launch_bounds__(HQ_NT) void larft_kernel(int b, const double* gram,
                                                       long ldg,
                                                       const double* tau,
                                                       double* T, long ldt,
                                                       const int* run_if) {
   if (run_if && *run_if == 0) return;
-  __shared__ double sT[kMaxB][kMaxB + 1];
-  __shared__ double sG[kMaxB][kMaxB + 1];
-  __shared__ double sX[kMaxB][kMaxB + 1];
-  const int t = threadIdx.x;
-  // sT: tau on the diagonal, zero elsewhere; sG: the strictly upper gram.
-#pragma unroll
-  for (int u = 0; u < kMaxB * kMaxB / HQ_NT; ++u) {
-    const int idx = t + u * HQ_NT, i = idx & (kMaxB - 1), j = idx >> 6;
-    const bool in = i < b && j < b;
-    sG[i][j] = (in && i < j) ? gram[i + j * ldg] : 0.0;
-    sT[i][j] = (in && i == j) ? tau[i] : 0.0;
-  }
-  __syncthreads();
-  // Level s: for the pairs of column blocks B1 = [c0, c0+s), B2 = [c0+s,
-  // c0+2s), element (r, cc) of the s x s block (row c0+r, column c0+s+cc).
-  // s is a compile-time power of two after unrolling, so the index split is
-  // shifts and masks.
-#pragma unroll
-  for (int lv = 0; lv < 6; ++lv) {
-    const int s2 = 1 << lv;
-    if (s2 >= b) break;
-    const int per = s2 * s2, npairs = (b + 2 * s2 - 1) / (2 * s2);
-    // X(B1, B2) = G(B1, B2) T(B2, B2).
-    for (int e = t; e < npairs * per; e += HQ_NT) {
-      const int pr = e >> (2 * lv), rem = e & (per - 1);
-      const int r = rem >> lv, cc = rem & (s2 - 1);
-      const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
-      if (col >= b) continue;
-      double acc = 0.0;
-#pragma unroll
-      for (int u = 0; u < s2; ++u)  // u <= cc: T(B2, B2) is upper
-        acc += (u <= cc) ? sG[c0 + r][c1 + u] * sT[c1 + u][col] : 0.0;
-      sX[c0 + r][col] = acc;
-    }
-    __syncthreads();
-    // T(B1, B2) = -T(B1, B1) X(B1, B2).
-    for (int e = t; e < npairs * per; e += HQ_NT) {
-      const int pr = e >> (2 * lv), rem = e & (per - 1);
-      const int r = rem >> lv, cc = rem & (s2 - 1);
-      const int c0 = pr * 2 * s2, c1 = c0 + s2, col = c1 + cc;
-      if (col >= b) continue;
-      double acc = 0.0;
-#pragma unroll
-      for (int u = 0; u < s2; ++u)  // u >= r: T(B1, B1) is upper
-        acc += (u >= r) ? sT[c0 + r][c0 + u] * sX[c0 + u][col] : 0.0;
-      sT[c0 + r][col] = -acc;
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int u = 0; u < kMaxB * kMaxB / HQ_NT; ++u) {
-    const int idx = t + u * HQ_NT, i = idx & (kMaxB - 1), j = idx >> 6;
-    if (i < b && j < b) T[i + (long)j * ldt] = (i <= j) ? sT[i][j] : 0.0;
-  }
+  __shared__ double sP[kMaxB][kMaxB + 1];
+  larft_lds<HQ_NT>(b, gram, ldg, tau, T, ldt, sP);
 }
 
 // Applies the panel's column permutation to rows [0, nrows) of the b panel

@@ -49,6 +49,10 @@ struct PanelArgs {
   // Run only when null or *run_if != 0 (the fallback of the Cholesky-QR
   // panel); otherwise the launch only restores its exchange sentinel.
   const int* run_if;
+  // Non-null: workgroup 0 also forms T (b x b, ld ldt) from gram and tau at
+  // the end, as larft would (null: T is left to a larft launch).
+  double* T;
+  long ldt;
 };
 
 // k_cqr.hip: panel QRCP from two Gram passes (pivoted Cholesky QR with the
