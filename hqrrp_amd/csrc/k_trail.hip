@@ -843,8 +843,9 @@ __global__ __launch_bounds__(HQ_NT) void trail_sum_kernel(TrailArgs a,
 
 }  // namespace
 
-// The block tail of the HQRRP driver, per 64 virtual columns (one
-// workgroup each), after the split-K partials of M = V'[C C2]:
+// The block tail of the HQRRP driver, per 32 virtual columns (two
+// workgroups for each 64-column group of the partials, so that their loads
+// spread over twice as many CUs), after the split-K partials of M = V'[C C2]:
 //   W   = T' sum_s part[s]       (written as W' for the trailing update)
 //   R12 = C(0:b, :) - V1 W        (V1: the top b x b block of V, unit lower;
 //                                  R12 is row block j of R, written to R)
@@ -852,23 +853,29 @@ __global__ __launch_bounds__(HQ_NT) void trail_sum_kernel(TrailArgs a,
 // Q' = T' P' and B' = G'(0:b, :) - V1 Q', written over G'(0:b, :): the
 // reference's B = G1 - (G1 U11 + G2 U21) T U11' (NoFLA_HQRRP_WY_blk_var4.c:
 // 646-686).  A second launch (trail_y2_kernel) then applies
-// Y2 -= B R12 (691-693).  Each product is 64 x 64 x 64 on the matrix cores
-// with both operands in LDS.  Eight wavefronts per workgroup, so that twice
-// as many partial loads are in flight as with four (the slices are still
-// added in slice order); each product's 16 x 16 tile is one wavefront's.
+// Y2 -= B R12 (691-693).  Each product is 64 x 64 x 32 on the matrix cores
+// with both operands in LDS (67 KB: two workgroups per CU).  Eight
+// wavefronts per workgroup, each one 16 x 16 tile of each product; the
+// slices are added in slice order and every tile's products in the same
+// order as for any other split, so the results do not depend on it.
 namespace {
 constexpr int kTP = 65;  // LDS pitch of the 64-wide tiles
 constexpr int kTB = 81;  // LDS pitch of B' (r <= 80 rows)
 constexpr int kTT = 512;  // threads of a tail workgroup
+constexpr int kTH = 33;   // LDS pitch of the 32-wide tiles
 __global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
-  __shared__ double Ts[64][kTP];   // T(p, q)
-  __shared__ double Ms[64][kTP];   // M(p, c)
-  __shared__ double V1s[64][kTP];  // V1(i, p)
-  __shared__ double Ws[64][kTP];   // W(q, c)
+  // T and V1 share one array: T(p, q) on and above the diagonal, V1(i, p)
+  // below it (its unit diagonal is implied where it is read).
+  __shared__ double TVs[64][kTP];
+  __shared__ double Ms[64][kTH];  // M(p, c)
+  __shared__ double Ws[64][kTH];  // W(q, c)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l15 = lane & 15, l4 = lane >> 4;
   const int b = a.b;
-  const long v0 = (long)blockIdx.x * 64;  // first virtual column
+  // Two workgroups per 64 virtual columns (one group of the partials), each
+  // 32 of them.
+  const long v0 = (long)(blockIdx.x >> 1) * 64;  // the group's first
+  const int hc = 32 * (blockIdx.x & 1);          // this half's offset
   long c0, nc, ldc, ldr;
   const double* Cb;
   double* Rb;
@@ -887,40 +894,37 @@ __global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
   } else {
     return;
   }
+  const long h0 = c0 + hc;  // first column of this workgroup
   const long slice = (long)b * a.ncv;
   const int S = a.sk ? a.vp.count(a.vp.group(v0)) : a.S;  // partial slices
-  // Wavefront w: columns 16 (w % 4) .. + 15 of the tile and row tiles
-  // qb, qb + 1 (qb = 2 (w / 4)) of both products.
-  const int cw = (wave & 3) * 16 + l15;  // this lane's column of the tile
-  const int qb = 2 * (wave >> 2);
-  const bool cok = c0 + cw < nc;
-  const long gc = c0 + cw;
+  // Wavefront w: columns 16 (w % 2) .. + 15 of the half and row tile w / 2
+  // of both products.
+  const int cw = (wave & 1) * 16 + l15;  // this lane's column of the half
+  const int qt = wave >> 1;
+  const bool cok = h0 + cw < nc;
+  const long gc = h0 + cw;
   // Every load up front (T, V1, the top rows for R12, then the partial
   // slices summed in slice order, four slices' loads in flight at a time),
-  // then the LDS stores.  T and V1 go into LDS unmasked (T's upper triangle
-  // and V1's unit lower triangle are applied where they are read), so an
-  // interior tile with all 64 reflectors (fast) issues every load without a
-  // predicate.
-  const bool fast = b == 64 && c0 + 64 <= nc;
-  constexpr int kPer = 64 * 64 / kTT;
-  hq_d4 top[2];
+  // then the LDS stores.  An interior half with all 64 reflectors (fast)
+  // issues every load without a predicate.
+  const bool fast = b == 64 && h0 + 32 <= nc;
+  constexpr int kPer = 64 * 32 / kTT;  // partial entries per thread
+  constexpr int kPtv = 64 * 64 / kTT;  // T and V1 entries per thread
+  hq_d4 top;
   double msum[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) msum[u] = 0.0;
-  const long off0 = (long)(t & 63) + (v0 + (t >> 6)) * b;  // u = 0
+  const long off0 = (long)(t & 63) + (v0 + hc + (t >> 6)) * b;  // u = 0
   if (fast) {
-    double tv[kPer], vv[kPer];
+    double tv[kPtv], vv[kPtv];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPtv; ++u) {
       const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
       tv[u] = a.T[p + (long)c * a.ldt];
       vv[u] = a.V[p + (long)c * a.ldv];
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        top[q][e] = Cb[(qb + q) * 16 + l4 + 4 * e + gc * ldc];
+    for (int e = 0; e < 4; ++e) top[e] = Cb[qt * 16 + l4 + 4 * e + gc * ldc];
     const int S4 = S & ~3;
     for (int q0 = 0; q0 < S4; q0 += 4) {
       double x[4][kPer];
@@ -944,25 +948,22 @@ __global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
       for (int u = 0; u < kPer; ++u) msum[u] += x[u];
     }
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPtv; ++u) {
       const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
-      Ts[p][c] = tv[u];
-      V1s[p][c] = vv[u];
+      TVs[p][c] = (p <= c) ? tv[u] : vv[u];
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPtv; ++u) {
       const int idx = t + u * kTT, p = idx & 63, c = idx >> 6;
-      Ts[p][c] = (p <= c && c < b) ? a.T[p + (long)c * a.ldt] : 0.0;
-      V1s[p][c] = (p < b && c < b) ? a.V[p + (long)c * a.ldv] : 0.0;
+      TVs[p][c] = (p <= c) ? ((c < b) ? a.T[p + (long)c * a.ldt] : 0.0)
+                           : ((p < b && c < b) ? a.V[p + (long)c * a.ldv] : 0.0);
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = (qb + q) * 16 + l4 + 4 * e;
-        top[q][e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int i = qt * 16 + l4 + 4 * e;
+      top[e] = (cok && i < b) ? Cb[i + gc * ldc] : 0.0;
+    }
     // The slices four at a time: 4 x kPer independent loads in flight, then
     // added in slice order.
     const bool pok = (t & 63) < b;
@@ -975,7 +976,7 @@ __global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
           const int c = (t >> 6) + u * (kTT / 64);
-          x[k][u] = (sok && pok && c0 + c < nc)
+          x[k][u] = (sok && pok && h0 + c < nc)
                         ? src[(long)u * (kTT / 64) * b]
                         : 0.0;
         }
@@ -992,54 +993,37 @@ __global__ __launch_bounds__(kTT) void trail_tail_kernel(TailArgs a) {
     Ms[idx & 63][idx >> 6] = msum[u];
   }
   __syncthreads();
+  const int qr = qt * 16 + l15;
   // W = T' M: A = T' (q x p) from T(p, q), B = M.
   {
-    hq_d4 acc[2];
+    hq_d4 acc = hq_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 2; ++q) acc[q] = hq_d4{0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < 64; kk += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+          (kk + l4 <= qr) ? TVs[kk + l4][qr] : 0.0, Ms[kk + l4][cw], acc, 0,
+          0, 0);
 #pragma unroll
-    for (int kk = 0; kk < 64; kk += 4) {
-      const double bf = Ms[kk + l4][cw];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int qr = (qb + q) * 16 + l15;
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            (kk + l4 <= qr) ? Ts[kk + l4][qr] : 0.0, bf, acc[q], 0, 0, 0);
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int qq = qt * 16 + l4 + 4 * e;
+      Ws[qq][cw] = acc[e];
+      if (cok && qq < b) a.W[v0 + hc + cw + (long)qq * a.ldw] = acc[e];
     }
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int qq = (qb + q) * 16 + l4 + 4 * e;
-        Ws[qq][cw] = acc[q][e];
-        if (cok && qq < b) a.W[v0 + cw + (long)qq * a.ldw] = acc[q][e];
-      }
   }
   __syncthreads();  // W complete
   // R12 (or B') = top rows - V1 W: A = -V1 (i x p), B = W; the top rows
   // straight into the accumulators.
   {
-    hq_d4 acc[2];
+    hq_d4 acc = top;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) acc[q] = top[q];
+    for (int kk = 0; kk < 64; kk += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+          -vmask(TVs[qr][kk + l4], qr, kk + l4), Ws[kk + l4][cw], acc, 0, 0,
+          0);
 #pragma unroll
-    for (int kk = 0; kk < 64; kk += 4) {
-      const double bf = Ws[kk + l4][cw];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int qr = (qb + q) * 16 + l15;
-        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            -vmask(V1s[qr][kk + l4], qr, kk + l4), bf, acc[q], 0, 0, 0);
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int i = qt * 16 + l4 + 4 * e;
+      if (cok && i < b) Rb[i + gc * ldr] = acc[e];
     }
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = (qb + q) * 16 + l4 + 4 * e;
-        if (cok && i < b) Rb[i + gc * ldr] = acc[q][e];
-      }
   }
 }
 
@@ -1147,8 +1131,8 @@ __global__ __launch_bounds__(HQ_NT) void trail_y2_kernel(TailArgs a) {
 hipError_t launch_trail_tail(const TailArgs& a, hipStream_t s) {
   if (a.b < 1 || a.b > 64 || a.nc < 1 || a.S < 1 || (a.C2 && a.nc2 > 80))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trail_tail_kernel, dim3(hq_cdiv(a.ncv, 64)), dim3(kTT),
-                     0, s, a);
+  hipLaunchKernelGGL(trail_tail_kernel, dim3(2 * hq_cdiv(a.ncv, 64)),
+                     dim3(kTT), 0, s, a);
   return hipGetLastError();
 }
 
