@@ -815,9 +815,11 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   a.ldt = ldt;
   // V'C: the stream-K partition over all wavefronts of one workgroup per
   // CU when every wavefront gets >= 256 rows of a 64-column group; else
-  // 64-column tiles (two workgroups per CU) with K split uniformly so that
-  // the grid fills but does not exceed those slots (a second, partial round
-  // of workgroups would double the time).
+  // 64-column tiles (two workgroups per CU) with K split uniformly, in
+  // slices of at least 64 rows, so that the grid fills but does not exceed
+  // those slots (a second, partial round of workgroups would double the
+  // time; a slice walks its rows serially, so few long slices leave a small
+  // product latency-bound).
   const long nch = hq_cdiv(mrows, kVtcChunk);
   const long ngroups = hq_cdiv(a.nc, 64) + hq_cdiv(a.nc2, 64);
   const long waves = 4L * h->cu_count;
@@ -832,7 +834,7 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
     S = 1;
     for (long g = 0; g < ngroups; ++g) S = std::max(S, (long)vp.count(g));
   } else {
-    const long maxs = std::max(1L, (long)mrows / 256);
+    const long maxs = std::max(1L, (long)mrows / 64);
     const long tiles = hq_cdiv(nc, 64);
     const long slots = 2L * h->cu_count;
     const int kbk = trail_kbk();
