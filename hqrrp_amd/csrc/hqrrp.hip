@@ -814,7 +814,7 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   a.T = T;
   a.ldt = ldt;
   // V'C: the stream-K partition over all wavefronts of one workgroup per
-  // CU when every wavefront gets >= 256 rows of a 64-column group; else
+  // CU when every wavefront gets >= 128 rows of a 64-column group; else
   // 64-column tiles (two workgroups per CU) with K split uniformly, in
   // slices of at least 64 rows, so that the grid fills but does not exceed
   // those slots (a second, partial round of workgroups would double the
@@ -824,7 +824,7 @@ int trail_prepare(hqrrp_handle_t h, int mrows, int nc, int b, const double* V,
   const long ngroups = hq_cdiv(a.nc, 64) + hq_cdiv(a.nc2, 64);
   const long waves = 4L * h->cu_count;
   long S;
-  if (nch * ngroups >= 16 * waves) {
+  if (nch * ngroups >= 8 * waves) {
     a.sk = 1;
     a.sk_waves = (int)waves;
     a.sk_nch = nch;
