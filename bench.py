@@ -58,7 +58,7 @@ PEAK_HBM = 8.0e12         # MI355X HBM3E, B/s
 # the KiB summed over the phase's kernels of all 255 blocks, per block.
 # Keyed by (phase, m, n, nb); other shapes report null.
 MEASURED_TRAFFIC = {("trail_update", 16384, 16384, 64):
-                    round((106360651.3125 * 2.0 + 179232136.875) * 1024 / 255)}
+                    round((106359050.4375 * 2.0 + 179231455.125) * 1024 / 255)}
 
 # Phase -> (bound, kernel symbol prefix in rocprof)
 PHASE_BOUND = {
